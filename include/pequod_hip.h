@@ -1,0 +1,215 @@
+/*
+ * pequod_hip.h -- C ABI of the MI355X implementation of pequod's time-step
+ * hot path: the modal Helmholtz (PV) inversion and the CABARET advection
+ * step.  Library: libpequod_hip.so (built from pequod_amd/csrc).
+ *
+ * Conventions
+ *   - Every entry point returns an int status: PQ_OK (0) or a PQ_ERR_* code;
+ *     pq_last_error() returns a message for the calling thread.  The library
+ *     never exits the process; the Fortran shim turns a nonzero status into
+ *     the reference's labort (debug.h:10, qg3p_base.F90:383-403).
+ *   - Arrays use the reference's Fortran layout: column-major, i fastest.
+ *     A "g x g view" (ptr, ld1, ld2) addresses element (i, j, l) (1-based) at
+ *     ptr[(i-1) + ld1*((j-1) + ld2*(l-1))], so the sdata slice
+ *     data(2:G+1, 2:G+1, :) of a data(G+2, G+2, L) array is passed as
+ *     (&data(2,2,1), G+2, G+2) without a copy (qg3p_base.F90:1322-1323).
+ *   - Entry points whose names end in _dev take device pointers and enqueue
+ *     work on the handle's stream without synchronising.  Entry points
+ *     without the suffix take host pointers: if the pointer was registered
+ *     with pq_mirror_register the device mirror is used in place (no PCIe
+ *     traffic); otherwise the data is copied in and out and the call is
+ *     synchronous (the plain drop-in).
+ *   - Streams are passed as void* (a hipStream_t); NULL selects the handle's
+ *     own stream.
+ *   - Handles are independent and re-entrant: each owns its device buffers,
+ *     plans and stream (the reference's caches and CABARET work arrays are
+ *     `save` variables and not re-entrant, qg3p_aux.F90:1084,1185,1256).
+ */
+#ifndef PEQUOD_HIP_H
+#define PEQUOD_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQ_OK 0
+#define PQ_ERR_ARG 1      /* invalid argument or unsupported size           */
+#define PQ_ERR_HIP 2      /* HIP runtime failure                            */
+#define PQ_ERR_NAN 3      /* NaN psiC (qg3p_aux.F90:504-509 labort)         */
+#define PQ_WARN_LARGE 4   /* |psiC| > 1e100 (qg3p_aux.F90:510-517 warning)  */
+#define PQ_ERR_NOMEM 5    /* device memory exhausted                        */
+
+/* Arithmetic of the sine transforms.  Modal projection coefficients, the
+ * eigendecomposition, psiC and the mass correction constants are FP64 in
+ * every mode.                                                              */
+#define PQ_PREC_FP64 0    /* FP64 storage and transforms (plain drop-in)   */
+#define PQ_PREC_FP32 1    /* FP32 storage and FP32 transforms              */
+#define PQ_PREC_MIXED 2   /* FP64 storage, FP32 transforms                 */
+
+const char *pq_last_error(void);
+/* Library/runtime information: device name and CU count into buf.         */
+int pq_device_info(char *buf, int len);
+
+/* ---- device memory helpers (for callers without their own allocator) -- */
+int pq_malloc(void **dev_ptr, size_t bytes);
+int pq_free(void *dev_ptr);
+int pq_memcpy_h2d(void *dev, const void *host, size_t bytes);
+int pq_memcpy_d2h(void *host, const void *dev, size_t bytes);
+int pq_synchronize(void);
+
+/* ---- device mirrors of host arrays (keeps Fortran state resident) ------ */
+/* Registers [host, host+bytes) and allocates a device copy (uploaded now). */
+int pq_mirror_register(void *host, size_t bytes);
+int pq_mirror_unregister(void *host);
+/* Device address mirroring host pointer p (anywhere inside a registered
+ * range), or NULL.                                                        */
+void *pq_mirror_device_ptr(const void *host);
+int pq_mirror_push(const void *host); /* host -> device for the whole range */
+int pq_mirror_pull(void *host);       /* device -> host for the whole range */
+
+/* ======================================================================
+ * Helmholtz solver: replaces qg_helmholtz's allocate(helm_c, decomp, ss),
+ * helm(helm_c, zeta) and deallocate(helm_c)
+ * (qg3p_helmholtz.F90:292-295, 614-769, 389; the seam is the renamed import
+ * at qg3p_aux.F90:13-17).  Solves (Delta_5 - ss_l) psi_l = zeta_l in grid
+ * units with homogeneous Dirichlet values, per layer, in place, with the
+ * reference's output boundary semantics (column i=1 and row j=jj keep their
+ * input, column i=ii and row j=1 are zeroed on the interior).
+ * ii = jj = G = n + 2 with n + 1 = 2^a (the reference's sizes) or 3 * 2^a.
+ * ==================================================================== */
+typedef struct pq_helm pq_helm;
+int pq_helm_create(pq_helm **h, int ii, int jj, int layers, const double *ss,
+                   int precision);
+int pq_helm_destroy(pq_helm *h);
+int pq_helm_solve(pq_helm *h, double *zeta, int ld1, int ld2);
+/* zeta is a device g x g x layers view of FP64 (or FP32 for PQ_PREC_FP32). */
+int pq_helm_solve_dev(pq_helm *h, void *zeta, long ld1, long ld2, void *stream);
+
+/* ======================================================================
+ * PV inversion: replaces allocate_helmholtz_cache(hcache, decomp, strat,
+ * vbc) and pv_inversion(strat, vbc, zeta, hcache, psi) for the closed basin
+ * (qg3p_aux.F90:152-193, 452-479): z_to_modes, helm per mode, Dirichlet
+ * boundary, mean psiC and the mass-conserving correction (modes_to_z_btcu,
+ * qg3p_diagnostics.F90:615-635).
+ * z_to_modes and modes_to_z are L x L column-major as in the reference's
+ * stratification type (qg3p_diagnostics.F90:25-47); modes_to_z(:,1) must
+ * be the barotropic mode (all ones) with ss(1) = 0.
+ * A handle serves up to batch_max members of the same G and L in one call.
+ * ==================================================================== */
+typedef struct pq_inv pq_inv;
+int pq_inv_create(pq_inv **h, int g, int layers, const double *ss,
+                  const double *modes_to_z, const double *z_to_modes,
+                  int precision, int batch_max);
+int pq_inv_destroy(pq_inv *h);
+/* Host drop-in: zeta/psi are g x g x L views (FP64).  Synchronous unless
+ * both arrays are registered mirrors.  Returns PQ_ERR_NAN / PQ_WARN_LARGE
+ * like the reference's checks.                                           */
+int pq_inv_run(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *psi,
+               int ldp1, int ldp2);
+/* Device batch: zeta[b] / psi[b] are device g x g x L views (FP64, or FP32
+ * for PQ_PREC_FP32), b < batch.  Enqueued on stream; no synchronisation.  */
+int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1,
+                   long ldz2, void *const *psi, long ldp1, long ldp2,
+                   void *stream);
+/* Waits for the handle's last run and returns its status (NaN / large
+ * psiC); psic (may be NULL) receives batch*L values of the last run.       */
+int pq_inv_status(pq_inv *h, double *psic);
+/* The handle's stream (hipStream_t). */
+void *pq_inv_stream(pq_inv *h);
+
+/* ======================================================================
+ * CABARET stages on device arrays (qg3p_aux.F90:917-1642 and the boundary
+ * routines at 643-915).  Array shapes as in the reference on one domain:
+ * node fields (G+2, G+2, L); i-faces (G+3, G+2, L); j-faces (G+2, G+3, L);
+ * res / res_beta (G, G, L); force_wind (G, G).  All FP64.  These are
+ * asynchronous on `stream`.
+ * ==================================================================== */
+/* cabaret_cfl (qg3p_aux.F90:920-943): writes max|u| * dt to *cfl_host
+ * (synchronous).                                                         */
+int pq_cabaret_cfl_dev(int g, int layers, double dt, const double *ui,
+                       const double *uj, double *cfl_host, void *stream);
+int pq_cabaret_predictor_dev(int g, int layers, double dt,
+                             const double *zeta_old, const double *res,
+                             double *zeta, void *stream);
+int pq_cabaret_beta_dev(int g, int layers, double dt, double tweight_a,
+                        const double *beta, const double *ui, double *zeta,
+                        double *res_beta, void *stream);
+int pq_cabaret_forcing_dev(int g, int layers, double dt,
+                           const double *force_wind, double viscosity,
+                           double visc_bot, const double *rel, double *zeta,
+                           void *stream);
+int pq_cabaret_cell_boundaries_dev(int g, int layers, double tweight_u,
+                                   const double *psi, double *ui, double *uj,
+                                   double *uimo, double *ujmo, void *stream);
+int pq_cabaret_cell_boundaries_velocity_dev(int g, int layers,
+                                            const double *psi, double *ui,
+                                            double *uj, void *stream);
+/* zeta_fi / zeta_fj are updated in place; scratch must hold one i-face and
+ * one j-face array ((G+3)(G+2)L + (G+2)(G+3)L doubles) or be NULL (the
+ * library then uses a per-size internal buffer).                          */
+int pq_cabaret_extrapolator_dev(int g, int layers, double dt,
+                                const double *zeta, const double *zeta_old,
+                                const double *ui, const double *uj,
+                                double *zeta_fi, double *zeta_fj,
+                                double *scratch, void *stream);
+int pq_cabaret_flux_dev(int g, int layers, const double *ui, const double *uj,
+                        const double *zeta_fi, const double *zeta_fj,
+                        double *flux_i, double *flux_j, void *stream);
+int pq_cabaret_corrector_dev(int g, int layers, double dt,
+                             const double *flux_i, const double *flux_j,
+                             double *res, double *zeta, void *stream);
+/* rel_from_zeta_and_psi + apply_rel_bc (qg3p_aux.F90:738-915);
+ * bc_type 1 partial slip (alpha), 2 no slip, 3 free slip.                 */
+int pq_rel_from_zeta_and_psi_dev(int g, int layers, const double *s1,
+                                 const double *s2, int bc_type, double alpha,
+                                 const double *zeta, const double *psi,
+                                 double *rel, void *stream);
+/* update_zeta_bc (qg3p_aux.F90:643-734). */
+int pq_update_zeta_bc_dev(int g, int layers, const double *s1,
+                          const double *s2, const double *rel,
+                          const double *psi, double *zeta, void *stream);
+
+/* ======================================================================
+ * Device-resident model: one CABARET time step in the driver's order
+ * (qg3.F90:590-967), with the PV inversion, for state that stays in HBM.
+ * ==================================================================== */
+typedef struct pq_model pq_model;
+typedef struct {
+  int g, layers;
+  const double *s1, *s2;   /* [L] stratification (non-dimensional)        */
+  const double *ss, *modes_to_z, *z_to_modes; /* from the host eigensolve */
+  const double *beta;      /* [L] non-dimensional beta                     */
+  const double *force_wind;/* (G, G), already scaled (qg3.F90:253-279)     */
+  double viscosity, visc_bot;
+  int bc_type;
+  double alpha;
+  int adapt_dt;
+  double target_cfl, max_dt, max_dt_increase;
+  double dt;               /* initial (non-dimensional) time step          */
+  int precision;           /* PQ_PREC_* of the inversion                   */
+} pq_model_params;
+
+int pq_model_create(pq_model **m, const pq_model_params *p);
+int pq_model_destroy(pq_model *m);
+/* Start like START_PERTURB (qg3.F90:393-425): zeta(G,G,L) host interior
+ * field is installed, boundary set to zero, inverted, rel/zeta_bc applied,
+ * CABARET work arrays zeroed, zeta_old = zeta, model_time = dt/2.         */
+int pq_model_init(pq_model *m, const double *zeta_sdata_host);
+/* Run nsteps full steps; if final_partial, the last step stops after the
+ * flux stage (the driver's exit point, qg3.F90:815-946).                  */
+int pq_model_step(pq_model *m, int nsteps, int final_partial);
+/* Copy a field to the host: which = 0 psi, 1 zeta, 2 rel, 3 zeta_old,
+ * 4 ui, 5 uj, 6 zeta_fi, 7 zeta_fj, 8 res, 9 res_beta, 10 uimo, 11 ujmo,
+ * 12 flux_i, 13 flux_j.  Sizes follow the reference shapes.               */
+int pq_model_get(pq_model *m, int which, double *host, size_t count);
+int pq_model_set(pq_model *m, int which, const double *host, size_t count);
+/* dt, tweight_a, tweight_u, model_time */
+int pq_model_scalars(pq_model *m, double *out4);
+void *pq_model_stream(pq_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

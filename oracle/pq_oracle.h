@@ -1,0 +1,146 @@
+/*
+ * pq_oracle.h -- CPU FP64 restatement of pequod's time-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path
+ * is compared against (tests/, __graft_entry__.smoke(), and the
+ * `cpu_baseline` leg of bench.py).  Nothing in pequod_amd/ links, loads or
+ * calls it; the product path fails loudly when its HIP library is missing.
+ *
+ * Layout conventions are those of the Fortran reference (column-major, i is
+ * the fastest index, 1-based in the comments below):
+ *   node field  data(G+2, G+2, L)  (qg3p_base.F90:1310-1325); the owned nodes
+ *               are data(2:G+1, 2:G+1, :) ("sdata"); G = n + 2 nodes per side.
+ *   i-faces     data(G+3, G+2, L)  (FLUX_II, qg3p_base.F90:2808-2810)
+ *   j-faces     data(G+2, G+3, L)  (FLUX_JJ, qg3p_base.F90:2811-2813)
+ *   residuals   res(G, G, L)        (qg3.F90:80)
+ * All routines describe a single, non-periodic domain (p_a = p_b = p_c =
+ * p_d = -1), for which the reference's halo exchanges are no-ops.
+ */
+#ifndef PQ_ORACLE_H
+#define PQ_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- sine transform (qg3p_helmholtz.F90:168-183, 1370-1591) ---- */
+typedef struct pqo_sint pqo_sint;
+pqo_sint *pqo_sint_create(int n);
+void pqo_sint_destroy(pqo_sint *p);
+/* Unnormalised DST-I in place on x[0..n-1]:
+ *   X_k = 2 sum_{j=1..n} x_j sin(pi j k / (n+1)).                            */
+void pqo_sint_apply(pqo_sint *p, double *x);
+
+/* ---- modal Helmholtz solver (qg3p_helmholtz.F90:292-386, 614-769) ---- */
+typedef struct pqo_helm pqo_helm;
+pqo_helm *pqo_helm_create(int g, int layers, const double *ss);
+void pqo_helm_destroy(pqo_helm *h);
+/* In place on zeta(ld1, ld2, layers); uses zeta(1:g, 1:g, :).             */
+void pqo_helm_solve(pqo_helm *h, double *zeta, int ld1, int ld2);
+
+/* ---- boundary helpers (qg3p_laplace_bc.F90:55-87, 193-224) ---- */
+void pqo_apply_dirichlet_bc(int g, int layers, double bc, double *zeta,
+                            int ld1, int ld2);
+void pqo_add_laplace_bc_rhs(int g, int layers, double bc, double *rhs,
+                            int ld1, int ld2);
+/* Trapezoid mean of one g x g layer (qg3p_diagnostics.F90:669-720).       */
+double pqo_mean(int g, const double *psi, int ld1);
+
+/* ---- stratification (qg3p_diagnostics.F90:210-422) ---- */
+/* s1[l], s2[l] as in the reference's stratification type (s1(1) and s2(L)
+ * unused).  Outputs: ss[L], modes_to_z[L*L], z_to_modes[L*L] (column-major).
+ * Returns 0 on success.                                                    */
+int pqo_strat_modes(int layers, const double *s1, const double *s2,
+                    double *ss, double *modes_to_z, double *z_to_modes);
+void pqo_strat_l2(double rd, double h1, double h2, double *s1, double *s2);
+void pqo_strat_l3(double rd1, double rd2, double h1, double h2, double h3,
+                  double *s1, double *s21, double *s22, double *s3);
+
+/* ---- PV inversion (qg3p_aux.F90:152-230, 452-521) ---- */
+typedef struct pqo_inv pqo_inv;
+pqo_inv *pqo_inv_create(int g, int layers, const double *ss,
+                        const double *modes_to_z, const double *z_to_modes);
+void pqo_inv_destroy(pqo_inv *v);
+/* zeta and psi are g x g x L views with leading dimensions (ld1, ld2).
+ * Returns 0, 3 (NaN psiC) or 4 (|psiC| > 1e100, result still written).    */
+int pqo_inv_run(pqo_inv *v, const double *zeta, int ldz1, int ldz2,
+                double *psi, int ldp1, int ldp2);
+/* Correction field corr(g, g, L) and last psiC[L].                        */
+const double *pqo_inv_corr(const pqo_inv *v);
+const double *pqo_inv_psic(const pqo_inv *v);
+
+/* ---- CABARET stages (qg3p_aux.F90:643-1642) ---- */
+double pqo_cabaret_cfl(int g, int layers, double dt, const double *ui,
+                       const double *uj);
+void pqo_cabaret_adapt_dt(int g, int layers, double target_cfl, double max_dt,
+                          double max_dt_increase, const double *ui,
+                          const double *uj, double *dt, double *tweight_a,
+                          double *tweight_u);
+void pqo_cabaret_predictor(int g, int layers, double dt,
+                           const double *zeta_old, const double *res,
+                           double *zeta);
+void pqo_cabaret_beta(int g, int layers, double dt, double tweight_a,
+                      const double *beta, const double *ui, double *zeta,
+                      double *res_beta);
+void pqo_cabaret_forcing(int g, int layers, double dt,
+                         const double *force_wind, double viscosity,
+                         double visc_bot, const double *rel, double *zeta);
+void pqo_cabaret_cell_boundaries_velocity(int g, int layers, const double *psi,
+                                          double *ui, double *uj);
+void pqo_cabaret_cell_boundaries(int g, int layers, double tweight_u,
+                                 const double *psi, double *ui, double *uj,
+                                 double *uimo, double *ujmo);
+void pqo_cabaret_extrapolator(int g, int layers, double dt, const double *zeta,
+                              const double *zeta_old, const double *ui,
+                              const double *uj, double *zeta_fi,
+                              double *zeta_fj);
+void pqo_cabaret_flux(int g, int layers, const double *ui, const double *uj,
+                      const double *zeta_fi, const double *zeta_fj,
+                      double *flux_i, double *flux_j);
+void pqo_cabaret_corrector(int g, int layers, double dt, const double *flux_i,
+                           const double *flux_j, double *res, double *zeta);
+
+/* relative vorticity and boundary values (qg3p_aux.F90:643-915) */
+enum { PQO_BC_PARTIAL_SLIP = 1, PQO_BC_NO_SLIP = 2, PQO_BC_FREE_SLIP = 3 };
+void pqo_rel_from_zeta_and_psi(int g, int layers, const double *s1,
+                               const double *s2, int bc_type, double alpha,
+                               const double *zeta, const double *psi,
+                               double *rel);
+void pqo_apply_rel_bc(int g, int layers, int bc_type, double alpha,
+                      const double *psi, double *rel);
+void pqo_update_zeta_bc(int g, int layers, const double *s1, const double *s2,
+                        const double *rel, const double *psi, double *zeta);
+
+/* Wind stress curl field force_wind(g, g) (qg3.F90:253-279). */
+void pqo_wind_field(int g, double tau, double wind_shift, double wind_tilt,
+                    double wind_asym, double *force_wind);
+
+/* ---- one full CABARET time step in the driver's order (qg3.F90:590-967) */
+typedef struct {
+  int g, layers;
+  double *psi, *rel, *zeta, *zeta_old;  /* (g+2)(g+2)L */
+  double *zeta_fi, *ui, *uimo, *flux_i; /* (g+3)(g+2)L */
+  double *zeta_fj, *uj, *ujmo, *flux_j; /* (g+2)(g+3)L */
+  double *res, *res_beta;               /* g g L */
+  double *force_wind;                   /* g g   */
+  double dt, tweight_a, tweight_u, model_time;
+} pqo_state;
+
+typedef struct {
+  const double *beta, *s1, *s2; /* [L] */
+  double viscosity, visc_bot;
+  int bc_type;
+  double alpha;
+  int adapt_dt;
+  double target_cfl, max_dt, max_dt_increase;
+} pqo_params;
+
+/* Runs stages 1-13 (adapt_dt .. flux); if do_corrector, also the corrector
+ * and zeta_old <- zeta.  Returns the inversion status.                      */
+int pqo_step(pqo_state *s, const pqo_params *p, pqo_inv *inv,
+             int do_corrector);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
