@@ -204,3 +204,153 @@ def wind_field(g, tau, wind_shift=0.0, wind_tilt=0.0, wind_asym=1.0):
     fw = np.zeros((g, g), order="F")
     lib().pqo_wind_field(g, tau, wind_shift, wind_tilt, wind_asym, _p(fw))
     return fw
+
+
+# --------------------------------------------------------------------------
+# CABARET stages and the full step
+def _g_L_node(a):
+    return a.shape[0] - 2, (a.shape[2] if a.ndim == 3 else 1)
+
+
+def cabaret_cfl(dt, ui, uj):
+    g, L = ui.shape[0] - 3, ui.shape[2]
+    return lib().pqo_cabaret_cfl(g, L, dt, _p(ui), _p(uj))
+
+
+def cabaret_adapt_dt(target_cfl, max_dt, max_dt_increase, ui, uj, dt):
+    g, L = ui.shape[0] - 3, ui.shape[2]
+    d, a, u = ctypes.c_double(dt), ctypes.c_double(), ctypes.c_double()
+    lib().pqo_cabaret_adapt_dt(g, L, target_cfl, max_dt, max_dt_increase, _p(ui), _p(uj),
+                               ctypes.cast(ctypes.byref(d), _dp), ctypes.cast(ctypes.byref(a), _dp),
+                               ctypes.cast(ctypes.byref(u), _dp))
+    return d.value, a.value, u.value
+
+
+def cabaret_predictor(dt, zeta_old, res, zeta):
+    g, L = _g_L_node(zeta)
+    lib().pqo_cabaret_predictor(g, L, dt, _p(zeta_old), _p(res), _p(zeta))
+
+
+def cabaret_beta(dt, tw_a, beta, ui, zeta, res_beta):
+    g, L = _g_L_node(zeta)
+    lib().pqo_cabaret_beta(g, L, dt, tw_a, _p(f64(beta)), _p(ui), _p(zeta), _p(res_beta))
+
+
+def cabaret_forcing(dt, fw, visc, visc_bot, rel, zeta):
+    g, L = _g_L_node(zeta)
+    lib().pqo_cabaret_forcing(g, L, dt, _p(fw), visc, visc_bot, _p(rel), _p(zeta))
+
+
+def cabaret_cell_boundaries(tw_u, psi, ui, uj, uimo, ujmo):
+    g, L = _g_L_node(psi)
+    lib().pqo_cabaret_cell_boundaries(g, L, tw_u, _p(psi), _p(ui), _p(uj), _p(uimo), _p(ujmo))
+
+
+def cabaret_cell_boundaries_velocity(psi, ui, uj):
+    g, L = _g_L_node(psi)
+    lib().pqo_cabaret_cell_boundaries_velocity(g, L, _p(psi), _p(ui), _p(uj))
+
+
+def cabaret_extrapolator(dt, zeta, zeta_old, ui, uj, zfi, zfj):
+    g, L = _g_L_node(zeta)
+    lib().pqo_cabaret_extrapolator(g, L, dt, _p(zeta), _p(zeta_old), _p(ui), _p(uj), _p(zfi),
+                                   _p(zfj))
+
+
+def cabaret_flux(ui, uj, zfi, zfj, flux_i, flux_j):
+    g, L = ui.shape[0] - 3, ui.shape[2]
+    lib().pqo_cabaret_flux(g, L, _p(ui), _p(uj), _p(zfi), _p(zfj), _p(flux_i), _p(flux_j))
+
+
+def cabaret_corrector(dt, flux_i, flux_j, res, zeta):
+    g, L = _g_L_node(zeta)
+    lib().pqo_cabaret_corrector(g, L, dt, _p(flux_i), _p(flux_j), _p(res), _p(zeta))
+
+
+def rel_from_zeta_and_psi(s1, s2, bc_type, alpha, zeta, psi, rel):
+    g, L = _g_L_node(zeta)
+    lib().pqo_rel_from_zeta_and_psi(g, L, _p(f64(s1)), _p(f64(s2)), bc_type, alpha, _p(zeta),
+                                    _p(psi), _p(rel))
+
+
+def update_zeta_bc(s1, s2, rel, psi, zeta):
+    g, L = _g_L_node(zeta)
+    lib().pqo_update_zeta_bc(g, L, _p(f64(s1)), _p(f64(s2)), _p(rel), _p(psi), _p(zeta))
+
+
+class State(ctypes.Structure):
+    _fields_ = [("g", ctypes.c_int), ("layers", ctypes.c_int)] + [
+        (n, _dp) for n in ("psi", "rel", "zeta", "zeta_old", "zeta_fi", "ui", "uimo", "flux_i",
+                           "zeta_fj", "uj", "ujmo", "flux_j", "res", "res_beta", "force_wind")] + [
+        (n, ctypes.c_double) for n in ("dt", "tweight_a", "tweight_u", "model_time")]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("beta", _dp), ("s1", _dp), ("s2", _dp), ("viscosity", ctypes.c_double),
+                ("visc_bot", ctypes.c_double), ("bc_type", ctypes.c_int),
+                ("alpha", ctypes.c_double), ("adapt_dt", ctypes.c_int),
+                ("target_cfl", ctypes.c_double), ("max_dt", ctypes.c_double),
+                ("max_dt_increase", ctypes.c_double)]
+
+
+class Model:
+    """The reference's CABARET time loop on one domain (qg3.F90:590-967),
+    driving the oracle's stages; state arrays are Fortran-ordered numpy."""
+
+    NODE = ("psi", "rel", "zeta", "zeta_old")
+    FI = ("zeta_fi", "ui", "uimo", "flux_i")
+    FJ = ("zeta_fj", "uj", "ujmo", "flux_j")
+
+    def __init__(self, g, strat_s1, strat_s2, ss, m2z, z2m, beta, force_wind, *, viscosity,
+                 visc_bot, dt, bc_type=1, alpha=0.0, adapt_dt=False, target_cfl=0.1,
+                 max_dt=0.0, max_dt_increase=1.5):
+        L = len(ss)
+        self.g, self.layers = g, L
+        self.a = {}
+        for n in self.NODE:
+            self.a[n] = np.zeros((g + 2, g + 2, L), order="F")
+        for n in self.FI:
+            self.a[n] = np.zeros((g + 3, g + 2, L), order="F")
+        for n in self.FJ:
+            self.a[n] = np.zeros((g + 2, g + 3, L), order="F")
+        self.a["res"] = np.zeros((g, g, L), order="F")
+        self.a["res_beta"] = np.zeros((g, g, L), order="F")
+        self.a["force_wind"] = f64(force_wind)
+        self.s1, self.s2, self.beta = f64(strat_s1), f64(strat_s2), f64(beta)
+        self.inv = Inversion(g, ss, m2z, z2m)
+        self.st = State(g, L, *[_p(self.a[n]) for n in (
+            "psi", "rel", "zeta", "zeta_old", "zeta_fi", "ui", "uimo", "flux_i", "zeta_fj", "uj",
+            "ujmo", "flux_j", "res", "res_beta", "force_wind")], dt, 0.5, 0.5, dt / 2.0)
+        self.pr = Params(_p(self.beta), _p(self.s1), _p(self.s2), viscosity, visc_bot, bc_type,
+                         alpha, int(adapt_dt), target_cfl, max_dt, max_dt_increase)
+        L_ = lib()
+        L_.pqo_step.restype = ctypes.c_int
+        L_.pqo_step.argtypes = [ctypes.POINTER(State), ctypes.POINTER(Params), ctypes.c_void_p,
+                                ctypes.c_int]
+
+    def init(self, zeta_sdata):
+        """START_PERTURB-style start (qg3.F90:393-425) from an interior field."""
+        g, L = self.g, self.layers
+        z = self.a["zeta"]
+        z[:] = 0.0
+        z[2:g, 2:g, :] = f64(zeta_sdata)[1:g - 1, 1:g - 1, :]
+        off = 8 * (1 + (g + 2))  # element (2, 2): the sdata view
+        rc = lib().pqo_inv_run(self.inv.h, ctypes.cast(z.ctypes.data + off, _dp), g + 2, g + 2,
+                               ctypes.cast(self.a["psi"].ctypes.data + off, _dp), g + 2, g + 2)
+        rel_from_zeta_and_psi(self.s1, self.s2, self.pr.bc_type, self.pr.alpha, z, self.a["psi"],
+                              self.a["rel"])
+        update_zeta_bc(self.s1, self.s2, self.a["rel"], self.a["psi"], z)
+        self.a["zeta_old"][:] = z
+        return rc
+
+    def step(self, nsteps=1, final_partial=False):
+        for k in range(nsteps):
+            last = final_partial and k == nsteps - 1
+            rc = lib().pqo_step(ctypes.byref(self.st), ctypes.byref(self.pr), self.inv.h,
+                                0 if last else 1)
+            if rc == 3:
+                raise FloatingPointError("NaN psiC encountered")
+
+    def scalars(self):
+        return dict(dt=self.st.dt, tweight_a=self.st.tweight_a, tweight_u=self.st.tweight_u,
+                    model_time=self.st.model_time)

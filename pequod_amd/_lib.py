@@ -1,0 +1,106 @@
+"""Loader for libpequod_hip.so, the HIP implementation behind the C ABI in
+include/pequod_hip.h.  There is no fallback: if the library is missing or
+cannot be loaded, every entry point raises."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpequod_hip.so")
+
+PQ_OK, PQ_ERR_ARG, PQ_ERR_HIP, PQ_ERR_NAN, PQ_WARN_LARGE, PQ_ERR_NOMEM = range(6)
+PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
+
+_i, _d, _vp, _sz, _l = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_long
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); mirrors include/pequod_hip.h
+SIGNATURES = {
+    "pq_last_error": (ctypes.c_char_p, []),
+    "pq_device_info": (_i, [ctypes.c_char_p, _i]),
+    "pq_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
+    "pq_free": (_i, [_vp]),
+    "pq_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+    "pq_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "pq_synchronize": (_i, []),
+    "pq_mirror_register": (_i, [_vp, _sz]),
+    "pq_mirror_unregister": (_i, [_vp]),
+    "pq_mirror_device_ptr": (_vp, [_vp]),
+    "pq_mirror_push": (_i, [_vp]),
+    "pq_mirror_pull": (_i, [_vp]),
+    "pq_helm_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _dp, _i]),
+    "pq_helm_destroy": (_i, [_vp]),
+    "pq_helm_solve": (_i, [_vp, _dp, _i, _i]),
+    "pq_helm_solve_dev": (_i, [_vp, _vp, _l, _l, _vp]),
+    "pq_inv_create": (_i, [ctypes.POINTER(_vp), _i, _i, _dp, _dp, _dp, _i, _i]),
+    "pq_inv_destroy": (_i, [_vp]),
+    "pq_inv_run": (_i, [_vp, _dp, _i, _i, _dp, _i, _i]),
+    "pq_inv_run_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _vp]),
+    "pq_inv_status": (_i, [_vp, _dp]),
+    "pq_inv_stream": (_vp, [_vp]),
+    "pq_cabaret_cfl_dev": (_i, [_i, _i, _d, _vp, _vp, _dp, _vp]),
+    "pq_cabaret_predictor_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_beta_dev": (_i, [_i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_forcing_dev": (_i, [_i, _i, _d, _vp, _d, _d, _vp, _vp, _vp]),
+    "pq_cabaret_cell_boundaries_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_cell_boundaries_velocity_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_extrapolator_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_flux_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_corrector_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp]),
+    "pq_rel_from_zeta_and_psi_dev": (_i, [_i, _i, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp]),
+    "pq_update_zeta_bc_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_model_create": (_i, [ctypes.POINTER(_vp), _vp]),
+    "pq_model_destroy": (_i, [_vp]),
+    "pq_model_init": (_i, [_vp, _dp]),
+    "pq_model_step": (_i, [_vp, _i, _i]),
+    "pq_model_get": (_i, [_vp, _i, _dp, _sz]),
+    "pq_model_set": (_i, [_vp, _i, _dp, _sz]),
+    "pq_model_scalars": (_i, [_vp, _dp]),
+    "pq_model_stream": (_vp, [_vp]),
+}
+
+
+class PqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pequod_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libpequod_hip.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError(f"{path} not found: build it with `python -m pequod_amd.build`")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, allow_warn: bool = False) -> int:
+    if rc == PQ_OK or (allow_warn and rc == PQ_WARN_LARGE):
+        return rc
+    msg = load().pq_last_error().decode(errors="replace")
+    raise PqError(rc, msg)
+
+
+class PqModelParams(ctypes.Structure):
+    """pq_model_params of include/pequod_hip.h."""
+    _fields_ = [
+        ("g", _i), ("layers", _i),
+        ("s1", _dp), ("s2", _dp),
+        ("ss", _dp), ("modes_to_z", _dp), ("z_to_modes", _dp),
+        ("beta", _dp), ("force_wind", _dp),
+        ("viscosity", _d), ("visc_bot", _d),
+        ("bc_type", _i), ("alpha", _d),
+        ("adapt_dt", _i), ("target_cfl", _d), ("max_dt", _d), ("max_dt_increase", _d),
+        ("dt", _d), ("precision", _i),
+    ]

@@ -1,0 +1,740 @@
+// pq_cabaret.hip -- CABARET PV advection stages and the device-resident
+// time step on MI355X.
+//
+// Each stage kernel restates one routine of qg3p_aux.F90 (cited per kernel)
+// on a single, non-periodic domain, with the reference's array shapes and
+// 1-based index ranges (macros below).  FP contraction is disabled in this
+// file so each statement rounds exactly as written, in the order written;
+// the CPU oracle is compiled the same way, which makes the stage kernels
+// bit-for-bit comparable with it.
+//
+// Index ranges on one domain (qg3p_base.F90:887-941): owned nodes in `data`
+// are 2..G+1 (fa_*), interior nodes 3..G (f_*); i-faces of FLUX_II arrays
+// sit between data nodes i-1 and i.
+#pragma clang fp contract(off)
+
+#include "pq_common.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// 1-based element addresses of the reference's arrays.
+struct Dims {
+  int g, L;
+  __device__ __host__ size_t nd(int i, int j, int l) const {
+    return (size_t)(i - 1) + (size_t)(g + 2) * ((size_t)(j - 1) + (size_t)(g + 2) * l);
+  }
+  __device__ __host__ size_t fi(int i, int j, int l) const {
+    return (size_t)(i - 1) + (size_t)(g + 3) * ((size_t)(j - 1) + (size_t)(g + 2) * l);
+  }
+  __device__ __host__ size_t fj(int i, int j, int l) const {
+    return (size_t)(i - 1) + (size_t)(g + 2) * ((size_t)(j - 1) + (size_t)(g + 3) * l);
+  }
+  __device__ __host__ size_t rs(int i, int j, int l) const {
+    return (size_t)(i - 1) + (size_t)g * ((size_t)(j - 1) + (size_t)g * l);
+  }
+  size_t n_nd() const { return (size_t)(g + 2) * (g + 2) * L; }
+  size_t n_fi() const { return (size_t)(g + 3) * (g + 2) * L; }
+  size_t n_fj() const { return (size_t)(g + 2) * (g + 3) * L; }
+  size_t n_rs() const { return (size_t)g * g * L; }
+};
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 grid_for(int ni, int nj, int L) {
+  return dim3((ni + BX - 1) / BX, (nj + BY - 1) / BY, L);
+}
+
+// Thread -> (i, j, l) over i in [i0, i0+ni), j in [j0, j0+nj).
+#define PQ_IDX(i0, ni, j0, nj)                                                  \
+  const int i = (i0) + (int)(blockIdx.x * BX + threadIdx.x);                    \
+  const int j = (j0) + (int)(blockIdx.y * BY + threadIdx.y);                    \
+  const int l = (int)blockIdx.z;                                                \
+  if (i >= (i0) + (ni) || j >= (j0) + (nj)) return;
+
+// cabaret_predictor (qg3p_aux.F90:993-1020): f range 3..g
+__global__ void k_predictor(Dims d, double dt, const double *__restrict__ zeta_old,
+                            const double *__restrict__ res, double *__restrict__ zeta) {
+  PQ_IDX(3, d.g - 2, 3, d.g - 2);
+  zeta[d.nd(i, j, l)] = zeta_old[d.nd(i, j, l)] + 0.5 * dt * res[d.rs(i - 1, j - 1, l)];
+}
+
+// cabaret_beta (qg3p_aux.F90:1023-1061): l range 2..g-1 of sdata
+__global__ void k_beta(Dims d, double dt, double tw_a, const double *__restrict__ beta,
+                       const double *__restrict__ ui, double *__restrict__ zeta,
+                       double *__restrict__ res_beta) {
+  PQ_IDX(2, d.g - 2, 2, d.g - 2);
+  const double rb = dt * (0.5 * (ui[d.fi(i + 2, j + 1, l)] + ui[d.fi(i + 1, j + 1, l)]) * beta[l]);
+  const size_t z = d.nd(i + 1, j + 1, l);
+  zeta[z] = zeta[z] + (1.0 + tw_a) * rb - tw_a * res_beta[d.rs(i, j, l)];
+  res_beta[d.rs(i, j, l)] = rb;
+}
+
+// cabaret_predictor followed by cabaret_beta, one pass (the beta update of
+// data node (i, j) only reads that node's predicted value).
+__global__ void k_predictor_beta(Dims d, double dt, double tw_a,
+                                 const double *__restrict__ beta,
+                                 const double *__restrict__ zeta_old,
+                                 const double *__restrict__ res,
+                                 const double *__restrict__ ui, double *__restrict__ zeta,
+                                 double *__restrict__ res_beta) {
+  PQ_IDX(3, d.g - 2, 3, d.g - 2);
+  double z = zeta_old[d.nd(i, j, l)] + 0.5 * dt * res[d.rs(i - 1, j - 1, l)];
+  const double rb = dt * (0.5 * (ui[d.fi(i + 1, j, l)] + ui[d.fi(i, j, l)]) * beta[l]);
+  z = z + (1.0 + tw_a) * rb - tw_a * res_beta[d.rs(i - 1, j - 1, l)];
+  zeta[d.nd(i, j, l)] = z;
+  res_beta[d.rs(i - 1, j - 1, l)] = rb;
+}
+
+// cabaret_forcing with WIND_FORCING (qg3p_aux.F90:1065-1164)
+__global__ void k_forcing(Dims d, double dt, const double *__restrict__ fw, double visc,
+                          double visc_bot, const double *__restrict__ rel,
+                          double *__restrict__ zeta) {
+  PQ_IDX(3, d.g - 2, 3, d.g - 2);
+  const int L = d.L;
+  const double lap = -4.0 * rel[d.nd(i, j, l)] + rel[d.nd(i - 1, j, l)] + rel[d.nd(i + 1, j, l)] +
+                     rel[d.nd(i, j - 1, l)] + rel[d.nd(i, j + 1, l)];
+  const double f = fw[(size_t)(i - 2) + (size_t)d.g * (j - 2)];
+  double rv;
+  if (L == 1)
+    rv = dt * (f - visc_bot * rel[d.nd(i, j, 0)] + visc * lap);
+  else if (l == 0)
+    rv = dt * (f + visc * lap);
+  else if (l < L - 1)
+    rv = dt * visc * lap;
+  else
+    rv = dt * (-visc_bot * rel[d.nd(i, j, l)] + visc * lap);
+  zeta[d.nd(i, j, l)] = zeta[d.nd(i, j, l)] + rv;
+}
+
+__device__ __forceinline__ double psi_edge(const Dims &d, const double *psi, int i, int j, int l) {
+  return 0.25 * (psi[d.nd(i, j, l)] + psi[d.nd(i, j - 1, l)] + psi[d.nd(i - 1, j, l)] +
+                 psi[d.nd(i - 1, j - 1, l)]);
+}
+
+// cabaret_cell_boundaries_time_integrated (qg3p_aux.F90:1235-1313).  One
+// thread per (i, j) in 3..g+1 x 3..g+1 covers both face sets.
+__global__ void k_cell_boundaries(Dims d, double tw, const double *__restrict__ psi,
+                                  double *__restrict__ ui, double *__restrict__ uj,
+                                  double *__restrict__ uimo, double *__restrict__ ujmo) {
+  PQ_IDX(3, d.g - 1, 3, d.g - 1);
+  const int g = d.g;
+  const double pe = psi_edge(d, psi, i, j, l);
+  if (j <= g) {
+    const double uim = -psi_edge(d, psi, i, j + 1, l) + pe;
+    const size_t a = d.fi(i, j, l);
+    ui[a] = (1.0 + tw) * uim - tw * uimo[a];
+    uimo[a] = uim;
+  }
+  if (i <= g) {
+    const double ujm = +psi_edge(d, psi, i + 1, j, l) - pe;
+    const size_t a = d.fj(i, j, l);
+    uj[a] = (1.0 + tw) * ujm - tw * ujmo[a];
+    ujmo[a] = ujm;
+  }
+}
+
+// cabaret_cell_boundaries_velocity (qg3p_aux.F90:1168-1231)
+__global__ void k_cell_boundaries_velocity(Dims d, const double *__restrict__ psi,
+                                           double *__restrict__ ui, double *__restrict__ uj) {
+  PQ_IDX(3, d.g - 1, 3, d.g - 1);
+  const int g = d.g;
+  const double pe = psi_edge(d, psi, i, j, l);
+  if (j <= g) ui[d.fi(i, j, l)] = -psi_edge(d, psi, i, j + 1, l) + pe;
+  if (i <= g) uj[d.fj(i, j, l)] = +psi_edge(d, psi, i + 1, j, l) - pe;
+}
+
+__device__ __forceinline__ double max3(double a, double b, double c) {
+  const double m = a > b ? a : b;
+  return m > c ? m : c;
+}
+__device__ __forceinline__ double min3(double a, double b, double c) {
+  const double m = a < b ? a : b;
+  return m < c ? m : c;
+}
+
+// One face of cabaret_extrapolator with CABARET_LIMITER
+// (qg3p_aux.F90:1362-1404 and 1450-1484).  lo/hi: is the upwind neighbour
+// face beyond the boundary (fb range test on a single domain).
+__device__ __forceinline__ double extrap_face(double dt, double u1, double fi1, double u_m,
+                                              double f_m, double u_p, double f_p,
+                                              double zn_m, double zo_m, double zn_p,
+                                              double zo_p, bool bnd_lo, bool bnd_hi) {
+  double fi2, psi_n, d_f;
+  if (u1 >= 0.0) {
+    if (!bnd_lo) {
+      fi2 = f_m;
+      const double v0 = (u1 + u_m) / 2.0;
+      psi_n = zn_m;
+      d_f = -(psi_n - zo_m) * 2.0 - dt * v0 * (fi1 - fi2);
+    } else {
+      fi2 = zn_m;
+      psi_n = zn_m;
+      d_f = -(psi_n - zo_m);
+    }
+  } else {
+    if (!bnd_hi) {
+      fi2 = f_p;
+      const double v0 = (u1 + u_p) / 2.0;
+      psi_n = zn_p;
+      d_f = -(psi_n - zo_p) * 2.0 - dt * v0 * (fi2 - fi1);
+    } else {
+      fi2 = zn_p;
+      psi_n = zn_p;
+      d_f = -(psi_n - zo_p);
+    }
+  }
+  double f = 2.0 * psi_n - fi2;
+  const double fmx = max3(fi1, fi2, psi_n) - d_f;
+  const double fmn = min3(fi1, fi2, psi_n) - d_f;
+  if (f > fmx) f = fmx;
+  if (f < fmn) f = fmn;
+  return f;
+}
+
+// cabaret_extrapolator, new face values into fin / fjn.
+__global__ void k_extrapolator(Dims d, double dt, const double *__restrict__ zeta,
+                               const double *__restrict__ zeta_old, const double *__restrict__ ui,
+                               const double *__restrict__ uj, const double *__restrict__ zfi,
+                               const double *__restrict__ zfj, double *__restrict__ fin,
+                               double *__restrict__ fjn) {
+  PQ_IDX(3, d.g - 1, 3, d.g - 1);
+  const int g = d.g;
+  if (j <= g) {  // i-face (i, j), i in 3..g+1
+    const size_t a = d.fi(i, j, l);
+    const bool lo = !(i > 3), hi = !(i < g + 1);
+    const double u1 = ui[a], fi1 = zfi[a];
+    const double um = lo ? 0.0 : ui[d.fi(i - 1, j, l)], fm = lo ? 0.0 : zfi[d.fi(i - 1, j, l)];
+    const double up = hi ? 0.0 : ui[d.fi(i + 1, j, l)], fp = hi ? 0.0 : zfi[d.fi(i + 1, j, l)];
+    fin[a] = extrap_face(dt, u1, fi1, um, fm, up, fp, zeta[d.nd(i - 1, j, l)],
+                         zeta_old[d.nd(i - 1, j, l)], zeta[d.nd(i, j, l)],
+                         zeta_old[d.nd(i, j, l)], lo, hi);
+  }
+  if (i <= g) {  // j-face (i, j), j in 3..g+1
+    const size_t a = d.fj(i, j, l);
+    const bool lo = !(j > 3), hi = !(j < g + 1);
+    const double u1 = uj[a], fi1 = zfj[a];
+    const double um = lo ? 0.0 : uj[d.fj(i, j - 1, l)], fm = lo ? 0.0 : zfj[d.fj(i, j - 1, l)];
+    const double up = hi ? 0.0 : uj[d.fj(i, j + 1, l)], fp = hi ? 0.0 : zfj[d.fj(i, j + 1, l)];
+    fjn[a] = extrap_face(dt, u1, fi1, um, fm, up, fp, zeta[d.nd(i, j - 1, l)],
+                         zeta_old[d.nd(i, j - 1, l)], zeta[d.nd(i, j, l)],
+                         zeta_old[d.nd(i, j, l)], lo, hi);
+  }
+}
+
+// Copy the extrapolated faces back (qg3p_aux.F90:1526-1543).
+__global__ void k_face_copy(Dims d, const double *__restrict__ fin, const double *__restrict__ fjn,
+                            double *__restrict__ zfi, double *__restrict__ zfj) {
+  PQ_IDX(3, d.g - 1, 3, d.g - 1);
+  if (j <= d.g) zfi[d.fi(i, j, l)] = fin[d.fi(i, j, l)];
+  if (i <= d.g) zfj[d.fj(i, j, l)] = fjn[d.fj(i, j, l)];
+}
+
+// cabaret_flux (qg3p_aux.F90:1547-1594): i-faces 2..g+2 x 3..g, j-faces
+// 3..g x 2..g+2.
+__global__ void k_flux(Dims d, const double *__restrict__ ui, const double *__restrict__ uj,
+                       const double *__restrict__ zfi, const double *__restrict__ zfj,
+                       double *__restrict__ flux_i, double *__restrict__ flux_j) {
+  PQ_IDX(2, d.g + 1, 2, d.g + 1);
+  const int g = d.g;
+  if (j >= 3 && j <= g) flux_i[d.fi(i, j, l)] = ui[d.fi(i, j, l)] * zfi[d.fi(i, j, l)];
+  if (i >= 3 && i <= g) flux_j[d.fj(i, j, l)] = uj[d.fj(i, j, l)] * zfj[d.fj(i, j, l)];
+}
+
+// cabaret_corrector (qg3p_aux.F90:1597-1642)
+__global__ void k_corrector(Dims d, double dt, const double *__restrict__ flux_i,
+                            const double *__restrict__ flux_j, double *__restrict__ res,
+                            double *__restrict__ zeta) {
+  PQ_IDX(3, d.g - 2, 3, d.g - 2);
+  const double r = -(flux_i[d.fi(i + 1, j, l)] - flux_i[d.fi(i, j, l)] + flux_j[d.fj(i, j + 1, l)] -
+                     flux_j[d.fj(i, j, l)]);
+  res[d.rs(i - 1, j - 1, l)] = r;
+  zeta[d.nd(i, j, l)] = zeta[d.nd(i, j, l)] + 0.5 * dt * r;
+}
+
+// cabaret_corrector writing the corrected field into zeta_out (the step's
+// zeta_old for the next step) from zeta_in; boundary values are copied.
+// Covers all owned nodes 2..g+1.
+__global__ void k_corrector_to(Dims d, double dt, const double *__restrict__ flux_i,
+                               const double *__restrict__ flux_j, double *__restrict__ res,
+                               const double *__restrict__ zeta_in, double *__restrict__ zeta_out) {
+  PQ_IDX(1, d.g + 2, 1, d.g + 2);
+  const int g = d.g;
+  const size_t a = d.nd(i, j, l);
+  if (i >= 3 && i <= g && j >= 3 && j <= g) {
+    const double r = -(flux_i[d.fi(i + 1, j, l)] - flux_i[d.fi(i, j, l)] +
+                       flux_j[d.fj(i, j + 1, l)] - flux_j[d.fj(i, j, l)]);
+    res[d.rs(i - 1, j - 1, l)] = r;
+    zeta_out[a] = zeta_in[a] + 0.5 * dt * r;
+  } else {
+    zeta_out[a] = zeta_in[a];
+  }
+}
+
+// rel_from_zeta_and_psi + apply_rel_bc (qg3p_aux.F90:738-915) over owned
+// nodes 2..g+1.  Boundary nodes take the boundary formula; on corners the
+// left/right (j) edge formula wins, as it is applied last in the reference.
+__global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
+                      int bc_type, double alpha, const double *__restrict__ zeta,
+                      const double *__restrict__ psi, double *__restrict__ rel) {
+  PQ_IDX(2, d.g, 2, d.g);
+  const int g = d.g, L = d.L;
+  const size_t a = d.nd(i, j, l);
+  double r;
+  const bool jb = (j == 2 || j == g + 1), ib = (i == 2 || i == g + 1);
+  if (jb || ib) {
+    if (bc_type == 1) {
+      const double cff = alpha / (alpha + 3.0);
+      if (jb) {
+        const int s = (j == 2) ? 1 : -1;
+        r = cff * (-3.5 * psi[a] + 4.0 * psi[d.nd(i, j + s, l)] - 0.5 * psi[d.nd(i, j + 2 * s, l)]);
+      } else {
+        const int s = (i == 2) ? 1 : -1;
+        r = cff * (-3.5 * psi[a] + 4.0 * psi[d.nd(i + s, j, l)] - 0.5 * psi[d.nd(i + 2 * s, j, l)]);
+      }
+    } else if (bc_type == 2) {
+      if (jb) {
+        r = (j == 2) ? 2.0 * (psi[d.nd(i, 3, l)] - psi[a]) : 2.0 * (psi[d.nd(i, g, l)] - psi[a]);
+      } else {
+        r = (i == 2) ? 2.0 * (psi[d.nd(3, j, l)] - psi[a]) : 2.0 * (psi[d.nd(g, j, l)] - psi[a]);
+      }
+    } else {
+      r = 0.0;
+    }
+  } else if (L == 1) {
+    r = zeta[a];
+  } else if (l == 0) {
+    r = zeta[a] + s2[0] * (psi[a] - psi[d.nd(i, j, 1)]);
+  } else if (l < L - 1) {
+    r = zeta[a] + s1[l] * (psi[a] - psi[d.nd(i, j, l - 1)]) + s2[l] * (psi[a] - psi[d.nd(i, j, l + 1)]);
+  } else {
+    r = zeta[a] + s1[L - 1] * (psi[a] - psi[d.nd(i, j, L - 2)]);
+  }
+  rel[a] = r;
+}
+
+// update_zeta_bc (qg3p_aux.F90:643-734): boundary nodes only.
+__global__ void k_zeta_bc(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
+                          const double *__restrict__ rel, const double *__restrict__ psi,
+                          double *__restrict__ zeta) {
+  PQ_IDX(2, d.g, 2, d.g);
+  const int g = d.g, L = d.L;
+  if (!(i == 2 || i == g + 1 || j == 2 || j == g + 1)) return;
+  const size_t a = d.nd(i, j, l);
+  double z;
+  if (L == 1) z = rel[a];
+  else if (l == 0) z = rel[a] - s2[0] * (psi[a] - psi[d.nd(i, j, 1)]);
+  else if (l == L - 1) z = rel[a] - s1[L - 1] * (psi[a] - psi[d.nd(i, j, L - 2)]);
+  else z = rel[a] - s1[l] * (psi[a] - psi[d.nd(i, j, l - 1)]) - s2[l] * (psi[a] - psi[d.nd(i, j, l + 1)]);
+  zeta[a] = z;
+}
+
+// max |u| over the cabaret_cfl face ranges (qg3p_aux.F90:937-940); the
+// non-negative doubles are ordered like their bit patterns.
+__global__ void k_maxabs(Dims d, const double *__restrict__ ui, const double *__restrict__ uj,
+                         unsigned long long *__restrict__ out) {
+  const int i = 3 + (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = 3 + (int)(blockIdx.y * BY + threadIdx.y);
+  const int l = (int)blockIdx.z;
+  const int g = d.g;
+  double m = 0.0;
+  if (i <= g + 1 && j <= g + 1) {
+    if (j <= g) m = fmax(m, fabs(ui[d.fi(i, j, l)]));
+    if (i <= g) m = fmax(m, fabs(uj[d.fj(i, j, l)]));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __shared__ double sm[BX * BY / 64];
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  if ((tid & 63) == 0) sm[tid >> 6] = m;
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < BX * BY / 64; ++w) m = fmax(m, sm[w]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+hipStream_t S(void *s) { return (hipStream_t)s; }
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    pq_set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return PQ_ERR_HIP;
+  }
+  return PQ_OK;
+}
+
+#define CHECK_GL(g, L, fn)                                                     \
+  PQ_CHECK_ARG((g) >= 5 && (L) >= 1 && (L) <= 64, "%s: bad size g=%d L=%d", fn, g, L)
+
+}  // namespace
+
+// -------------------------------------------------------------------- C ABI
+extern "C" {
+
+int pq_cabaret_cfl_dev(int g, int L, double dt, const double *ui, const double *uj,
+                       double *cfl_host, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_cfl_dev");
+  PQ_CHECK_ARG(ui && uj && cfl_host, "pq_cabaret_cfl_dev: null pointer");
+  Dims d{g, L};
+  unsigned long long *dm = nullptr;
+  PQ_HIP(hipMallocAsync((void **)&dm, sizeof(unsigned long long), S(stream)));
+  PQ_HIP(hipMemsetAsync(dm, 0, sizeof(unsigned long long), S(stream)));
+  hipLaunchKernelGGL(k_maxabs, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, ui, uj, dm);
+  unsigned long long bits = 0;
+  PQ_HIP(hipMemcpyAsync(&bits, dm, sizeof(bits), hipMemcpyDeviceToHost, S(stream)));
+  PQ_HIP(hipFreeAsync(dm, S(stream)));
+  PQ_HIP(hipStreamSynchronize(S(stream)));
+  double m;
+  std::memcpy(&m, &bits, sizeof(m));
+  *cfl_host = m * dt;
+  return PQ_OK;
+}
+
+int pq_cabaret_predictor_dev(int g, int L, double dt, const double *zeta_old, const double *res,
+                             double *zeta, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_predictor_dev");
+  hipLaunchKernelGGL(k_predictor, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+                     dt, zeta_old, res, zeta);
+  return check_launch("pq_cabaret_predictor_dev");
+}
+
+int pq_cabaret_beta_dev(int g, int L, double dt, double tw_a, const double *beta, const double *ui,
+                        double *zeta, double *res_beta, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_beta_dev");
+  hipLaunchKernelGGL(k_beta, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, dt,
+                     tw_a, beta, ui, zeta, res_beta);
+  return check_launch("pq_cabaret_beta_dev");
+}
+
+int pq_cabaret_forcing_dev(int g, int L, double dt, const double *fw, double visc,
+                           double visc_bot, const double *rel, double *zeta, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_forcing_dev");
+  hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+                     dt, fw, visc, visc_bot, rel, zeta);
+  return check_launch("pq_cabaret_forcing_dev");
+}
+
+int pq_cabaret_cell_boundaries_dev(int g, int L, double tw, const double *psi, double *ui,
+                                   double *uj, double *uimo, double *ujmo, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_cell_boundaries_dev");
+  hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream),
+                     Dims{g, L}, tw, psi, ui, uj, uimo, ujmo);
+  return check_launch("pq_cabaret_cell_boundaries_dev");
+}
+
+int pq_cabaret_cell_boundaries_velocity_dev(int g, int L, const double *psi, double *ui,
+                                            double *uj, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_cell_boundaries_velocity_dev");
+  hipLaunchKernelGGL(k_cell_boundaries_velocity, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0,
+                     S(stream), Dims{g, L}, psi, ui, uj);
+  return check_launch("pq_cabaret_cell_boundaries_velocity_dev");
+}
+
+int pq_cabaret_extrapolator_dev(int g, int L, double dt, const double *zeta,
+                                const double *zeta_old, const double *ui, const double *uj,
+                                double *zfi, double *zfj, double *scratch, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_extrapolator_dev");
+  Dims d{g, L};
+  double *tmp = scratch;
+  if (!tmp) PQ_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (d.n_fi() + d.n_fj()), S(stream)));
+  hipLaunchKernelGGL(k_extrapolator, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, dt,
+                     zeta, zeta_old, ui, uj, zfi, zfj, tmp, tmp + d.n_fi());
+  hipLaunchKernelGGL(k_face_copy, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, tmp,
+                     tmp + d.n_fi(), zfi, zfj);
+  if (!scratch) PQ_HIP(hipFreeAsync(tmp, S(stream)));
+  return check_launch("pq_cabaret_extrapolator_dev");
+}
+
+int pq_cabaret_flux_dev(int g, int L, const double *ui, const double *uj, const double *zfi,
+                        const double *zfj, double *flux_i, double *flux_j, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_flux_dev");
+  hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, ui,
+                     uj, zfi, zfj, flux_i, flux_j);
+  return check_launch("pq_cabaret_flux_dev");
+}
+
+int pq_cabaret_corrector_dev(int g, int L, double dt, const double *flux_i, const double *flux_j,
+                             double *res, double *zeta, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_corrector_dev");
+  hipLaunchKernelGGL(k_corrector, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+                     dt, flux_i, flux_j, res, zeta);
+  return check_launch("pq_cabaret_corrector_dev");
+}
+
+int pq_rel_from_zeta_and_psi_dev(int g, int L, const double *s1, const double *s2, int bc_type,
+                                 double alpha, const double *zeta, const double *psi, double *rel,
+                                 void *stream) {
+  CHECK_GL(g, L, "pq_rel_from_zeta_and_psi_dev");
+  PQ_CHECK_ARG(bc_type >= 1 && bc_type <= 3, "pq_rel_from_zeta_and_psi_dev: bad bc_type %d", bc_type);
+  hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
+                     bc_type, alpha, zeta, psi, rel);
+  return check_launch("pq_rel_from_zeta_and_psi_dev");
+}
+
+int pq_update_zeta_bc_dev(int g, int L, const double *s1, const double *s2, const double *rel,
+                          const double *psi, double *zeta, void *stream) {
+  CHECK_GL(g, L, "pq_update_zeta_bc_dev");
+  hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
+                     rel, psi, zeta);
+  return check_launch("pq_update_zeta_bc_dev");
+}
+
+// ------------------------------------------------------------------ model
+struct pq_model {
+  int g = 0, L = 0;
+  Dims d{};
+  pq_inv *inv = nullptr;
+  hipStream_t stream = nullptr;
+  // device fields
+  double *psi = nullptr, *rel = nullptr, *zeta = nullptr, *zeta_old = nullptr;
+  double *zfi = nullptr, *ui = nullptr, *uimo = nullptr, *flux_i = nullptr;
+  double *zfj = nullptr, *uj = nullptr, *ujmo = nullptr, *flux_j = nullptr;
+  double *res = nullptr, *res_beta = nullptr, *fw = nullptr;
+  double *fin = nullptr, *fjn = nullptr;
+  double *s1 = nullptr, *s2 = nullptr, *beta = nullptr;
+  unsigned long long *maxu = nullptr;
+  unsigned long long *maxu_host = nullptr;
+  // scalars
+  double dt = 0, tw_a = 0.5, tw_u = 0.5, model_time = 0;
+  double visc = 0, visc_bot = 0, alpha = 0;
+  int bc_type = 1, adapt = 0;
+  double target_cfl = 0, max_dt = 0, max_dt_inc = 0;
+};
+
+static double **model_field(pq_model *m, int which, size_t *count) {
+  const Dims &d = m->d;
+  switch (which) {
+    case 0: *count = d.n_nd(); return &m->psi;
+    case 1: *count = d.n_nd(); return &m->zeta;
+    case 2: *count = d.n_nd(); return &m->rel;
+    case 3: *count = d.n_nd(); return &m->zeta_old;
+    case 4: *count = d.n_fi(); return &m->ui;
+    case 5: *count = d.n_fj(); return &m->uj;
+    case 6: *count = d.n_fi(); return &m->zfi;
+    case 7: *count = d.n_fj(); return &m->zfj;
+    case 8: *count = d.n_rs(); return &m->res;
+    case 9: *count = d.n_rs(); return &m->res_beta;
+    case 10: *count = d.n_fi(); return &m->uimo;
+    case 11: *count = d.n_fj(); return &m->ujmo;
+    case 12: *count = d.n_fi(); return &m->flux_i;
+    case 13: *count = d.n_fj(); return &m->flux_j;
+    default: *count = 0; return nullptr;
+  }
+}
+
+int pq_model_destroy(pq_model *m) {
+  if (!m) return PQ_OK;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (double *p : {m->psi, m->rel, m->zeta, m->zeta_old, m->zfi, m->ui, m->uimo, m->flux_i,
+                    m->zfj, m->uj, m->ujmo, m->flux_j, m->res, m->res_beta, m->fw, m->fin, m->fjn,
+                    m->s1, m->s2, m->beta})
+    (void)hipFree(p);
+  (void)hipFree(m->maxu);
+  (void)hipHostFree(m->maxu_host);
+  pq_inv_destroy(m->inv);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return PQ_OK;
+}
+
+int pq_model_create(pq_model **out, const pq_model_params *p) {
+  PQ_CHECK_ARG(out && p, "pq_model_create: null argument");
+  PQ_CHECK_ARG(p->s1 && p->s2 && p->ss && p->modes_to_z && p->z_to_modes && p->beta && p->force_wind,
+               "pq_model_create: null parameter array");
+  PQ_CHECK_ARG(p->dt > 0.0, "pq_model_create: dt must be > 0");
+  pq_model *m = new pq_model();
+  m->g = p->g;
+  m->L = p->layers;
+  m->d = Dims{p->g, p->layers};
+  int rc = pq_inv_create(&m->inv, p->g, p->layers, p->ss, p->modes_to_z, p->z_to_modes, p->precision, 1);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  auto fail = [&](int code) {
+    pq_model_destroy(m);
+    return code;
+  };
+  const Dims &d = m->d;
+  struct A {
+    double **p;
+    size_t n;
+  } allocs[] = {{&m->psi, d.n_nd()},  {&m->rel, d.n_nd()},    {&m->zeta, d.n_nd()},
+                {&m->zeta_old, d.n_nd()}, {&m->zfi, d.n_fi()}, {&m->ui, d.n_fi()},
+                {&m->uimo, d.n_fi()}, {&m->flux_i, d.n_fi()}, {&m->zfj, d.n_fj()},
+                {&m->uj, d.n_fj()},   {&m->ujmo, d.n_fj()},   {&m->flux_j, d.n_fj()},
+                {&m->res, d.n_rs()},  {&m->res_beta, d.n_rs()}, {&m->fw, (size_t)p->g * p->g},
+                {&m->fin, d.n_fi()},  {&m->fjn, d.n_fj()},    {&m->s1, (size_t)p->layers},
+                {&m->s2, (size_t)p->layers}, {&m->beta, (size_t)p->layers}};
+  for (auto &a : allocs) {
+    if (hipMalloc((void **)a.p, a.n * sizeof(double)) != hipSuccess) return fail(PQ_ERR_NOMEM);
+    if (hipMemset(*a.p, 0, a.n * sizeof(double)) != hipSuccess) return fail(PQ_ERR_HIP);
+  }
+  if (hipMalloc((void **)&m->maxu, sizeof(unsigned long long)) != hipSuccess) return fail(PQ_ERR_NOMEM);
+  if (hipHostMalloc((void **)&m->maxu_host, sizeof(unsigned long long), 0) != hipSuccess)
+    return fail(PQ_ERR_NOMEM);
+  (void)hipMemcpy(m->fw, p->force_wind, sizeof(double) * p->g * p->g, hipMemcpyHostToDevice);
+  (void)hipMemcpy(m->s1, p->s1, sizeof(double) * p->layers, hipMemcpyHostToDevice);
+  (void)hipMemcpy(m->s2, p->s2, sizeof(double) * p->layers, hipMemcpyHostToDevice);
+  (void)hipMemcpy(m->beta, p->beta, sizeof(double) * p->layers, hipMemcpyHostToDevice);
+  m->stream = (hipStream_t)pq_inv_stream(m->inv);
+  m->dt = p->dt;
+  m->visc = p->viscosity;
+  m->visc_bot = p->visc_bot;
+  m->bc_type = p->bc_type;
+  m->alpha = p->alpha;
+  m->adapt = p->adapt_dt;
+  m->target_cfl = p->target_cfl;
+  m->max_dt = p->max_dt;
+  m->max_dt_inc = p->max_dt_increase;
+  *out = m;
+  return PQ_OK;
+}
+
+void *pq_model_stream(pq_model *m) { return m ? (void *)m->stream : nullptr; }
+
+static int model_invert(pq_model *m) {
+  const int ld = m->g + 2;
+  const void *z[1] = {m->zeta + 1 + ld};
+  void *p[1] = {m->psi + 1 + ld};
+  return pq_inv_run_dev(m->inv, 1, z, ld, ld, p, ld, ld, m->stream);
+}
+
+int pq_model_init(pq_model *m, const double *zeta_sdata) {
+  PQ_CHECK_ARG(m && zeta_sdata, "pq_model_init: null argument");
+  const int g = m->g, L = m->L, ld = g + 2;
+  const Dims &d = m->d;
+  std::vector<double> z(d.n_nd(), 0.0);
+  for (int l = 0; l < L; ++l)
+    for (int j = 2; j <= g - 1; ++j)
+      for (int i = 2; i <= g - 1; ++i)
+        z[d.nd(i + 1, j + 1, l)] = zeta_sdata[(size_t)(i - 1) + (size_t)g * ((j - 1) + (size_t)g * l)];
+  // zero CABARET arrays (qg3.F90:413-424)
+  for (double *p : {m->zfi, m->zfj, m->ui, m->uj, m->uimo, m->ujmo, m->flux_i, m->flux_j})
+    PQ_HIP(hipMemsetAsync(p, 0, sizeof(double) * (p == m->zfi || p == m->ui || p == m->uimo || p == m->flux_i ? d.n_fi() : d.n_fj()), m->stream));
+  PQ_HIP(hipMemsetAsync(m->res, 0, sizeof(double) * d.n_rs(), m->stream));
+  PQ_HIP(hipMemsetAsync(m->res_beta, 0, sizeof(double) * d.n_rs(), m->stream));
+  PQ_HIP(hipMemsetAsync(m->psi, 0, sizeof(double) * d.n_nd(), m->stream));
+  PQ_HIP(hipMemsetAsync(m->rel, 0, sizeof(double) * d.n_nd(), m->stream));
+  PQ_HIP(hipMemcpyAsync(m->zeta, z.data(), sizeof(double) * d.n_nd(), hipMemcpyHostToDevice, m->stream));
+  (void)ld;
+  int rc = model_invert(m);
+  if (rc) return rc;
+  rc = pq_rel_from_zeta_and_psi_dev(g, L, m->s1, m->s2, m->bc_type, m->alpha, m->zeta, m->psi, m->rel, m->stream);
+  if (rc) return rc;
+  rc = pq_update_zeta_bc_dev(g, L, m->s1, m->s2, m->rel, m->psi, m->zeta, m->stream);
+  if (rc) return rc;
+  PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, m->stream));
+  m->tw_a = 0.5;
+  m->tw_u = 0.5;
+  m->model_time = m->dt / 2.0;
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  return pq_inv_status(m->inv, nullptr);
+}
+
+// cabaret_adapt_dt (qg3p_aux.F90:946-990) with the scalar update on the host.
+static int model_adapt_dt(pq_model *m) {
+  const Dims &d = m->d;
+  PQ_HIP(hipMemsetAsync(m->maxu, 0, sizeof(unsigned long long), m->stream));
+  hipLaunchKernelGGL(k_maxabs, grid_for(m->g - 1, m->g - 1, m->L), dim3(BX, BY), 0, m->stream, d,
+                     m->ui, m->uj, m->maxu);
+  PQ_HIP(hipMemcpyAsync(m->maxu_host, m->maxu, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        m->stream));
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  double max_u;
+  std::memcpy(&max_u, m->maxu_host, sizeof(double));
+  const double old_dt = m->dt;
+  const double a = std::fabs(max_u);
+  const double spacing = a == 0.0 ? 2.2250738585072014e-308 : std::nextafter(a, INFINITY) - a;
+  const double tol = 10.0 * std::fmax(spacing, 2.220446049250313e-16);
+  if (std::fabs(max_u - 0.0) > tol) {
+    double dd = m->target_cfl / max_u;
+    dd = std::fmin(dd, std::fmin(old_dt * m->max_dt_inc, m->max_dt));
+    m->dt = dd;
+  }
+  m->tw_a = 0.5 * m->dt / old_dt;
+  m->tw_u = m->dt / (old_dt + m->dt);
+  return PQ_OK;
+}
+
+int pq_model_step(pq_model *m, int nsteps, int final_partial) {
+  PQ_CHECK_ARG(m != nullptr && nsteps >= 0, "pq_model_step: bad arguments");
+  const int g = m->g, L = m->L;
+  const Dims &d = m->d;
+  hipStream_t s = m->stream;
+  for (int it = 0; it < nsteps; ++it) {
+    if (m->adapt) {
+      int rc = model_adapt_dt(m);
+      if (rc) return rc;
+    }
+    const double dt = m->dt;
+    hipLaunchKernelGGL(k_predictor_beta, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->tw_a,
+                       m->beta, m->zeta_old, m->res, m->ui, m->zeta, m->res_beta);
+    int rc = model_invert(m);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->bc_type,
+                       m->alpha, m->zeta, m->psi, m->rel);
+    hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,
+                       m->psi, m->zeta);
+    m->model_time += 0.5 * dt;
+    hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw, m->visc,
+                       m->visc_bot, m->rel, m->zeta);
+    hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->tw_u,
+                       m->psi, m->ui, m->uj, m->uimo, m->ujmo);
+    hipLaunchKernelGGL(k_extrapolator, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, dt, m->zeta,
+                       m->zeta_old, m->ui, m->uj, m->zfi, m->zfj, m->fin, m->fjn);
+    hipLaunchKernelGGL(k_face_copy, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->fin, m->fjn,
+                       m->zfi, m->zfj);
+    hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, m->ui, m->uj, m->zfi,
+                       m->zfj, m->flux_i, m->flux_j);
+    const bool last = (it == nsteps - 1) && final_partial;
+    if (!last) {
+      // corrector into zeta_old, so that zeta_old <- zeta needs no copy; the
+      // next predictor rewrites the interior of zeta and keeps its boundary.
+      hipLaunchKernelGGL(k_corrector_to, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
+                         m->flux_i, m->flux_j, m->res, m->zeta, m->zeta_old);
+      PQ_HIP(hipMemcpyAsync(m->zeta, m->zeta_old, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+      m->model_time += 0.5 * dt;
+    }
+    rc = check_launch("pq_model_step");
+    if (rc) return rc;
+  }
+  return PQ_OK;
+}
+
+int pq_model_get(pq_model *m, int which, double *host, size_t count) {
+  PQ_CHECK_ARG(m && host, "pq_model_get: null argument");
+  size_t n = 0;
+  double **f = model_field(m, which, &n);
+  PQ_CHECK_ARG(f != nullptr, "pq_model_get: unknown field %d", which);
+  PQ_CHECK_ARG(count >= n, "pq_model_get: field %d needs %zu values, buffer holds %zu", which, n, count);
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  PQ_HIP(hipMemcpy(host, *f, n * sizeof(double), hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+int pq_model_set(pq_model *m, int which, const double *host, size_t count) {
+  PQ_CHECK_ARG(m && host, "pq_model_set: null argument");
+  size_t n = 0;
+  double **f = model_field(m, which, &n);
+  PQ_CHECK_ARG(f != nullptr, "pq_model_set: unknown field %d", which);
+  PQ_CHECK_ARG(count >= n, "pq_model_set: field %d needs %zu values, buffer holds %zu", which, n, count);
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  PQ_HIP(hipMemcpy(*f, host, n * sizeof(double), hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+int pq_model_scalars(pq_model *m, double *out4) {
+  PQ_CHECK_ARG(m && out4, "pq_model_scalars: null argument");
+  out4[0] = m->dt;
+  out4[1] = m->tw_a;
+  out4[2] = m->tw_u;
+  out4[3] = m->model_time;
+  return PQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+"""The C ABI library loads and exports every function include/pequod_hip.h
+declares (no compute calls: runs without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pequod_hip.h")
+LIB = os.path.join(ROOT, "pequod_amd", "libpequod_hip.so")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(pq_\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for must in ("pq_helm_create", "pq_helm_solve", "pq_inv_create", "pq_inv_run",
+                 "pq_inv_run_dev", "pq_cabaret_extrapolator_dev", "pq_model_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.skip("libpequod_hip.so not built")
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\s[TW]\s(pq_\w+)$", out, flags=re.M))
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_binds():
+    if not os.path.exists(LIB):
+        pytest.skip("libpequod_hip.so not built")
+    from pequod_amd import _lib
+    L = _lib.load()
+    for n in declared():
+        assert hasattr(L, n)
+    assert set(_lib.SIGNATURES) >= set(declared())
+    # pure host entry points work without a device
+    assert L.pq_last_error() is not None
+
+
+def test_supported_sizes_rejected_without_device():
+    """Argument validation happens before any device work."""
+    if not os.path.exists(LIB):
+        pytest.skip("libpequod_hip.so not built")
+    from pequod_amd import _lib
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    ss = (ctypes.c_double * 1)(0.0)
+    rc = L.pq_helm_create(ctypes.byref(h), 100, 100, 1, ss, 0)
+    assert rc == _lib.PQ_ERR_ARG
+    assert b"unsupported" in L.pq_last_error()

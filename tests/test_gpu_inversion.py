@@ -1,0 +1,171 @@
+"""Parity of the HIP Helmholtz solve and PV inversion with the FP64 oracle.
+
+The oracle restates helm / pv_inversion (qg3p_helmholtz.F90:614-769,
+qg3p_aux.F90:452-521) with FFTPACK-style sine transforms along i and the
+Thomas sweep along j; the GPU path uses sine transforms along both axes.
+Both solve the same discrete system, so FP64 results agree to round-off.
+Tolerances: FP64 max relative error 1e-12 per layer; FP32 and mixed 1e-6
+(the BASELINE's bound for the mixed-precision path).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pequod_amd import qg, strat as S, synth
+from pequod_amd._lib import PQ_ERR_NAN, PqError
+
+from helpers import from_dev, random_strat, rel_err, to_dev
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [9, 13, 17, 25, 33, 49, 65, 97, 129, 257, 769, 1025]
+
+
+@pytest.mark.parametrize("g", SIZES)
+@pytest.mark.parametrize("L", [1, 3])
+def test_helm_fp64_matches_oracle(dev, g, L):
+    print(f"helm fp64 g={g} L={L}", flush=True)
+    rng = np.random.default_rng(g * 10 + L)
+    ss = np.concatenate([[0.0], rng.uniform(0.01, 1.0, L - 1)])
+    z = np.asfortranarray(rng.standard_normal((g, g, L)))
+    ref = z.copy(order="F")
+    O.Helm(g, ss).solve(ref)
+    out = z.copy(order="F")
+    qg.HelmCache(g, g, ss).helm(out)  # host drop-in path
+    # interior to round-off, boundary semantics exactly
+    assert rel_err(out[1:-1, 1:-1], ref[1:-1, 1:-1]) < 1e-12
+    for sl in [np.s_[0, :], np.s_[-1, :], np.s_[:, 0], np.s_[:, -1]]:
+        np.testing.assert_array_equal(out[sl], ref[sl])
+
+
+@pytest.mark.parametrize("g", [33, 129])
+def test_helm_device_inplace(dev, g):
+    import torch
+    rng = np.random.default_rng(7)
+    ss = np.array([0.0, 0.3])
+    z = np.asfortranarray(rng.standard_normal((g, g, 2)))
+    ref = z.copy(order="F")
+    O.Helm(g, ss).solve(ref)
+    t = to_dev(z, dev)
+    qg.HelmCache(g, g, ss).helm(t)
+    torch.cuda.synchronize()
+    out = from_dev(t)
+    assert rel_err(out[1:-1, 1:-1], ref[1:-1, 1:-1]) < 1e-12
+    np.testing.assert_array_equal(out[0, :], ref[0, :])
+    np.testing.assert_array_equal(out[:, -1], ref[:, -1])
+
+
+@pytest.mark.parametrize("g", [17, 33, 65, 129, 257])
+@pytest.mark.parametrize("p,q", [(1, 1), (3, 5), (2, 7)])
+def test_helm_eigenfunction_closed_form(dev, g, p, q):
+    """sin(p pi i/N) sin(q pi j/N) is an exact eigenfunction of the 5-point
+    operator: psi = rhs / (lambda_p + lambda_q - ss)."""
+    N = g - 1
+    i = np.arange(g)
+    f = np.sin(np.pi * p * i / N)[:, None] * np.sin(np.pi * q * i / N)[None, :]
+    ss = np.array([0.0, 0.5])
+    z = np.asfortranarray(np.stack([f, f], axis=2))
+    qg.HelmCache(g, g, ss).helm(z)
+    O.apply_dirichlet_bc(z, 0.0)
+    lam = lambda k: -4 * np.sin(k * np.pi / (2 * N)) ** 2
+    for l in range(2):
+        exp = f / (lam(p) + lam(q) - ss[l])
+        # round-off is bounded by eps * cond * |rhs| / |smallest eigenvalue|;
+        # a wrong index or sign gives O(|exp|) errors
+        scale = np.max(np.abs(f)) / abs(2 * lam(1) - ss[l])
+        assert np.max(np.abs(z[:, :, l] - exp)) < 2e-16 * N * N * scale
+
+
+def _oracle_inv(g, st, zeta):
+    inv = O.Inversion(g, st.ss, st.modes_to_z, st.z_to_modes)
+    psi, rc = inv.run(zeta)
+    assert rc == 0
+    return psi
+
+
+@pytest.mark.parametrize("g", [9, 13, 33, 65, 97, 257, 769, 1025])
+@pytest.mark.parametrize("L", [1, 2, 3, 5, 8])
+def test_inversion_fp64_matches_oracle(dev, g, L):
+    if g > 300 and L > 3:
+        pytest.skip("large sizes covered with L <= 3")
+    print(f"pv_inversion fp64 g={g} L={L}", flush=True)
+    rng = np.random.default_rng(1000 + g + L)
+    s1, s2 = random_strat(L, rng)
+    st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
+    z = np.asfortranarray(rng.standard_normal((g, g, L)))
+    ref = _oracle_inv(g, st, z)
+    psi = np.zeros((g, g, L), order="F")
+    hc = qg.HelmholtzCache(g, st, "fp64")
+    hc.run_host(z, psi)
+    assert rel_err(psi, ref) < 1e-12
+
+
+def test_inversion_sdata_views_and_batch(dev):
+    """Device batch of 3 members passed as sdata views of (G+2)^2 arrays."""
+    import torch
+    g, L, B = 65, 3, 3
+    rng = np.random.default_rng(5)
+    s1, s2 = random_strat(L, rng)
+    st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
+    hc = qg.HelmholtzCache(g, st, "fp64", batch_max=B)
+    zs, ps, refs = [], [], []
+    for b in range(B):
+        full = np.asfortranarray(rng.standard_normal((g + 2, g + 2, L)))
+        refs.append(_oracle_inv(g, st, np.asfortranarray(full[1:-1, 1:-1])))
+        zs.append(to_dev(full, dev))
+        ps.append(torch.zeros_like(zs[-1]))
+    hc.run_dev([z[:, 1:, 1:] for z in zs], [p[:, 1:, 1:] for p in ps])
+    rc, psic = hc.status()
+    assert rc == 0
+    for b in range(B):
+        out = from_dev(ps[b])
+        assert rel_err(out[1:-1, 1:-1], refs[b]) < 1e-12
+        # halo untouched
+        assert np.all(out[0, :, :] == 0) and np.all(out[-1, :, :] == 0)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-6), ("mixed", 1e-6)])
+@pytest.mark.parametrize("cfg", ["C1"])
+def test_inversion_reduced_precision_k3(dev, prec, tol, cfg):
+    """Seeded k^-3 PV on the C1 grid: max relative psi error <= 1e-6 vs the
+    FP64 oracle, and the Helmholtz/mass constraint: every output layer has a
+    zero trapezoid mean (follows from the mass correction)."""
+    import torch
+    c = synth.config(cfg)
+    st = c.stratification()
+    q = c.q()
+    g = c.g
+    ref = _oracle_inv(g, st, q)
+    hc = qg.HelmholtzCache(g, st, prec)
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    zt = to_dev(q, dev, dt)
+    pt = torch.zeros_like(zt)
+    hc.run_dev([zt], [pt])
+    assert hc.status()[0] == 0
+    out = from_dev(pt)
+    err = rel_err(out, ref)
+    print(f"{prec} rel err {err:.3e}")
+    assert err < tol
+    for l in range(st.layers):
+        assert abs(O.mean(out[:, :, l])) < 1e-6 * np.max(np.abs(ref[:, :, l]))
+
+
+def test_inversion_nan_status(dev):
+    g, L = 33, 2
+    st = S.from_parameters(np.ones(L), [0.1], [0.2])
+    z = np.asfortranarray(np.ones((g, g, L)))
+    z[10, 10, 0] = np.nan
+    psi = np.zeros((g, g, L), order="F")
+    hc = qg.HelmholtzCache(g, st, "fp64")
+    with pytest.raises(PqError) as e:
+        hc.run_host(z, psi)
+    assert e.value.code == PQ_ERR_NAN
+    # the handle recovers for the next call
+    z[10, 10, 0] = 1.0
+    hc.run_host(z, psi)
+
+
+def test_inversion_rejects_unsupported_size(dev):
+    st = S.from_parameters(np.ones(2), [0.1], [0.2])
+    with pytest.raises(PqError):
+        qg.HelmholtzCache(100, st, "fp64")

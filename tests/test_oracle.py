@@ -1,0 +1,347 @@
+"""Pins the FP64 oracle (CPU) against the reference's own known answers
+(tests/golden/known_answers.json, transcribed from its unit tests), closed
+forms and invariants.  No GPU needed."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pequod_amd import strat as S, synth
+
+from helpers import random_strat, rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KA = json.load(open(os.path.join(HERE, "golden", "known_answers.json")))
+
+
+def _fe(a, b):
+    """fe of test_tools.F90:86-94: |a - b| <= 10 max(spacing(a), epsilon)."""
+    sp = np.spacing(abs(a)) if a != 0 else np.finfo(float).tiny
+    return abs(a - b) <= KA["fne_factor"]["value"] * max(sp, np.finfo(float).eps)
+
+
+# ------------------------------------------------------------ sine transform
+def _dst_exact(x):
+    """Brute-force DST-I in long double with exactly reduced angles."""
+    n = len(x)
+    N = n + 1
+    j = np.arange(1, n + 1)
+    out = np.zeros(n, dtype=np.longdouble)
+    for k in range(1, n + 1):
+        m = (j * k) % (2 * N)
+        out[k - 1] = 2 * np.sum(np.asarray(x, np.longdouble) *
+                                np.sin(np.pi * m.astype(np.longdouble) / N))
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 11, 15, 23, 31, 47, 63, 95, 127, 191, 255])
+def test_sint_brute_force(n):
+    x = np.random.default_rng(n).standard_normal(n)
+    X = O.sint(x)
+    ref = _dst_exact(x).astype(float)
+    assert np.max(np.abs(X - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref))) * np.log2(n + 2)
+
+
+@pytest.mark.parametrize("n", [7, 63, 767, 1023])
+def test_sint_involution(n):
+    """sint o sint = 2 (n+1) I (qg3p_helmholtz.F90:672-678 relies on it)."""
+    x = np.random.default_rng(n).standard_normal(n)
+    y = O.sint(O.sint(x)) / (2 * (n + 1))
+    assert np.max(np.abs(y - x)) < 1e-13
+
+
+def test_sint_linearity_and_basis():
+    n = 31
+    e = np.zeros(n)
+    e[4] = 1.0
+    X = O.sint(e)
+    k = np.arange(1, n + 1)
+    assert np.allclose(X, 2 * np.sin(np.pi * 5 * k / (n + 1)), atol=1e-14)
+
+
+# ------------------------------------------------------------ helmholtz
+@pytest.mark.parametrize("g", [9, 33, 129])
+@pytest.mark.parametrize("ss", [0.0, 0.7])
+def test_helm_eigenfunctions(g, ss):
+    N = g - 1
+    i = np.arange(g)
+    for p, q in [(1, 1), (2, 5), (N - 1, 3)]:
+        f = np.sin(np.pi * p * i / N)[:, None] * np.sin(np.pi * q * i / N)[None, :]
+        z = np.asfortranarray(f[:, :, None].copy())
+        O.Helm(g, [ss]).solve(z)
+        O.apply_dirichlet_bc(z, 0.0)
+        lam = lambda k: -4 * np.sin(k * np.pi / (2 * N)) ** 2
+        exp = f / (lam(p) + lam(q) - ss)
+        # round-off is bounded by eps * cond * |rhs| / |smallest eigenvalue|;
+        # a wrong index or sign gives O(|exp|) errors
+        scale = np.max(np.abs(f)) / abs(2 * lam(1) - ss)
+        assert np.max(np.abs(z[:, :, 0] - exp)) < 2e-16 * N * N * scale
+
+
+def test_helm_residual_random():
+    """Discrete residual of the 5-point Helmholtz operator on the interior."""
+    g, ss = 65, 0.3
+    rng = np.random.default_rng(0)
+    rhs = np.asfortranarray(rng.standard_normal((g, g, 1)))
+    psi = rhs.copy(order="F")
+    O.Helm(g, [ss]).solve(psi)
+    O.apply_dirichlet_bc(psi, 0.0)
+    p = psi[:, :, 0]
+    lap = p[2:, 1:-1] + p[:-2, 1:-1] + p[1:-1, 2:] + p[1:-1, :-2] - 4 * p[1:-1, 1:-1]
+    res = lap - ss * p[1:-1, 1:-1] - rhs[1:-1, 1:-1, 0]
+    assert np.max(np.abs(res)) < 1e-12 * np.max(np.abs(rhs))
+
+
+def _order_errors(ss, gs):
+    m = []
+    for g in gs:
+        N = g - 1
+        i = np.arange(1, g + 1)
+        x = (i[None, :] - 1) / N   # j index -> x
+        y = (g - i[:, None]) / N   # i index -> y
+        s = np.sin(np.pi * x) * np.sin(np.pi * y)
+        if ss == 0.0:
+            phi = -np.pi ** 2 * s * (1.0 / N ** 2 + 1.0 / N ** 2)
+        else:
+            phi = -s * (np.pi ** 2 / N ** 2 + np.pi ** 2 / N ** 2 + ss)
+        z = np.asfortranarray(phi[:, :, None])
+        O.Helm(g, [ss]).solve(z)
+        O.apply_dirichlet_bc(z, 0.0)
+        m.append(np.max(np.abs(z[:, :, 0] - s)))
+    return np.array(m)
+
+
+def test_helm_convergence_orders():
+    """The reference's order test (unittest/test_helmholtz.F90:48-136):
+    Poisson order >= 2, Helmholtz (ss = 0.5) order >= 3.9."""
+    ka = KA["helmholtz_order"]
+    gs = ka["g"][:6]
+    for ss, lim in [(0.0, ka["poisson_min_order"]), (ka["helmholtz_ss"], ka["helmholtz_min_order"])]:
+        m = _order_errors(ss, gs)
+        o = -np.log10(m[1:] / m[:-1]) / np.log10(np.array(gs[1:]) / np.array(gs[:-1]))
+        assert np.all(o >= lim), (ss, o)
+
+
+def test_laplace_bc_known_answer():
+    """helm + add_laplace_bc_rhs reproduce psi = 1 (test_laplace_bc.F90:27-44)."""
+    ka = KA["laplace_bc"]
+    g = ka["grid"]
+    phi0 = np.asfortranarray(np.ones((g, g, 1)))
+    O.Helm(g, [0.0]).solve(phi0)
+    O.apply_dirichlet_bc(phi0, 0.0)
+    phi1 = np.asfortranarray(np.ones((g, g, 1)))
+    O.add_laplace_bc_rhs(phi1, ka["bc"])
+    O.Helm(g, [0.0]).solve(phi1)
+    O.apply_dirichlet_bc(phi1, ka["bc"])
+    d = phi1 - phi0
+    assert _fe(d.min(), ka["expect_min"]) and _fe(d.max(), ka["expect_max"])
+    # laplace_bc_rhs(decomp, 1) alone: zero field with the boundary RHS
+    phi = np.asfortranarray(np.zeros((g, g, 1)))
+    O.add_laplace_bc_rhs(phi, 1.0)
+    O.Helm(g, [0.0]).solve(phi)
+    O.apply_dirichlet_bc(phi, 1.0)
+    assert np.allclose(phi, 1.0, atol=1e-14)
+
+
+# ------------------------------------------------------------ mean
+@pytest.mark.parametrize("case", KA["mean"]["cases"])
+def test_mean_known_answers(case):
+    n = case["n"]
+    a = np.full((n, n), case["fill"])
+    if case["zero_block"]:
+        i0, i1, j0, j1 = case["zero_block"]
+        a[i0 - 1:i1, j0 - 1:j1] = 0.0
+    assert _fe(O.mean(a), case["expect"])
+
+
+# ------------------------------------------------------------ stratification
+def test_stratification_known_answers():
+    ka = KA["stratification"]
+    h = ka["h"]
+    s21, s12 = O.strat_l2(ka["rd_bcl1"], h[0], h[1])
+    ss, m2z, z2m = O.strat_modes([0.0, s12], [s21, 0.0])
+    assert ss[0] == 0.0 and _fe(1.0 / np.sqrt(ss[1]), ka["two_layer"]["rd"][0])
+    assert np.allclose(z2m @ np.ones(2), ka["two_layer"]["z_ones_to_modes"], atol=1e-15)
+    a, b, c, d = O.strat_l3(ka["rd_bcl1"], ka["rd_bcl2"], *h)
+    ss, m2z, z2m = O.strat_modes([0.0, b, d], [a, c, 0.0])
+    # the oracle orders baroclinic modes by ascending ss (largest Rd first)
+    rd = sorted(1.0 / np.sqrt(ss[1:]))
+    assert all(_fe(x, y) or abs(x - y) < 1e-9 * y for x, y in zip(rd, ka["three_layer"]["rd"]))
+    assert np.allclose(z2m @ np.ones(3), ka["three_layer"]["z_ones_to_modes"], atol=1e-15)
+    assert np.allclose(m2z @ z2m, np.eye(3), atol=1e-14)
+
+
+@pytest.mark.parametrize("L", [2, 3, 5, 8])
+def test_strat_host_matches_oracle(L):
+    """pequod_amd.strat (host FP64) vs the oracle's eigen-solver: same ss,
+    and the same inversion operator M diag(.) D."""
+    rng = np.random.default_rng(L)
+    s1, s2 = random_strat(L, rng)
+    st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
+    ss, m2z, z2m = O.strat_modes(s1, s2)
+    assert np.allclose(st.ss, ss, rtol=1e-13, atol=1e-15)
+    w = rng.standard_normal(L)
+    assert np.allclose(st.modes_to_z @ np.diag(w) @ st.z_to_modes, m2z @ np.diag(w) @ z2m, atol=1e-12)
+    # S = M diag(-ss) D
+    assert np.allclose(st.modes_to_z @ np.diag(-st.ss) @ st.z_to_modes, st.s_mat, atol=1e-13)
+
+
+# ------------------------------------------------------------ PV inversion
+def _inv_case(g, L, seed):
+    rng = np.random.default_rng(seed)
+    s1, s2 = random_strat(L, rng)
+    st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
+    z = np.asfortranarray(rng.standard_normal((g, g, L)))
+    return st, z
+
+
+def test_inversion_invariants():
+    """pv_inversion: every psi layer has zero trapezoid mean; the modal
+    fields satisfy the Helmholtz equation; independent of baroclinic mode
+    order and eigenvector scaling (qg3p_diagnostics.F90:615-635)."""
+    g, L = 33, 3
+    st, z = _inv_case(g, L, 3)
+    inv = O.Inversion(g, st.ss, st.modes_to_z, st.z_to_modes)
+    psi, rc = inv.run(z)
+    assert rc == 0
+    for l in range(L):
+        assert abs(O.mean(psi[:, :, l])) < 1e-13 * np.max(np.abs(psi))
+    # permuted / rescaled modes give the same psi
+    perm = [0, 2, 1]
+    scale = np.array([1.0, -3.0, 0.25])
+    m2z = st.modes_to_z[:, perm] * scale
+    z2m = np.linalg.inv(m2z)
+    psi2, _ = O.Inversion(g, st.ss[perm], m2z, z2m).run(z)
+    assert rel_err(psi2, psi) < 1e-12
+    # layer-space equation: Delta psi_l + (S psi)_l = zeta_l - c_l on the
+    # interior, where the constants c_l come from the mass correction; so
+    # the residual is constant in space per layer
+    for l in range(L):
+        p = psi[:, :, l]
+        lap = p[2:, 1:-1] + p[:-2, 1:-1] + p[1:-1, 2:] + p[1:-1, :-2] - 4 * p[1:-1, 1:-1]
+        sp = sum(st.s_mat[l, k] * psi[1:-1, 1:-1, k] for k in range(L))
+        r = lap + sp - z[1:-1, 1:-1, l]
+        assert np.max(np.abs(r - r.mean())) < 1e-11 * np.max(np.abs(z))
+
+
+def test_inversion_barotropic_only():
+    """One layer: psi = helm_0(zeta) - mean (qg3p_diagnostics.F90:631)."""
+    g = 17
+    rng = np.random.default_rng(9)
+    z = np.asfortranarray(rng.standard_normal((g, g, 1)))
+    psi, rc = O.Inversion(g, [0.0], np.ones((1, 1)), np.ones((1, 1))).run(z)
+    h = z.copy(order="F")
+    O.Helm(g, [0.0]).solve(h)
+    O.apply_dirichlet_bc(h, 0.0)
+    assert np.allclose(psi, h - O.mean(h[:, :, 0]), atol=1e-14)
+
+
+# ------------------------------------------------------------ CABARET
+def test_cabaret_cfl_known_answers():
+    ka = KA["cabaret_cfl"]
+    g = ka["grid"]["iis"]
+    for c in ka["cases"]:
+        ui = np.full((g + 3, g + 2, 1), c["u_i"], order="F")
+        uj = np.full((g + 2, g + 3, 1), c["u_j"], order="F")
+        if "u_i_at_10_10" in c:
+            ui[9, 9, 0] = c["u_i_at_10_10"]
+        if "u_j_at_10_10" in c:
+            uj[9, 9, 0] = c["u_j_at_10_10"]
+        assert _fe(O.cabaret_cfl(c["dt"], ui, uj), c["expect"])
+
+
+def _small_model(g=33, adapt=False, wind=True):
+    c = synth.config("C1")
+    c.n = g - 2
+    nd = c.nondim()
+    st = nd["strat"]
+    fw = c.wind() if wind else np.zeros((g, g), order="F")
+    m = O.Model(g, st.s1, st.s2, st.ss, st.modes_to_z, st.z_to_modes, nd["beta"], fw,
+                viscosity=nd["viscosity"], visc_bot=nd["visc_bot"], dt=nd["dt"],
+                alpha=nd["alpha"], adapt_dt=adapt, target_cfl=0.1, max_dt=10.0)
+    return c, nd, m
+
+
+def test_cabaret_zero_state_stays_zero_without_wind():
+    c, nd, m = _small_model(wind=False)
+    m.init(np.zeros((m.g, m.g, m.layers), order="F"))
+    m.step(5)
+    for k, v in m.a.items():
+        if k != "force_wind":
+            assert np.all(v == 0.0), k
+
+
+def test_cabaret_flux_form_conservation():
+    """With beta, wind, viscosity and drag off, the corrector changes the
+    interior PV sum only through fluxes at the boundary faces
+    (manual/manual.tex:702-704)."""
+    g, L = 33, 2
+    rng = np.random.default_rng(11)
+    z = np.asfortranarray(rng.standard_normal((g + 2, g + 2, L)))
+    fli = np.asfortranarray(rng.standard_normal((g + 3, g + 2, L)))
+    flj = np.asfortranarray(rng.standard_normal((g + 2, g + 3, L)))
+    res = np.zeros((g, g, L), order="F")
+    z0 = z.copy(order="F")
+    dt = 0.37
+    O.cabaret_corrector(dt, fli, flj, res, z)
+    for l in range(L):
+        dsum = (z[2:g, 2:g, l] - z0[2:g, 2:g, l]).sum()
+        # interior nodes 3..g: faces 3 and g+1 bound the region
+        bflux = (fli[g, 2:g, l] - fli[2, 2:g, l]).sum() + (flj[2:g, g, l] - flj[2:g, 2, l]).sum()
+        assert abs(dsum + 0.5 * dt * bflux) < 1e-12 * np.abs(z).sum()
+
+
+def test_cabaret_constant_field_preserved():
+    """A spatially constant PV with uniform face values and zero
+    velocity is stationary under extrapolator + flux + corrector."""
+    g, L = 17, 1
+    z = np.full((g + 2, g + 2, L), 3.0, order="F")
+    zo = z.copy(order="F")
+    ui = np.zeros((g + 3, g + 2, L), order="F")
+    uj = np.zeros((g + 2, g + 3, L), order="F")
+    zfi = np.full((g + 3, g + 2, L), 3.0, order="F")
+    zfj = np.full((g + 2, g + 3, L), 3.0, order="F")
+    fli, flj = np.zeros_like(zfi), np.zeros_like(zfj)
+    res = np.zeros((g, g, L), order="F")
+    O.cabaret_extrapolator(0.1, z, zo, ui, uj, zfi, zfj)
+    assert np.all(zfi[2:g + 1, 2:g, :] == 3.0) and np.all(zfj[2:g, 2:g + 1, :] == 3.0)
+    O.cabaret_flux(ui, uj, zfi, zfj, fli, flj)
+    O.cabaret_corrector(0.1, fli, flj, res, z)
+    assert np.all(z == 3.0)
+
+
+def test_cabaret_uniform_advection_exact_shift():
+    """Uniform flow u = 1 along i with dt = 1 (Courant number 1).  From a
+    consistent state -- face values equal to the upwind node value and the
+    residual equal to minus the flux divergence -- one CABARET step
+    (predictor, extrapolator with limiter, flux, corrector) translates the
+    profile by exactly one node: q^{n+1/2}_k = (q_k + q_{k-1}) / 2,
+    q_f^{n+1} = 2 q^{n+1/2}_{k-1} - q_f^n = q_{k-1}, d_f = 0 so the limiter is
+    inactive, and q^{n+1}_k = q_{k-1}.  A wrong upwind side, limiter sign or
+    half-step factor breaks the exact shift."""
+    g, L = 33, 1
+    dt = 1.0
+    ui = np.zeros((g + 3, g + 2, L), order="F")
+    ui[:, 2:g, 0] = 1.0           # every i-face of the interior columns 3..g
+    uj = np.zeros((g + 2, g + 3, L), order="F")
+    prof = np.where(np.arange(1, g + 3) < 12, 1.0, 0.0)   # 1-based data index
+    z = np.asfortranarray(np.repeat(prof[:, None], g + 2, axis=1)[:, :, None])
+    zo = z.copy(order="F")
+    zfi = np.zeros((g + 3, g + 2, L), order="F")
+    zfi[1:g + 2, :, 0] = z[0:g + 1, :, 0]   # face i (1-based) = node i-1
+    zfj = np.zeros((g + 2, g + 3, L), order="F")
+    res = np.zeros((g, g, L), order="F")
+    # res(i-1, j-1) = -(f(i+1) - f(i)) = q(i-1) - q(i) for data nodes i = 3..g
+    res[1:g - 1, :, 0] = (prof[1:g - 1] - prof[2:g])[:, None]
+    fli, flj = np.zeros_like(zfi), np.zeros_like(zfj)
+    zh = z.copy(order="F")
+    O.cabaret_predictor(dt, zo, res, zh)
+    O.cabaret_extrapolator(dt, zh, zo, ui, uj, zfi, zfj)
+    O.cabaret_flux(ui, uj, zfi, zfj, fli, flj)
+    O.cabaret_corrector(dt, fli, flj, res, zh)
+    col = zh[:, g // 2, 0]
+    shifted = np.concatenate([[prof[0]], prof[:-1]])
+    assert np.array_equal(col[3:g], shifted[3:g])
+    assert np.array_equal(zfi[3:g, g // 2, 0], prof[2:g - 1])
