@@ -1,0 +1,210 @@
+"""Benchmark of the PV (modal Helmholtz) inversion on one MI355X.
+
+Metric: 3-layer 1023^2 Helmholtz inversions/s at <= 1e-6 relative psi error
+vs FP64, with p50/p99 latency.  Workload (C2 grid): 3-layer QG, interior
+1023 x 1023 (G = 1025), dx = 5 km, H = [300, 700, 3000] m,
+g' = [0.02, 0.01] m/s^2, seeded k^-3 PV with RMS 1e-5 s^-1.
+
+A step is one call of the inversion over a batch of B independent PV fields
+already resident in HBM (inputs cycle through a ring of distinct buffers
+larger than the 256 MB Infinity Cache).  value = B * steps / wall time.
+The same run also reports: the plain FP64 drop-in rate, per-pass kernel
+durations (HIP events) with the roofline of the dominant pass, single-call
+p50/p99 latency, the max relative psi error of the measured path against
+the FP64 oracle, CABARET steps/s of the device-resident FP64 model, and the
+oracle's CPU rate on the host (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--precision", default="fp64", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--ring", type=int, default=24, help="distinct input/output buffers")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cabaret", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not built)"}))
+        sys.exit(2)
+
+    import torch
+
+    from pequod_amd import qg, synth
+
+    dev = torch.device("cuda:0")
+    c = synth.config("C2")
+    st = c.stratification()
+    g, L, B = c.g, st.layers, args.batch
+    q = c.q()  # (G, G, L) Fortran order, non-dimensional
+
+    def dev_field(a, dtype):
+        return torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(dev, dtype)
+
+    def run_rate(prec, steps, warmup, batch):
+        dt = torch.float32 if prec == "fp32" else torch.float64
+        hc = qg.HelmholtzCache(g, st, prec, batch_max=batch)
+        base = dev_field(q, dt)
+        nring = max(args.ring, batch)
+        # distinct inputs: the seeded field with a per-buffer amplitude
+        zs = [base * (1.0 + 0.01 * k) for k in range(nring)]
+        ps = [torch.empty_like(base) for _ in range(nring)]
+        torch.cuda.synchronize()
+
+        def call(k):
+            idx = [(k * batch + b) % nring for b in range(batch)]
+            hc.run_dev([zs[i] for i in idx], [ps[i] for i in idx])
+
+        for k in range(warmup):
+            call(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            call(k)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rc, _ = hc.status()
+        assert rc == 0, f"inversion status {rc}"
+        return hc, zs, ps, (t1 - t0), base
+
+    log(f"[bench] C2 {args.precision} batch {B}: {args.steps} steps")
+    hc, zs, ps, wall, base = run_rate(args.precision, args.steps, args.warmup, B)
+    ms_per_step = 1000.0 * wall / args.steps
+    value = B * args.steps / wall
+
+    # per-pass kernel durations (HIP events on the handle's stream)
+    torch.cuda.synchronize()
+    passes = hc.pass_times(zs[:B], ps[:B], reps=20)
+    el = 4 if args.precision == "fp32" else 8
+    el_t = 4 if args.precision != "fp64" else 8  # transform / scratch precision
+    names = ["k_rows_fwd", "k_cols", "k_rows_inv"]
+    dom = int(np.argmax(passes))
+    # algorithmic bytes per launch: one read + one write of the L x G x G
+    # field per member (B_inv = 2 L G^2 s); the column pass moves the
+    # scratch field (n^2 per layer) once each way.
+    alg = {
+        "k_rows_fwd": B * L * g * g * el + B * L * (g - 1) ** 2 * el_t,
+        "k_cols": B * 2 * L * (g - 1) ** 2 * el_t,
+        "k_rows_inv": B * L * (g - 1) ** 2 * el_t + B * L * g * g * el,
+    }
+    ach = alg[names[dom]] / (passes[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(f"{args.precision}_b{B}", {}).get(names[dom])
+        except Exception:
+            traffic = None
+
+    # single-call latency (B = 1), HIP events on torch's stream
+    hc1, zs1, ps1, _, _ = run_rate(args.precision, 5, 5, 1)
+    lat = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for k in range(200):
+        e0.record()
+        hc1.run_dev([zs1[k % len(zs1)]], [ps1[k % len(ps1)]])
+        e1.record()
+        e1.synchronize()
+        lat.append(e0.elapsed_time(e1))
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+    # plain FP64 drop-in rate on the same workload
+    fp64_rate = None
+    if args.precision != "fp64":
+        _, _, _, w64, _ = run_rate("fp64", max(20, args.steps // 4), 5, B)
+        fp64_rate = B * max(20, args.steps // 4) / w64
+
+    # device-resident FP64 CABARET model (C2 grid): steps/s
+    cab = None
+    if not args.no_cabaret:
+        nd = c.nondim()
+        m = qg.Model(g, st, nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                     visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1, alpha=nd["alpha"])
+        m.init(q)
+        m.step(5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nst = 50
+        m.step(nst)
+        m.get("zeta")  # synchronises the model stream
+        cab = nst / (time.perf_counter() - t0)
+        m.close()
+
+    # CPU baseline (oracle, FP64, one core) on a bounded sample, and the
+    # error of the measured path against it on the same input.
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        inv = O.Inversion(g, st.ss, st.modes_to_z, st.z_to_modes)
+        ref, rc = inv.run(q)
+        assert rc == 0
+        t0 = time.perf_counter()
+        n_cpu = 0
+        while time.perf_counter() - t0 < 10.0:
+            inv.run(q)
+            n_cpu += 1
+        cpu_rate = n_cpu / (time.perf_counter() - t0)
+        cpu = {"value": round(cpu_rate, 3), "unit": "inversions/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} C2 inversions (1023^2 x 3, FP64) in ~10 s on one host core"}
+        # error of the measured path on the unscaled seeded field
+        out = torch.empty_like(base)
+        hc.run_dev([base], [out])
+        torch.cuda.synchronize()
+        o = out.double().cpu().numpy().transpose(2, 1, 0)
+        err = float(max(np.max(np.abs(o[:, :, l] - ref[:, :, l])) / np.max(np.abs(ref[:, :, l]))
+                        for l in range(L)))
+
+    line = {
+        "metric": "3-layer 1023^2 Helmholtz inversions/s at <=1e-6 rel err vs FP64; p50/p99 ms",
+        "value": round(value, 1),
+        "unit": "inversions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
+        "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
+        "config": {"workload": "C2: 3-layer QG 1023^2 interior (G=1025), dx 5 km, PV inversion",
+                   "batch_per_step": B, "precision": args.precision, "ring_buffers": max(args.ring, B)},
+        "p50_ms": round(p50, 4),
+        "p99_ms": round(p99, 4),
+        "max_rel_psi_err_vs_fp64": err,
+        "fp64_dropin_inversions_per_s": None if fp64_rate is None else round(fp64_rate, 1),
+        "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
+        "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -20,7 +20,7 @@
  *     traffic); otherwise the data is copied in and out and the call is
  *     synchronous (the plain drop-in).
  *   - Streams are passed as void* (a hipStream_t); NULL selects the handle's
- *     own stream.
+ *     own stream and PQ_STREAM_NULL the HIP null (default) stream.
  *   - Handles are independent and re-entrant: each owns its device buffers,
  *     plans and stream (the reference's caches and CABARET work arrays are
  *     `save` variables and not re-entrant, qg3p_aux.F90:1084,1185,1256).
@@ -40,6 +40,10 @@ extern "C" {
 #define PQ_ERR_NAN 3      /* NaN psiC (qg3p_aux.F90:504-509 labort)         */
 #define PQ_WARN_LARGE 4   /* |psiC| > 1e100 (qg3p_aux.F90:510-517 warning)  */
 #define PQ_ERR_NOMEM 5    /* device memory exhausted                        */
+
+/* Stream argument meaning "the HIP null stream" (NULL means the handle's
+ * own stream).                                                             */
+#define PQ_STREAM_NULL ((void *)-1)
 
 /* Arithmetic of the sine transforms.  Modal projection coefficients, the
  * eigendecomposition, psiC and the mass correction constants are FP64 in
@@ -118,6 +122,13 @@ int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1,
 int pq_inv_status(pq_inv *h, double *psic);
 /* The handle's stream (hipStream_t). */
 void *pq_inv_stream(pq_inv *h);
+/* Measurement helper (no counterpart in the reference): runs the three
+ * passes of one inversion separately, reps times each, on the handle's
+ * stream between HIP events; ms3 receives the mean duration of the row
+ * forward, column and row inverse pass in milliseconds.                   */
+int pq_inv_pass_times(pq_inv *h, int batch, const void *const *zeta,
+                      long ldz1, long ldz2, void *const *psi, long ldp1,
+                      long ldp2, int reps, float *ms3);
 
 /* ======================================================================
  * CABARET stages on device arrays (qg3p_aux.F90:917-1642 and the boundary

@@ -356,7 +356,9 @@ __global__ void k_maxabs(Dims d, const double *__restrict__ ui, const double *__
   }
 }
 
-hipStream_t S(void *s) { return (hipStream_t)s; }
+// Stage entry points have no handle: NULL and PQ_STREAM_NULL both mean the
+// null stream.
+hipStream_t S(void *s) { return pq_resolve_stream(s, (hipStream_t)0); }
 
 int check_launch(const char *what) {
   hipError_t e = hipGetLastError();
@@ -470,6 +472,9 @@ int pq_rel_from_zeta_and_psi_dev(int g, int L, const double *s1, const double *s
                                  void *stream) {
   CHECK_GL(g, L, "pq_rel_from_zeta_and_psi_dev");
   PQ_CHECK_ARG(bc_type >= 1 && bc_type <= 3, "pq_rel_from_zeta_and_psi_dev: bad bc_type %d", bc_type);
+  // one layer: set(rel, zeta) copies the whole data array, halo included
+  if (L == 1)
+    PQ_HIP(hipMemcpyAsync(rel, zeta, sizeof(double) * Dims{g, L}.n_nd(), hipMemcpyDeviceToDevice, S(stream)));
   hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
                      bc_type, alpha, zeta, psi, rel);
   return check_launch("pq_rel_from_zeta_and_psi_dev");
@@ -676,6 +681,8 @@ int pq_model_step(pq_model *m, int nsteps, int final_partial) {
                        m->beta, m->zeta_old, m->res, m->ui, m->zeta, m->res_beta);
     int rc = model_invert(m);
     if (rc) return rc;
+    if (L == 1)
+      PQ_HIP(hipMemcpyAsync(m->rel, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->bc_type,
                        m->alpha, m->zeta, m->psi, m->rel);
     hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,

@@ -33,6 +33,14 @@ void pq_set_error(const char *fmt, ...);
 // N = 3 * 2^a (12 <= N <= 3072).
 bool pq_supported_n(int n);
 
+// Stream argument of the C ABI: NULL selects the handle's own stream,
+// PQ_STREAM_NULL the HIP null (default) stream, anything else is a hipStream_t.
+inline hipStream_t pq_resolve_stream(void *s, hipStream_t own) {
+  if (s == nullptr) return own;
+  if (s == PQ_STREAM_NULL) return (hipStream_t)0;
+  return (hipStream_t)s;
+}
+
 namespace pq {
 
 template <typename T> struct cx {

@@ -148,7 +148,7 @@ k_cols(DevPlan P, const MemDesc *__restrict__ mems,
       ((sizeof(T) * ((size_t)nrows * P.RS + 2 * nfft * (P.tpf > 64 ? P.tpf / 64 : 1) + 2)) + 15) &
       ~size_t(15);
   double *red = reinterpret_cast<double *>(smem + red_off);
-  T *cD = reinterpret_cast<T *>(red + 8 * 16);  // D, M, ssN in T (L <= 8)
+  T *cD = reinterpret_cast<T *>(red + 512);  // D, M, ssN in T (L <= 8)
   T *cM = cD + 64;
   T *cS = cM + 64;
   const CoefLayout CL{L, N};
@@ -225,23 +225,25 @@ k_cols(DevPlan P, const MemDesc *__restrict__ mems,
   }
   if (md.mode != 0) return;
 
-  // Workgroup reduction of the psiC partials (FP64), then the last block of
-  // this member reduces all partials in a fixed order.
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int m = 0; m < L; ++m) {
-    double v = acc[m];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[m * 16 + wv] = v;
-  }
+  // Workgroup reduction of the psiC partials (FP64) by a fixed-order tree in
+  // LDS (any workgroup size), then the last block of this member reduces all
+  // partials in a fixed order.
+  const int nt = blockDim.x, tid = threadIdx.x;
+  int pw = 1;
+  while (pw < nt) pw <<= 1;
   __shared__ int s_last;
-  __syncthreads();
-  const int nblk = N / C;
-  if (threadIdx.x == 0) {
-    for (int m = 0; m < L; ++m) {
-      double s = 0.0;
-      for (int q = 0; q < nw; ++q) s += red[m * 16 + q];
-      md.part[(size_t)w.y * L + m] = s;
+  for (int m = 0; m < L; ++m) {
+    red[tid] = acc[m];
+    __syncthreads();
+    for (int st = pw >> 1; st > 0; st >>= 1) {
+      if (tid < st && tid + st < nt) red[tid] += red[tid + st];
+      __syncthreads();
     }
+    if (tid == 0) md.part[(size_t)w.y * L + m] = red[0];
+    __syncthreads();
+  }
+  const int nblk = N / C;
+  if (tid == 0) {
     __threadfence();
     const unsigned old = atomicAdd(md.counter, 1u);
     s_last = (old == (unsigned)(nblk - 1));
@@ -249,16 +251,26 @@ k_cols(DevPlan P, const MemDesc *__restrict__ mems,
   __syncthreads();
   if (s_last) {
     __threadfence();
-    if (threadIdx.x < L) {
-      const int m = threadIdx.x;
-      double s = 0.0;
-      for (int b = 0; b < nblk; ++b) s += vload(md.part + (size_t)b * L + m);
-      s = s / (double(N) * double(N));
-      md.psic[m] = s;
-      if (s != s) atomicOr(md.status, 1);
-      else if (fabs(s) > 1.0e100) atomicOr(md.status, 2);
+    // every thread sums a fixed strided subset of the partials, then a
+    // fixed-order tree in LDS: the reduction order never depends on timing
+    for (int m = 0; m < L; ++m) {
+      double part = 0.0;
+      for (int b = tid; b < nblk; b += nt) part += vload(md.part + (size_t)b * L + m);
+      red[tid] = part;
+      __syncthreads();
+      for (int st = pw >> 1; st > 0; st >>= 1) {
+        if (tid < st && tid + st < nt) red[tid] += red[tid + st];
+        __syncthreads();
+      }
+      if (tid == 0) {
+        const double sum = red[0] / (double(N) * double(N));
+        md.psic[m] = sum;
+        if (sum != sum) atomicOr(md.status, 1);
+        else if (fabs(sum) > 1.0e100) atomicOr(md.status, 2);
+      }
+      __syncthreads();
     }
-    if (threadIdx.x == 0) *md.counter = 0u;
+    if (tid == 0) *md.counter = 0u;
   }
 }
 
@@ -560,9 +572,9 @@ static size_t lds_rows_bytes(const pq_inv *h, int rows) {
 }
 
 static size_t lds_cols_bytes(const pq_inv *h, int rows) {
-  // rows + scan + 2, then reduction (8*16 doubles) and coefficient copies
+  // rows + scan + 2, then the reduction buffer (512 doubles) and coefficient copies
   const size_t el = h->prec ? 4 : 8;
-  return ((lds_rows_bytes(h, rows) + 15) & ~size_t(15)) + 8 * 16 * 8 + el * (64 + 64 + 8) + 16;
+  return ((lds_rows_bytes(h, rows) + 15) & ~size_t(15)) + 512 * 8 + el * (64 + 64 + 8) + 16;
 }
 
 static int inv_create(pq_inv **out, int g, int L, const double *ss,
@@ -598,6 +610,7 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss,
   const int E = choose_E(h->N);
   while (R > 2 && (R / 2) * (h->N / E) > 512) R /= 2;
   while (R > 2 && (size_t)R * (h->N + h->N / 16 + 1) * (fc ? 4 : 8) > 96 * 1024) R /= 2;
+  while (R > 2 && h->N % R != 0) R /= 2;  // tiles must cover N rows exactly
   int C = 2;
   while (C > 1 && ((L * C + 1) / 2) * (h->N / E) > 512) C /= 2;
   h->R = R;
@@ -665,11 +678,22 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss,
   return PQ_OK;
 }
 
+// Dynamic LDS above the 64 KiB default needs an explicit per-kernel limit
+// (gfx950 has 160 KiB of LDS per CU).  Any error of the attribute call is
+// cleared here; a launch that still exceeds the limit reports its own error.
+template <typename K> static void allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 64 * 1024) return;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipGetLastError();
+}
+
 template <typename Tin, typename T, int E>
 static void launch_fwd(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream_t s) {
   const DevPlan &P = h->plan.dp;
   const int threads = (h->R / 2) * P.tpf;
   const size_t lds = lds_rows_bytes(h, h->R);
+  allow_lds(k_rows_fwd<Tin, T, E>, lds);
   hipLaunchKernelGGL((k_rows_fwd<Tin, T, E>), dim3(nb * h->nwork1), dim3(threads), lds, s, P,
                      h->mems + mem0, pt, h->work1);
 }
@@ -680,6 +704,7 @@ static void launch_cols(pq_inv *h, int nb, int mem0, hipStream_t s) {
   const int nfft = (h->L * h->C + 1) / 2;
   const int threads = nfft * P.tpf;
   const size_t lds = lds_cols_bytes(h, 2 * nfft);
+  allow_lds(k_cols<T, E>, lds);
   hipLaunchKernelGGL((k_cols<T, E>), dim3(nb * h->nwork2), dim3(threads), lds, s, P, h->mems + mem0,
                      h->work2);
 }
@@ -689,33 +714,37 @@ static void launch_inv(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream_
   const DevPlan &P = h->plan.dp;
   const int threads = (h->R / 2) * P.tpf;
   const size_t lds = lds_rows_bytes(h, h->R);
+  allow_lds(k_rows_inv<T, Tout, E>, lds);
   hipLaunchKernelGGL((k_rows_inv<T, Tout, E>), dim3(nb * h->nwork1), dim3(threads), lds, s, P,
                      h->mems + mem0, pt, h->work1);
 }
 
+// pass_mask bit 0: row forward pass, bit 1: column pass, bit 2: row inverse.
 template <int E>
-static void launch_all_E(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream_t s) {
+static void launch_all_E(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream_t s,
+                         int pass_mask) {
   switch (h->prec) {
     case PQ_PREC_FP64:
-      launch_fwd<double, double, E>(h, nb, mem0, pt, s);
-      launch_cols<double, E>(h, nb, mem0, s);
-      launch_inv<double, double, E>(h, nb, mem0, pt, s);
+      if (pass_mask & 1) launch_fwd<double, double, E>(h, nb, mem0, pt, s);
+      if (pass_mask & 2) launch_cols<double, E>(h, nb, mem0, s);
+      if (pass_mask & 4) launch_inv<double, double, E>(h, nb, mem0, pt, s);
       break;
     case PQ_PREC_FP32:
-      launch_fwd<float, float, E>(h, nb, mem0, pt, s);
-      launch_cols<float, E>(h, nb, mem0, s);
-      launch_inv<float, float, E>(h, nb, mem0, pt, s);
+      if (pass_mask & 1) launch_fwd<float, float, E>(h, nb, mem0, pt, s);
+      if (pass_mask & 2) launch_cols<float, E>(h, nb, mem0, s);
+      if (pass_mask & 4) launch_inv<float, float, E>(h, nb, mem0, pt, s);
       break;
     default:
-      launch_fwd<double, float, E>(h, nb, mem0, pt, s);
-      launch_cols<float, E>(h, nb, mem0, s);
-      launch_inv<float, double, E>(h, nb, mem0, pt, s);
+      if (pass_mask & 1) launch_fwd<double, float, E>(h, nb, mem0, pt, s);
+      if (pass_mask & 2) launch_cols<float, E>(h, nb, mem0, s);
+      if (pass_mask & 4) launch_inv<float, double, E>(h, nb, mem0, pt, s);
       break;
   }
 }
 
 static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, long ldz2,
-                      void *const *psi, long ldp1, long ldp2, hipStream_t s) {
+                      void *const *psi, long ldp1, long ldp2, hipStream_t s,
+                      int pass_mask = 7) {
   PQ_CHECK_ARG(batch >= 1 && batch <= h->batch_max, "pq_inv_run_dev: batch %d outside 1..%d", batch,
                h->batch_max);
   PQ_CHECK_ARG(ldz1 >= h->g && ldz2 >= h->g && ldp1 >= h->g && ldp2 >= h->g,
@@ -736,10 +765,10 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
     pt.ldp1 = ldp1;
     pt.ldp2 = ldp2;
     switch (E) {
-      case 4: launch_all_E<4>(h, nb, b0, pt, s); break;
-      case 8: launch_all_E<8>(h, nb, b0, pt, s); break;
-      case 12: launch_all_E<12>(h, nb, b0, pt, s); break;
-      default: launch_all_E<16>(h, nb, b0, pt, s); break;
+      case 4: launch_all_E<4>(h, nb, b0, pt, s, pass_mask); break;
+      case 8: launch_all_E<8>(h, nb, b0, pt, s, pass_mask); break;
+      case 12: launch_all_E<12>(h, nb, b0, pt, s, pass_mask); break;
+      default: launch_all_E<16>(h, nb, b0, pt, s, pass_mask); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -783,8 +812,7 @@ int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1, lon
                    void *const *psi, long ldp1, long ldp2, void *stream) {
   PQ_CHECK_ARG(h != nullptr, "pq_inv_run_dev: null handle");
   PQ_CHECK_ARG(zeta && psi, "pq_inv_run_dev: null pointer table");
-  return inv_launch(h, batch, zeta, ldz1, ldz2, psi, ldp1, ldp2,
-                    stream ? (hipStream_t)stream : h->stream);
+  return inv_launch(h, batch, zeta, ldz1, ldz2, psi, ldp1, ldp2, pq_resolve_stream(stream, h->stream));
 }
 
 int pq_inv_status(pq_inv *h, double *psic) {
@@ -893,6 +921,30 @@ int pq_inv_run(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *psi, i
   return run_host(h, zeta, ldz1, ldz2, psi, ldp1, ldp2);
 }
 
+int pq_inv_pass_times(pq_inv *h, int batch, const void *const *zeta, long ldz1, long ldz2,
+                      void *const *psi, long ldp1, long ldp2, int reps, float *ms3) {
+  PQ_CHECK_ARG(h && zeta && psi && ms3 && reps >= 1, "pq_inv_pass_times: bad arguments");
+  hipEvent_t ev[2];
+  PQ_HIP(hipEventCreate(&ev[0]));
+  PQ_HIP(hipEventCreate(&ev[1]));
+  // full passes once so every pass reads valid intermediate data
+  int rc = inv_launch(h, batch, zeta, ldz1, ldz2, psi, ldp1, ldp2, h->stream);
+  for (int k = 0; k < 3 && rc == PQ_OK; ++k) {
+    PQ_HIP(hipEventRecord(ev[0], h->stream));
+    for (int r = 0; r < reps && rc == PQ_OK; ++r)
+      rc = inv_launch(h, batch, zeta, ldz1, ldz2, psi, ldp1, ldp2, h->stream, 1 << k);
+    PQ_HIP(hipEventRecord(ev[1], h->stream));
+    PQ_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    PQ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    ms3[k] = ms / reps;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  return pq_inv_status(h, nullptr);
+}
+
 // ---- helm ---------------------------------------------------------------
 int pq_helm_create(pq_helm **out, int ii, int jj, int layers, const double *ss, int precision) {
   PQ_CHECK_ARG(out != nullptr, "pq_helm_create: null handle pointer");
@@ -923,7 +975,7 @@ int pq_helm_solve_dev(pq_helm *h, void *zeta, long ld1, long ld2, void *stream) 
   const void *zt[1] = {zeta};
   void *pt[1] = {zeta};
   return inv_launch(h->inv, 1, zt, ld1, ld2, pt, ld1, ld2,
-                    stream ? (hipStream_t)stream : h->inv->stream);
+                    pq_resolve_stream(stream, h->inv->stream));
 }
 
 }  // extern "C"

@@ -29,9 +29,15 @@ def _np_ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
+PQ_STREAM_NULL = ctypes.c_void_p(-1)
+
+
 def _stream_of(t):
+    """torch's current stream on t's device as a C-ABI stream argument (the
+    default stream has handle 0, which the ABI spells PQ_STREAM_NULL)."""
     import torch
-    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+    h = torch.cuda.current_stream(t.device).cuda_stream
+    return ctypes.c_void_p(h) if h else PQ_STREAM_NULL
 
 
 def _require_cuda(*ts):
@@ -70,7 +76,10 @@ class HelmCache:
 
     def deallocate(self):
         if getattr(self, "_h", None):
-            load().pq_helm_destroy(self._h)
+            try:
+                load().pq_helm_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     __del__ = deallocate
@@ -121,6 +130,20 @@ class HelmholtzCache:
         s = _vp(stream) if stream is not None else _stream_of(z0)
         check(load().pq_inv_run_dev(self._h, b, zt, ldz1, ldz2, pt, ldp1, ldp2, s))
 
+    def pass_times(self, zetas, psis, reps: int = 20):
+        """Mean duration [ms] of the row-forward, column and row-inverse
+        passes, each timed alone between HIP events on the handle's stream."""
+        b = len(zetas)
+        zt = (_vp * b)(*[_ptr(z) for z in zetas])
+        pt = (_vp * b)(*[_ptr(p) for p in psis])
+        z0, p0 = zetas[0], psis[0]
+        ldz1, ldz2 = z0.stride(1), z0.stride(0) // z0.stride(1)
+        ldp1, ldp2 = p0.stride(1), p0.stride(0) // p0.stride(1)
+        ms = (ctypes.c_float * 3)()
+        check(load().pq_inv_pass_times(self._h, b, zt, ldz1, ldz2, pt, ldp1, ldp2, reps, ms),
+              allow_warn=True)
+        return list(ms)
+
     def status(self):
         psic = np.zeros(self.layers * self.batch_max)
         rc = load().pq_inv_status(self._h, _np_ptr(psic))
@@ -128,7 +151,10 @@ class HelmholtzCache:
 
     def deallocate(self):
         if getattr(self, "_h", None):
-            load().pq_inv_destroy(self._h)
+            try:
+                load().pq_inv_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     __del__ = deallocate
@@ -290,7 +316,10 @@ class Model:
 
     def close(self):
         if getattr(self, "_h", None):
-            load().pq_model_destroy(self._h)
+            try:
+                load().pq_model_destroy(self._h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     __del__ = close
