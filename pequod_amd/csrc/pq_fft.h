@@ -37,6 +37,20 @@ struct DevPlan {
   const double *shat;  // double[N]: sum_{i=1..n} 2 sin(pi p i / N) (0 for even p)
 };
 
+// Synchronise the threads of one FFT group.  A group of at most 64 threads
+// lives inside one wavefront and owns its LDS region, so a wavefront-scope
+// fence + wave barrier suffices; larger groups need the workgroup barrier.
+// The branch is uniform across the workgroup.
+__device__ __forceinline__ void group_sync(int tpf) {
+  if (tpf <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- radices
 template <typename T> __device__ __forceinline__ void dft2(cx<T> *v) {
   cx<T> a = v[0];
@@ -125,7 +139,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int N, int lns, int t,
 #pragma unroll
     for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(j + r * NR)];
   }
-  __syncthreads();
+  group_sync(tpf);
   const int ns = 1 << lns;
   const int step = NR >> lns;  // N / (Ns R)
 #pragma unroll
@@ -141,7 +155,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int N, int lns, int t,
 #pragma unroll
     for (int r = 0; r < R; ++r) buf[pad16(base + r * ns)] = v[b][r];
   }
-  __syncthreads();
+  group_sync(tpf);
 }
 
 template <typename T, int E>
@@ -212,7 +226,9 @@ __device__ __forceinline__ void group_excl_scan2(T &a, T &b, int t, int tpf,
 // rows a and b; it may read LDS that aliases buf (all loads happen before the
 // first store).  The results are written to rowa / rowb at padded slots
 // 0..N-1 (slot k holds X_k; slot 0 is set to 0); rowa and rowb may alias buf.
-// The caller synchronises before reading the rows.
+// The caller synchronises before reading the rows.  Synchronisation inside
+// is per FFT group: the loader may only read LDS written before the call
+// and synchronised by the caller.
 template <typename T, int E, typename Loader>
 __device__ __forceinline__ void dst_pair(cx<T> *buf, T *rowa, T *rowb,
                                          const DevPlan &P, int t, T *scan_tmp,
@@ -240,14 +256,14 @@ __device__ __forceinline__ void dst_pair(cx<T> *buf, T *rowa, T *rowb,
       yhi[e] = {t2a - t1a, t2b - t1b};
     }
   }
-  __syncthreads();
+  group_sync(tpf);
 #pragma unroll
   for (int e = 0; e < H; ++e) {
     const int k = t + e * tpf;
     buf[pad16(k)] = ylo[e];
     buf[pad16(k == 0 ? N / 2 : N - k)] = yhi[e];
   }
-  __syncthreads();
+  group_sync(tpf);
   fft_inplace<T, E>(buf, P, t);
   // Separate the two real DFTs and form the outputs; thread t owns
   // m in [t H, t H + H).
@@ -275,7 +291,7 @@ __device__ __forceinline__ void dst_pair(cx<T> *buf, T *rowa, T *rowb,
     rb[e] = sb;
   }
   group_excl_scan2(sa, sb, t, tpf, scan_tmp);
-  __syncthreads();
+  group_sync(tpf);
 #pragma unroll
   for (int e = 0; e < H; ++e) {
     const int m = t * H + e;

@@ -53,7 +53,6 @@ struct MemDesc {
   void *scr;           // L x N x N scratch (compute type)
   double *part;        // psiC partials: (N / C) x L
   double *psic;        // L (final psiC)
-  unsigned *counter;   // completion counter of k_cols
   int *status;         // sticky: 1 NaN psiC, 2 |psiC| > 1e100
   int L;
   int mode;            // 0 pv_inversion, 1 helm
@@ -83,10 +82,6 @@ struct PtrTab {
   void *psi[kMaxBatchArg];
   long long ldz1, ldz2, ldp1, ldp2;
 };
-
-__device__ __forceinline__ double vload(const double *p) {
-  return *reinterpret_cast<const volatile double *>(p);
-}
 
 template <typename T> __device__ __forceinline__ T dev_exp(T x);
 template <> __device__ __forceinline__ float dev_exp<float>(float x) { return __expf(x); }
@@ -226,12 +221,12 @@ k_cols(DevPlan P, const MemDesc *__restrict__ mems,
   if (md.mode != 0) return;
 
   // Workgroup reduction of the psiC partials (FP64) by a fixed-order tree in
-  // LDS (any workgroup size), then the last block of this member reduces all
-  // partials in a fixed order.
+  // LDS (any workgroup size).  The row-inverse kernel reduces the per-block
+  // partials of a member; the kernel boundary makes them visible without
+  // device-scope fences (which would write back the XCD's L2).
   const int nt = blockDim.x, tid = threadIdx.x;
   int pw = 1;
   while (pw < nt) pw <<= 1;
-  __shared__ int s_last;
   for (int m = 0; m < L; ++m) {
     red[tid] = acc[m];
     __syncthreads();
@@ -241,36 +236,6 @@ k_cols(DevPlan P, const MemDesc *__restrict__ mems,
     }
     if (tid == 0) md.part[(size_t)w.y * L + m] = red[0];
     __syncthreads();
-  }
-  const int nblk = N / C;
-  if (tid == 0) {
-    __threadfence();
-    const unsigned old = atomicAdd(md.counter, 1u);
-    s_last = (old == (unsigned)(nblk - 1));
-  }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
-    // every thread sums a fixed strided subset of the partials, then a
-    // fixed-order tree in LDS: the reduction order never depends on timing
-    for (int m = 0; m < L; ++m) {
-      double part = 0.0;
-      for (int b = tid; b < nblk; b += nt) part += vload(md.part + (size_t)b * L + m);
-      red[tid] = part;
-      __syncthreads();
-      for (int st = pw >> 1; st > 0; st >>= 1) {
-        if (tid < st && tid + st < nt) red[tid] += red[tid + st];
-        __syncthreads();
-      }
-      if (tid == 0) {
-        const double sum = red[0] / (double(N) * double(N));
-        md.psic[m] = sum;
-        if (sum != sum) atomicOr(md.status, 1);
-        else if (fabs(sum) > 1.0e100) atomicOr(md.status, 2);
-      }
-      __syncthreads();
-    }
-    if (tid == 0) *md.counter = 0u;
   }
 }
 
@@ -290,19 +255,51 @@ k_rows_inv(DevPlan P, const MemDesc *__restrict__ mems, PtrTab ptrs,
   T *scan_tmp = lds + (size_t)R * P.RS + 2 * f * (P.tpf > 64 ? P.tpf / 64 : 1);
   const CoefLayout CL{L, N};
 
-  // Correction constants (FP64).
+  // psiC_m: reduce the column kernel's per-block partials in a fixed order
+  // (every workgroup of the member does the same reduction), then the
+  // correction constants (FP64).
   double kappa = 0.0;
   double beta[8];
   if (md.mode == 0) {
+    double *red = reinterpret_cast<double *>(
+        smem + ((sizeof(T) * ((size_t)R * P.RS + 2 * (blockDim.x >> P.lgtpf) *
+                               (P.tpf > 64 ? P.tpf / 64 : 1) + 2) + 15) & ~size_t(15)));
+    double *s_psic = red + 512;
+    const int nt = blockDim.x, tid = threadIdx.x, nblk = N / P.C;
+    int pw = 1;
+    while (pw < nt) pw <<= 1;
+    for (int m = 0; m < L; ++m) {
+      double part = 0.0;
+      for (int b = tid; b < nblk; b += nt) part += md.part[(size_t)b * L + m];
+      red[tid] = part;
+      __syncthreads();
+      for (int st = pw >> 1; st > 0; st >>= 1) {
+        if (tid < st && tid + st < nt) red[tid] += red[tid + st];
+        __syncthreads();
+      }
+      if (tid == 0) s_psic[m] = red[0] / (double(N) * double(N));
+      __syncthreads();
+    }
+    if (w.z == 0 && l == 0 && tid == 0) {
+      int st = 0;
+      for (int m = 0; m < L; ++m) {
+        const double v = s_psic[m];
+        md.psic[m] = v;
+        if (v != v) st |= 1;
+        else if (fabs(v) > 1.0e100) st |= 2;
+      }
+      if (st) atomicOr(md.status, st);
+    }
     const double *M = md.coef + CL.M();
     const double *ca = md.coef + CL.a();
     const double *cb = md.coef + CL.b();
-    kappa = vload(md.psic);
+    kappa = s_psic[0];
     for (int m = 1; m < L; ++m) {
-      const double pc = vload(md.psic + m);
+      const double pc = s_psic[m];
       kappa += M[l + L * m] * pc * ca[m];
       beta[m] = M[l + L * m] * pc * cb[m];
     }
+    __syncthreads();
   }
 
   // Transposed tile load: row r <- S[l][p][j0 + r].
@@ -497,7 +494,6 @@ struct pq_inv {
   double *coef = nullptr;
   void *scr = nullptr;
   double *part = nullptr, *psic = nullptr;
-  unsigned *counter = nullptr;
   int *status = nullptr;
   MemDesc *mems = nullptr;
   int4 *work1 = nullptr, *work2 = nullptr, *work3 = nullptr;
@@ -638,9 +634,7 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss,
   const int nblk2 = N / C;
   if (hipMalloc(&h->part, sizeof(double) * (size_t)nblk2 * L * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMalloc(&h->psic, sizeof(double) * (size_t)L * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
-  if (hipMalloc(&h->counter, sizeof(unsigned) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMalloc(&h->status, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
-  (void)hipMemset(h->counter, 0, sizeof(unsigned) * batch_max);
   (void)hipMemset(h->status, 0, sizeof(int) * batch_max);
   (void)hipMemset(h->psic, 0, sizeof(double) * (size_t)L * batch_max);
   std::vector<MemDesc> md(batch_max);
@@ -649,7 +643,6 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss,
     md[b].scr = (char *)h->scr + scr_member * b;
     md[b].part = h->part + (size_t)nblk2 * L * b;
     md[b].psic = h->psic + (size_t)L * b;
-    md[b].counter = h->counter + b;
     md[b].status = h->status + b;
     md[b].L = L;
     md[b].mode = mode;
@@ -713,7 +706,8 @@ template <typename T, typename Tout, int E>
 static void launch_inv(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream_t s) {
   const DevPlan &P = h->plan.dp;
   const int threads = (h->R / 2) * P.tpf;
-  const size_t lds = lds_rows_bytes(h, h->R);
+  // tile rows + scan scratch, then the psiC reduction buffer and psiC
+  const size_t lds = ((lds_rows_bytes(h, h->R) + 15) & ~size_t(15)) + 512 * 8 + 8 * 8;
   allow_lds(k_rows_inv<T, Tout, E>, lds);
   hipLaunchKernelGGL((k_rows_inv<T, Tout, E>), dim3(nb * h->nwork1), dim3(threads), lds, s, P,
                      h->mems + mem0, pt, h->work1);
@@ -794,7 +788,6 @@ int pq_inv_destroy(pq_inv *h) {
   (void)hipFree(h->scr);
   (void)hipFree(h->part);
   (void)hipFree(h->psic);
-  (void)hipFree(h->counter);
   (void)hipFree(h->status);
   (void)hipFree(h->mems);
   (void)hipFree(h->work1);
