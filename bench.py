@@ -44,6 +44,10 @@ def main():
     ap.add_argument("--ring", type=int, default=24, help="distinct input/output buffers")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cabaret", action="store_true")
+    ap.add_argument("--no-latency", action="store_true",
+                    help="skip the B=1 latency leg (every inversion launch then has the bench's batch)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 10 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -55,6 +59,15 @@ def main():
     from pequod_amd import qg, synth
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 10 copies of {args.copy_probe} MiB")
+        return
     c = synth.config("C2")
     st = c.stratification()
     g, L, B = c.g, st.layers, args.batch
@@ -119,16 +132,18 @@ def main():
             traffic = None
 
     # single-call latency (B = 1), HIP events on torch's stream
-    hc1, zs1, ps1, _, _ = run_rate(args.precision, 5, 5, 1)
-    lat = []
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for k in range(200):
-        e0.record()
-        hc1.run_dev([zs1[k % len(zs1)]], [ps1[k % len(ps1)]])
-        e1.record()
-        e1.synchronize()
-        lat.append(e0.elapsed_time(e1))
-    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    p50 = p99 = None
+    if not args.no_latency:
+        hc1, zs1, ps1, _, _ = run_rate(args.precision, 5, 5, 1)
+        lat = []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for k in range(200):
+            e0.record()
+            hc1.run_dev([zs1[k % len(zs1)]], [ps1[k % len(ps1)]])
+            e1.record()
+            e1.synchronize()
+            lat.append(e0.elapsed_time(e1))
+        p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
     # plain FP64 drop-in rate on the same workload
     fp64_rate = None
@@ -192,8 +207,8 @@ def main():
         "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
         "config": {"workload": "C2: 3-layer QG 1023^2 interior (G=1025), dx 5 km, PV inversion",
                    "batch_per_step": B, "precision": args.precision, "ring_buffers": max(args.ring, B)},
-        "p50_ms": round(p50, 4),
-        "p99_ms": round(p99, 4),
+        "p50_ms": None if p50 is None else round(p50, 4),
+        "p99_ms": None if p99 is None else round(p99, 4),
         "max_rel_psi_err_vs_fp64": err,
         "fp64_dropin_inversions_per_s": None if fp64_rate is None else round(fp64_rate, 1),
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},

@@ -122,6 +122,13 @@ int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1,
 int pq_inv_status(pq_inv *h, double *psic);
 /* The handle's stream (hipStream_t). */
 void *pq_inv_stream(pq_inv *h);
+/* Tile shapes (no counterpart in the reference; used by the autotuner):
+ * rows per tile of the row passes (also the scratch block height, a power
+ * of two in 2..16 dividing g-1) and columns per workgroup of the column pass
+ * (a power of two in 1..8).  Rejected shapes return PQ_ERR_ARG and leave the
+ * handle unchanged.  Results do not depend on the tiles.                  */
+int pq_inv_set_tiles(pq_inv *h, int rows, int cols);
+int pq_inv_get_tiles(pq_inv *h, int *rows, int *cols);
 /* Measurement helper (no counterpart in the reference): runs the three
  * passes of one inversion separately, reps times each, on the handle's
  * stream between HIP events; ms3 receives the mean duration of the row

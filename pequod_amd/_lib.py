@@ -39,6 +39,8 @@ SIGNATURES = {
     "pq_inv_run_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _vp]),
     "pq_inv_status": (_i, [_vp, _dp]),
     "pq_inv_stream": (_vp, [_vp]),
+    "pq_inv_set_tiles": (_i, [_vp, _i, _i]),
+    "pq_inv_get_tiles": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "pq_inv_pass_times": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _i,
                                ctypes.POINTER(ctypes.c_float)]),
     "pq_cabaret_cfl_dev": (_i, [_i, _i, _d, _vp, _vp, _dp, _vp]),

@@ -25,6 +25,11 @@ SOURCES = [
     "pq_cabaret.hip",
 ]
 
+# Transform lengths N = g - 1 with a kernel unit each (pq_inv_kernels.hip
+# compiled with -DPQ_N=N); must match PQ_INV_SIZES in csrc/pq_inv.h.
+INV_SIZES = [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024,
+             1536, 2048, 3072, 4096]
+
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -40,10 +45,14 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: str) -> str:
+def _compile(job) -> str:
+    src, n = job if isinstance(job, tuple) else (job, None)
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    stem = os.path.splitext(src)[0] + ("" if n is None else f"_n{n}")
+    obj = os.path.join(BUILD, stem + ".o")
     cmd = [_hipcc(), *HIP_FLAGS, "-c", path, "-o", obj]
+    if n is not None:
+        cmd.insert(1, f"-DPQ_N={n}")
     if src.endswith(".cpp"):
         cmd.insert(1, "-x")
         cmd.insert(2, "hip")
@@ -57,9 +66,12 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     """Compile every source and link libpequod_hip.so; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with cf.ThreadPoolExecutor(max_workers=min(len(srcs), 4)) as ex:
-        objs = list(ex.map(_compile, srcs))
+    jobs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    # largest units first so that the pool stays busy
+    jobs += [("pq_inv_kernels.hip", n) for n in sorted(INV_SIZES, reverse=True)]
+    workers = max(1, min(len(jobs), (os.cpu_count() or 4), 8))
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1,48 +1,92 @@
-// pq_fft.h -- LDS-resident FFT and DST-I building blocks for gfx950.
+// pq_fft.h -- LDS-resident FFT and DST-I building blocks for gfx950, with the
+// transform length a compile-time constant.
 //
-// One complex FFT of length N is computed by a group of TPF = N / E threads
-// (E complex elements per thread, E in {4, 8, 12, 16}).  Stages are Stockham
-// radix-R passes (R in {16, 8, 4, 2} and a final radix 3 for N = 3 * 2^a)
-// run in place: every thread loads all of its inputs into registers, the
-// workgroup synchronises, then results are written back.  LDS addresses are
-// padded with one element per 16 (pad16) to avoid bank conflicts for both
-// strided and thread-blocked accesses.
+// One complex FFT of length M is computed by a group of TPF = M / E threads
+// (E complex elements per thread).  Stages are Stockham radix-R passes
+// (R in {16, 8, 4, 2}, plus a final radix 3 when M = 3 * 2^a) run in place:
+// every thread loads its inputs into registers, the group synchronises, and
+// the results are written back.  LDS element addresses are padded with one
+// element per 16 (pad16) so that strided and thread-blocked accesses spread
+// over the banks.  With the plan known at compile time every loop unrolls
+// and most LDS offsets fold into immediates.
 //
-// dst_pair computes the unnormalised DST-I
-//     X_k = 2 sum_{j=1..n} x_j sin(pi j k / N),  N = n + 1,
-// of two real rows at once, following FFTPACK sint's formulation
-// (qg3p_helmholtz.F90:1380-1426): a pre-twiddle into a length-N real
-// sequence y, a real DFT of y (two rows packed as the real and imaginary
-// parts of one complex FFT), and a prefix sum for the odd outputs.
+// Two DST-I formulations, both computing the unnormalised
+//     X_k = 2 sum_{j=1..N-1} x_j sin(pi j k / N)
+// of two real rows a and b at once:
+//   * DstSint (FP64): FFTPACK sint's formulation (qg3p_helmholtz.F90:
+//     1380-1426): pre-twiddle into a length-N real sequence, one complex FFT
+//     of length N for both rows, and a prefix sum for the odd outputs.  The
+//     sum accumulates ~sqrt(N) eps relative to the FFT's output scale, which
+//     is harmless in FP64 but exceeds 1e-6 in FP32.
+//   * DstExt (FP32): the odd extension w (w_{2N-j} = -w_j) of a + i b, one
+//     complex FFT of length 2N, X_a = -Im W, X_b = Re W (the transform of a
+//     real odd sequence is imaginary).  No recurrence: ~eps log N accuracy.
 #pragma once
+
+#include <type_traits>
 
 #include "pq_common.h"
 
 namespace pq {
 
-struct DevPlan {
-  int N;      // FFT length, N = n + 1
-  int tpf;    // threads per FFT group = N / E
-  int lgtpf;  // log2(tpf)
-  int nst;    // number of FFT stages
-  int rad[8]; // radix of each stage
-  int lns[8]; // log2 of the Stockham span Ns before each stage
-  int R;      // rows per tile (row kernels)
-  int lgR;
-  int C;      // columns per tile (column kernel)
-  int RS;     // LDS row stride in elements of the compute type
-  const void *tw;    // cx<T>[N]: exp(-2 pi i k / N)
-  const void *was;   // T[N/2]:   2 sin(k pi / N)
-  const void *lam4;  // T[N]:     (2N)^2 * (-4 sin^2(p pi / (2N)))
-  const double *shat;  // double[N]: sum_{i=1..n} 2 sin(pi p i / N) (0 for even p)
+// ------------------------------------------------------------ plans
+constexpr int ilog2c(int x) {
+  int r = 0;
+  while ((1 << r) < x) ++r;
+  return r;
+}
+
+// Elements per thread for an FFT of length M in type T.
+template <typename T> constexpr int choose_E(int M) {
+  if (M % 3 == 0) return M >= 768 ? 24 : 12;
+  if (std::is_same<T, float>::value && M >= 2048) return 32;
+  if (M >= 512) return 16;
+  if (M >= 64) return 8;
+  return 4;
+}
+
+struct StageArr {
+  int nst;
+  int rad[12];
+  int lns[12];
+};
+
+constexpr StageArr make_stages(int M, int E) {
+  StageArr a{};
+  int rem = (M % 3 == 0) ? M / 3 : M;
+  int lg = 0;
+  a.nst = 0;
+  while (rem > 1) {
+    int r = 2;
+    if (16 <= E && E % 16 == 0 && rem % 16 == 0) r = 16;
+    else if (8 <= E && E % 8 == 0 && rem % 8 == 0) r = 8;
+    else if (4 <= E && E % 4 == 0 && rem % 4 == 0) r = 4;
+    a.rad[a.nst] = r;
+    a.lns[a.nst] = lg;
+    ++a.nst;
+    lg += ilog2c(r);
+    rem /= r;
+  }
+  if (M % 3 == 0) {
+    a.rad[a.nst] = 3;
+    a.lns[a.nst] = lg;
+    ++a.nst;
+  }
+  return a;
+}
+
+template <typename T, int M> struct FftPlan {
+  static constexpr int E = choose_E<T>(M);
+  static constexpr int TPF = M / E;
+  static constexpr StageArr S = make_stages(M, E);
 };
 
 // Synchronise the threads of one FFT group.  A group of at most 64 threads
 // lives inside one wavefront and owns its LDS region, so a wavefront-scope
-// fence + wave barrier suffices; larger groups need the workgroup barrier.
-// The branch is uniform across the workgroup.
-__device__ __forceinline__ void group_sync(int tpf) {
-  if (tpf <= 64) {
+// fence + wave barrier suffices; larger groups need the workgroup barrier
+// (reached by every group of the workgroup in lockstep).
+template <int TPF> __device__ __forceinline__ void group_sync() {
+  if constexpr (TPF <= 64) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -106,7 +150,7 @@ template <typename T> __device__ __forceinline__ void dft16(cx<T> *v) {
   // exp(-2 pi i k / 16), k = 0..7
   const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
   const T h = T(0.70710678118654752440);
-  const cx<T> w[8] = {{T(1), T(0)}, {c1, -s1}, {h, -h}, {s1, -c1},
+  const cx<T> w[8] = {{T(1), T(0)},  {c1, -s1},  {h, -h},  {s1, -c1},
                       {T(0), T(-1)}, {-s1, -c1}, {-h, -h}, {-c1, -s1}};
 #pragma unroll
   for (int k = 1; k < 8; ++k) o[k] = cmul(o[k], w[k]);
@@ -125,181 +169,232 @@ template <typename T, int R> __device__ __forceinline__ void dftR(cx<T> *v) {
   else dft16(v);
 }
 
-// One in-place Stockham stage of radix R over the group's buffer.
-template <typename T, int E, int R>
-__device__ __forceinline__ void fft_stage(cx<T> *buf, int N, int lns, int t,
-                                          int tpf,
-                                          const cx<T> *__restrict__ tw) {
+// One in-place Stockham stage of radix R: for j < M/R, k = j mod NS, the
+// inputs j + r M/R are twiddled by exp(-2 pi i r k / (NS R)), transformed,
+// and written to (j - k) R + k + r NS.
+template <typename T, int M, int E, int R, int LNS>
+__device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const cx<T> *__restrict__ tw) {
+  constexpr int TPF = M / E;
   constexpr int NB = E / R;
-  const int NR = N / R;
+  constexpr int NR = M / R;
+  constexpr int NS = 1 << LNS;
+  constexpr int STEP = NR >> LNS;  // M / (NS R)
   cx<T> v[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int j = t + b * tpf;
+    const int j = t + b * TPF;
 #pragma unroll
     for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(j + r * NR)];
   }
-  group_sync(tpf);
-  const int ns = 1 << lns;
-  const int step = NR >> lns;  // N / (Ns R)
+  group_sync<TPF>();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int j = t + b * tpf;
-    const int k = j & (ns - 1);
-    if (lns > 0) {
+    const int j = t + b * TPF;
+    const int k = j & (NS - 1);
+    if constexpr (LNS > 0) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * step]);
+      for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * STEP]);
     }
     dftR<T, R>(v[b]);
-    const int base = ((j >> lns) << lns) * R + k;
+    const int base = (j - k) * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[pad16(base + r * ns)] = v[b][r];
+    for (int r = 0; r < R; ++r) buf[pad16(base + r * NS)] = v[b][r];
   }
-  group_sync(tpf);
+  group_sync<TPF>();
 }
 
-template <typename T, int E>
-__device__ __forceinline__ void fft_inplace(cx<T> *buf, const DevPlan &P,
-                                            int t) {
-  const cx<T> *tw = reinterpret_cast<const cx<T> *>(P.tw);
-  for (int s = 0; s < P.nst; ++s) {
-    const int R = P.rad[s];
-    if constexpr (E % 16 == 0) {
-      if (R == 16) { fft_stage<T, E, 16>(buf, P.N, P.lns[s], t, P.tpf, tw); continue; }
-    }
-    if constexpr (E % 8 == 0) {
-      if (R == 8) { fft_stage<T, E, 8>(buf, P.N, P.lns[s], t, P.tpf, tw); continue; }
-    }
-    if constexpr (E % 4 == 0) {
-      if (R == 4) { fft_stage<T, E, 4>(buf, P.N, P.lns[s], t, P.tpf, tw); continue; }
-    }
-    if constexpr (E % 3 == 0) {
-      if (R == 3) { fft_stage<T, E, 3>(buf, P.N, P.lns[s], t, P.tpf, tw); continue; }
-    }
-    if constexpr (E % 2 == 0) {
-      if (R == 2) { fft_stage<T, E, 2>(buf, P.N, P.lns[s], t, P.tpf, tw); continue; }
-    }
+template <typename T, int M, int S>
+__device__ __forceinline__ void fft_stages(cx<T> *buf, int t, const cx<T> *__restrict__ tw) {
+  using P = FftPlan<T, M>;
+  if constexpr (S < P::S.nst) {
+    fft_stage<T, M, P::E, P::S.rad[S], P::S.lns[S]>(buf, t, tw);
+    fft_stages<T, M, S + 1>(buf, t, tw);
   }
 }
 
-// Exclusive prefix sum of (a, b) over the tpf threads of one FFT group.
-// tmp: per-group scratch of 2 * max(1, tpf / 64) elements.  Must be reached
-// by every thread of the workgroup (it may synchronise).
-template <typename T>
-__device__ __forceinline__ void group_excl_scan2(T &a, T &b, int t, int tpf,
-                                                 T *tmp) {
-  const int width = tpf < 64 ? tpf : 64;
-  const int lane = t & (width - 1);
+// Exclusive prefix sum of (a, b) over the TPF threads of one FFT group.
+// tmp: per-group scratch of 2 * (TPF / 64) elements when TPF > 64 (then the
+// workgroup synchronises, so every thread must reach this call).
+template <typename T, int TPF>
+__device__ __forceinline__ void group_excl_scan2(T &a, T &b, int t, T *tmp) {
+  constexpr int W = TPF < 64 ? TPF : 64;
+  const int lane = t & (W - 1);
   T ia = a, ib = b;
-  for (int d = 1; d < width; d <<= 1) {
-    T oa = __shfl_up(ia, d, width);
-    T ob = __shfl_up(ib, d, width);
+#pragma unroll
+  for (int d = 1; d < W; d <<= 1) {
+    T oa = __shfl_up(ia, d, W);
+    T ob = __shfl_up(ib, d, W);
     if (lane >= d) {
       ia += oa;
       ib += ob;
     }
   }
-  T ea = __shfl_up(ia, 1, width);
-  T eb = __shfl_up(ib, 1, width);
+  T ea = __shfl_up(ia, 1, W);
+  T eb = __shfl_up(ib, 1, W);
   if (lane == 0) {
     ea = T(0);
     eb = T(0);
   }
-  if (tpf > 64) {
+  if constexpr (TPF > 64) {
     const int w = t >> 6;
     if (lane == 63) {
       tmp[2 * w] = ia;
       tmp[2 * w + 1] = ib;
     }
     __syncthreads();
-    for (int q = 0; q < w; ++q) {
-      ea += tmp[2 * q];
-      eb += tmp[2 * q + 1];
-    }
+#pragma unroll
+    for (int q = 0; q < TPF / 64 - 1; ++q)
+      if (q < w) {
+        ea += tmp[2 * q];
+        eb += tmp[2 * q + 1];
+      }
     __syncthreads();
   }
   a = ea;
   b = eb;
 }
 
-// DST-I of two real rows.  ld(k, xa, xb) returns element k (1 <= k <= n) of
-// rows a and b; it may read LDS that aliases buf (all loads happen before the
-// first store).  The results are written to rowa / rowb at padded slots
-// 0..N-1 (slot k holds X_k; slot 0 is set to 0); rowa and rowb may alias buf.
-// The caller synchronises before reading the rows.  Synchronisation inside
-// is per FFT group: the loader may only read LDS written before the call
-// and synchronised by the caller.
-template <typename T, int E, typename Loader>
-__device__ __forceinline__ void dst_pair(cx<T> *buf, T *rowa, T *rowb,
-                                         const DevPlan &P, int t, T *scan_tmp,
-                                         Loader ld) {
-  constexpr int H = E / 2;
-  const int N = P.N, tpf = P.tpf;
-  const T *was = reinterpret_cast<const T *>(P.was);
-  cx<T> ylo[H], yhi[H];
+// ------------------------------------------------------------ DST engines
+// Interface: a group's LDS region of Region elements of T holds the two rows
+// a = region[0 .. RS) and b = region[RS .. 2 RS) (padded slots pad16(k),
+// k < N) on output: slot k = X_k, slot 0 = 0.  The loader ld(k, xa, xb),
+// called for 1 <= k <= N - 1 (and k = N/2 by DstSint), may read LDS that
+// aliases the region: all loads happen before the first store.  Only
+// group-level synchronisation happens inside; callers synchronise the
+// workgroup around cross-group LDS traffic.
+
+// Row stride (elements of T) of a padded row of N slots.
+template <int N> constexpr int row_stride() { return N + N / 16 + 1; }
+
+template <typename T_, int N_> struct DstSint {
+  using T = T_;
+  static constexpr int N = N_;
+  static constexpr int M = N;
+  using P = FftPlan<T, M>;
+  static constexpr int TPF = P::TPF;
+  static constexpr int E = P::E;
+  static constexpr int RS = row_stride<N>();
+  static constexpr int Region = 2 * RS;
+  static constexpr int ScanTmp = TPF > 64 ? 2 * (TPF / 64) : 0;
+
+  template <typename Loader>
+  __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp,
+                                             const cx<T> *__restrict__ tw,
+                                             const T *__restrict__ was, Loader ld) {
+    constexpr int H = E / 2;  // TPF * H = N / 2
+    cx<T> *buf = reinterpret_cast<cx<T> *>(region);
+    cx<T> ylo[H], yhi[H];
 #pragma unroll
-  for (int e = 0; e < H; ++e) {
-    const int k = t + e * tpf;
-    if (k == 0) {
-      T xa, xb;
-      ld(N / 2, xa, xb);
-      ylo[e] = {T(0), T(0)};
-      yhi[e] = {T(4) * xa, T(4) * xb};
-    } else {
-      T xka, xkb, xca, xcb;
-      ld(k, xka, xkb);
-      ld(N - k, xca, xcb);
-      const T w = was[k];
-      const T t1a = xka - xca, t2a = w * (xka + xca);
-      const T t1b = xkb - xcb, t2b = w * (xkb + xcb);
-      ylo[e] = {t1a + t2a, t1b + t2b};
-      yhi[e] = {t2a - t1a, t2b - t1b};
+    for (int e = 0; e < H; ++e) {
+      const int k = t + e * TPF;
+      if (k == 0) {
+        T xa, xb;
+        ld(N / 2, xa, xb);
+        ylo[e] = {T(0), T(0)};
+        yhi[e] = {T(4) * xa, T(4) * xb};
+      } else {
+        T xka, xkb, xca, xcb;
+        ld(k, xka, xkb);
+        ld(N - k, xca, xcb);
+        const T w = was[k];
+        const T t1a = xka - xca, t2a = w * (xka + xca);
+        const T t1b = xkb - xcb, t2b = w * (xkb + xcb);
+        ylo[e] = {t1a + t2a, t1b + t2b};
+        yhi[e] = {t2a - t1a, t2b - t1b};
+      }
+    }
+    group_sync<TPF>();
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      const int k = t + e * TPF;
+      buf[pad16(k)] = ylo[e];
+      buf[pad16(k == 0 ? N / 2 : N - k)] = yhi[e];
+    }
+    group_sync<TPF>();
+    fft_stages<T, M, 0>(buf, t, tw);
+    T ra[H], ia[H], rb[H], ib[H];
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      const int m = t * H + e;
+      const cx<T> ck = buf[pad16(m)];
+      const cx<T> cc = buf[pad16(m == 0 ? 0 : N - m)];
+      ra[e] = T(0.5) * (ck.x + cc.x);
+      ia[e] = T(0.5) * (ck.y - cc.y);
+      rb[e] = T(0.5) * (ck.y + cc.y);
+      ib[e] = T(0.5) * (cc.x - ck.x);
+    }
+    if (t == 0) {
+      ra[0] *= T(0.5);
+      rb[0] *= T(0.5);
+    }
+    T sa = T(0), sb = T(0);
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      sa += ra[e];
+      sb += rb[e];
+      ra[e] = sa;
+      rb[e] = sb;
+    }
+    group_excl_scan2<T, TPF>(sa, sb, t, scan_tmp);
+    group_sync<TPF>();
+    T *rowa = region, *rowb = region + RS;
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      const int m = t * H + e;
+      rowa[pad16(2 * m + 1)] = ra[e] + sa;
+      rowb[pad16(2 * m + 1)] = rb[e] + sb;
+      rowa[pad16(2 * m)] = m == 0 ? T(0) : -ia[e];
+      rowb[pad16(2 * m)] = m == 0 ? T(0) : -ib[e];
     }
   }
-  group_sync(tpf);
+};
+
+template <typename T_, int N_> struct DstExt {
+  using T = T_;
+  static constexpr int N = N_;
+  static constexpr int M = 2 * N;
+  using P = FftPlan<T, M>;
+  static constexpr int TPF = P::TPF;
+  static constexpr int E = P::E;
+  static constexpr int RS = row_stride<N>();
+  static constexpr int Region = 2 * (M + M / 16 + 1);
+  static constexpr int ScanTmp = 0;
+
+  template <typename Loader>
+  __device__ __forceinline__ static void run(T *region, int t, T *,
+                                             const cx<T> *__restrict__ tw,
+                                             const T *__restrict__, Loader ld) {
+    constexpr int H = E / 2;  // TPF * H = N
+    cx<T> *buf = reinterpret_cast<cx<T> *>(region);
+    cx<T> v[H];
 #pragma unroll
-  for (int e = 0; e < H; ++e) {
-    const int k = t + e * tpf;
-    buf[pad16(k)] = ylo[e];
-    buf[pad16(k == 0 ? N / 2 : N - k)] = yhi[e];
-  }
-  group_sync(tpf);
-  fft_inplace<T, E>(buf, P, t);
-  // Separate the two real DFTs and form the outputs; thread t owns
-  // m in [t H, t H + H).
-  T ra[H], ia[H], rb[H], ib[H];
+    for (int e = 0; e < H; ++e) {
+      const int k = t + e * TPF;
+      T xa = T(0), xb = T(0);
+      if (k != 0) ld(k, xa, xb);
+      v[e] = {xa, xb};
+    }
+    group_sync<TPF>();
 #pragma unroll
-  for (int e = 0; e < H; ++e) {
-    const int m = t * H + e;
-    const cx<T> ck = buf[pad16(m)];
-    const cx<T> cc = buf[pad16(m == 0 ? 0 : N - m)];
-    ra[e] = T(0.5) * (ck.x + cc.x);
-    ia[e] = T(0.5) * (ck.y - cc.y);
-    rb[e] = T(0.5) * (ck.y + cc.y);
-    ib[e] = T(0.5) * (cc.x - ck.x);
-  }
-  if (t == 0) {
-    ra[0] *= T(0.5);
-    rb[0] *= T(0.5);
-  }
-  T sa = T(0), sb = T(0);
+    for (int e = 0; e < H; ++e) {
+      const int k = t + e * TPF;
+      buf[pad16(k)] = v[e];  // w_k (w_0 = 0)
+      // w_{2N-k} = -w_k; the k = 0 thread writes w_N = 0 instead
+      buf[pad16(k == 0 ? N : M - k)] = k == 0 ? cx<T>{T(0), T(0)} : cx<T>{-v[e].x, -v[e].y};
+    }
+    group_sync<TPF>();
+    fft_stages<T, M, 0>(buf, t, tw);
 #pragma unroll
-  for (int e = 0; e < H; ++e) {
-    sa += ra[e];
-    sb += rb[e];
-    ra[e] = sa;
-    rb[e] = sb;
-  }
-  group_excl_scan2(sa, sb, t, tpf, scan_tmp);
-  group_sync(tpf);
+    for (int e = 0; e < H; ++e) v[e] = buf[pad16(t + e * TPF)];
+    group_sync<TPF>();
+    T *rowa = region, *rowb = region + RS;
 #pragma unroll
-  for (int e = 0; e < H; ++e) {
-    const int m = t * H + e;
-    rowa[pad16(2 * m + 1)] = ra[e] + sa;
-    rowb[pad16(2 * m + 1)] = rb[e] + sb;
-    rowa[pad16(2 * m)] = m == 0 ? T(0) : -ia[e];
-    rowb[pad16(2 * m)] = m == 0 ? T(0) : -ib[e];
+    for (int e = 0; e < H; ++e) {
+      const int k = t + e * TPF;
+      rowa[pad16(k)] = k == 0 ? T(0) : -v[e].y;
+      rowb[pad16(k)] = k == 0 ? T(0) : v[e].x;
+    }
   }
-}
+};
 
 }  // namespace pq

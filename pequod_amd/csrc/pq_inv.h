@@ -1,0 +1,94 @@
+// pq_inv.h -- types shared by the inversion kernels (pq_inv_kernels.hip,
+// compiled once per transform length) and the host side (pq_inversion.hip).
+#pragma once
+
+#include "pq_common.h"
+
+namespace pq {
+
+// Per-member descriptor (static part, built when a handle is created).
+struct MemDesc {
+  const double *coef;  // coefficient block, CoefLayout
+  void *scr;           // L x N x N scratch in the compute type, tile-blocked
+  double *part;        // psiC partials of the column pass: (N / C) x L
+  double *psic;        // L (final psiC of the last run)
+  int *status;         // sticky: 1 NaN psiC, 2 |psiC| > 1e100
+  int L;
+  int mode;            // 0 pv_inversion, 1 helm
+};
+
+// Coefficient block of one member (doubles):
+//   D[L*L]   z_to_modes, D(m,l) at m + L*l
+//   M[L*L]   modes_to_z, M(l,m) at l + L*m
+//   ss[L], a[L], b[L], ssN[L] = (2N)^2 ss
+//   tab[(L-1) * N/2 * 3]: (rho, gamma, eta) for mode m >= 1 (0-based) and
+//                         odd p = 2h + 1
+struct CoefLayout {
+  int L, N;
+  __host__ __device__ int D() const { return 0; }
+  __host__ __device__ int M() const { return L * L; }
+  __host__ __device__ int ss() const { return 2 * L * L; }
+  __host__ __device__ int a() const { return 2 * L * L + L; }
+  __host__ __device__ int b() const { return 2 * L * L + 2 * L; }
+  __host__ __device__ int ssN() const { return 2 * L * L + 3 * L; }
+  __host__ __device__ int tab() const { return 2 * L * L + 4 * L; }
+  __host__ __device__ size_t size() const {
+    return (size_t)tab() + (size_t)(L > 1 ? L - 1 : 0) * (N / 2) * 3;
+  }
+};
+
+// Member arrays of one launch (passed by value).
+constexpr int kMaxBatchArg = 64;
+struct PtrTab {
+  const void *zeta[kMaxBatchArg];
+  void *psi[kMaxBatchArg];
+  long long ldz1, ldz2, ldp1, ldp2;
+};
+
+// Launch-wide arguments of the three passes.
+struct InvArgs {
+  const MemDesc *mems;   // member descriptors, indexed by work.x
+  const int4 *work;      // one entry per workgroup
+  const void *tw;        // FFT twiddles of the engine: cx<T>[M], exp(-2 pi i k / M)
+  const void *was;       // DstSint pre-twiddle weights: T[N/2], 2 sin(k pi / N)
+  const void *lam4;      // T[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
+  const double *shat;    // double[N]: sum_i 2 sin(pi p i / N) (0 for even p)
+  int R, lgR;            // rows per tile of the row passes (scratch block height)
+  int C, lgC;            // columns per workgroup of the column pass
+};
+
+// Static properties of a transform engine (per precision and length).
+struct EngineInfo {
+  int M;             // complex FFT length
+  int tpf;           // threads per FFT group (one group = two rows)
+  int region;        // LDS elements of T per group
+  int scan;          // LDS elements of T of scan scratch per group
+  int elem;          // sizeof(T)
+};
+
+// Supported transform lengths N = 2^a (8..4096) and 3 * 2^a (12..3072); one
+// kernel unit (pq_inv_kernels.hip built with -DPQ_N=N) per length.
+#define PQ_INV_SIZES(X)                                                        \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384)   \
+  X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
+
+// Entry points of a kernel unit.  prec: PQ_PREC_*; pass: 0 row forward,
+// 1 column, 2 row inverse.
+#define PQ_INV_DECL(n)                                                         \
+  EngineInfo inv_info_##n(int prec);                                           \
+  hipError_t inv_launch_##n(int prec, int pass, const InvArgs &a,              \
+                            const PtrTab &pt, int blocks, int threads,         \
+                            size_t lds, hipStream_t s);
+PQ_INV_SIZES(PQ_INV_DECL)
+#undef PQ_INV_DECL
+
+// LDS bytes of a pass for `groups` FFT groups.
+inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups) {
+  size_t b = (size_t)e.elem * groups * (e.region + e.scan);
+  b = (b + 15) & ~size_t(15);
+  if (pass == 1) b += 512 * 8 + (size_t)e.elem * (64 + 64 + 8);
+  if (pass == 2) b += 512 * 8 + 8 * 8;
+  return b;
+}
+
+}  // namespace pq

@@ -1,0 +1,99 @@
+#include "debug.h"
+
+! qg_helmholtz_hip -- drop-in replacement of the reference's qg_helmholtz
+! module (qg3p_helmholtz.F90) on one MI355X.  It exports the same four names
+! that qg_aux imports (qg3p_aux.F90:13-17):
+!
+!   use qg_helmholtz_hip, only : solve_helmholtz => helm, &
+!     & helmholtz_solver_cache => helm_cache
+!   use qg_helmholtz_hip, only : allocate, deallocate
+!
+! allocate(helm_c, decomp, ss) / helm(helm_c, zeta) / deallocate(helm_c) keep
+! the reference's signatures and semantics: helm solves
+! (Delta_5 - ss_l) psi_l = zeta_l in place with homogeneous Dirichlet values
+! and the reference's output boundary values (qg3p_helmholtz.F90:614-769).
+! Errors abort through the reference's labort (debug.h).  One domain, closed
+! basin (decomp%n == 1, not periodic in j), G - 1 = 2^a or 3 * 2^a.
+module qg_helmholtz_hip
+
+  use iso_c_binding
+  use pequod_hip_c
+  use qg_parallel, only : decomposition, pabort
+
+  implicit none
+
+  private
+
+  integer, parameter :: real_8 = kind(1.0D0)
+
+  public :: helm
+  public :: helm_cache
+  public :: allocate, deallocate
+
+  type helm_cache
+    type(c_ptr) :: h = c_null_ptr
+    type(decomposition), pointer :: decomp => null()
+  end type helm_cache
+
+  interface allocate
+    module procedure allocate_helm_c
+  end interface allocate
+
+  interface deallocate
+    module procedure deallocate_helm_c
+  end interface deallocate
+
+contains
+
+  ! Allocate a helm cache (qg3p_helmholtz.F90:292-386).
+  subroutine allocate_helm_c(helm_c, decomp, ss)
+    type(helm_cache), intent(out) :: helm_c
+    type(decomposition), target, intent(in) :: decomp
+    real(kind = real_8), dimension(decomp%layers), intent(in) :: ss
+
+    integer(c_int) :: rc
+
+    if (decomp%n /= 1) then
+      labort("qg_helmholtz_hip: one domain only (run with one MPI process)")
+    end if
+    if (decomp%periodic_j) then
+      labort("qg_helmholtz_hip: closed basin only")
+    end if
+    helm_c%decomp => decomp
+    rc = pq_helm_create(helm_c%h, int(decomp%ii, c_int), int(decomp%jj, c_int), &
+      & int(decomp%layers, c_int), ss, PQ_PREC_FP64)
+    if (rc /= PQ_OK) then
+      labort("qg_helmholtz_hip: " // pq_error_message())
+    end if
+  end subroutine allocate_helm_c
+
+  ! Deallocate a helm cache (qg3p_helmholtz.F90:389).
+  subroutine deallocate_helm_c(helm_c)
+    type(helm_cache), intent(inout) :: helm_c
+
+    integer(c_int) :: rc
+
+    rc = pq_helm_destroy(helm_c%h)
+    helm_c%h = c_null_ptr
+    nullify(helm_c%decomp)
+  end subroutine deallocate_helm_c
+
+  ! On input zeta is the RHS, on output the solution (qg3p_helmholtz.F90:614).
+  subroutine helm(helm_c, zeta)
+    type(helm_cache), intent(inout) :: helm_c
+    real(kind = real_8), dimension(:, :, :), contiguous, target, intent(inout) :: zeta
+
+    integer(c_int) :: rc
+
+    assert(size(zeta, 1) == helm_c%decomp%ii)
+    assert(size(zeta, 2) == helm_c%decomp%jj)
+    assert(size(zeta, 3) == helm_c%decomp%layers)
+
+    rc = pq_helm_solve(helm_c%h, zeta, int(size(zeta, 1), c_int), &
+      & int(size(zeta, 2), c_int))
+    if (rc /= PQ_OK) then
+      labort("qg_helmholtz_hip: " // pq_error_message())
+    end if
+  end subroutine helm
+
+end module qg_helmholtz_hip

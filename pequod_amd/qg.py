@@ -144,6 +144,16 @@ class HelmholtzCache:
               allow_warn=True)
         return list(ms)
 
+    def set_tiles(self, rows: int, cols: int) -> bool:
+        """Select the tile shape (rows per row-pass tile, columns per
+        column-pass workgroup); False (handle unchanged) if not supported."""
+        return load().pq_inv_set_tiles(self._h, rows, cols) == _lib.PQ_OK
+
+    def tiles(self):
+        r, c = ctypes.c_int(), ctypes.c_int()
+        check(load().pq_inv_get_tiles(self._h, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
     def status(self):
         psic = np.zeros(self.layers * self.batch_max)
         rc = load().pq_inv_status(self._h, _np_ptr(psic))
