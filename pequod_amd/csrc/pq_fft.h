@@ -38,7 +38,7 @@ constexpr int ilog2c(int x) {
 
 // Elements per thread for an FFT of length M in type T.
 template <typename T> constexpr int choose_E(int M) {
-  if (M % 3 == 0) return M >= 768 ? 24 : 12;
+  if (M % 3 == 0) return (M >= 768 && std::is_same<T, float>::value) ? 24 : 12;
   if (std::is_same<T, float>::value && M >= 2048) return 32;
   if (M >= 512) return 16;
   if (M >= 64) return 8;
@@ -169,16 +169,53 @@ template <typename T, int R> __device__ __forceinline__ void dftR(cx<T> *v) {
   else dft16(v);
 }
 
+// Base twiddles of a thread: for every stage with LNS > 0 and every block b
+// of the thread (j = t + b TPF, k = j mod NS), w = exp(-2 pi i k / (NS R)),
+// loaded once per kernel from an FP64 table (tw64[n] = exp(-2 pi i n / M)).
+// The stage multiplies input r by w^r, formed by FP64 recurrence (relative
+// error <= (R - 1) eps_64, then rounded once to T), so the stages issue no
+// memory loads for twiddles.
+template <typename T, int M> struct TwBase {
+  using P = FftPlan<T, M>;
+  static constexpr int offset(int S) {
+    int c = 0;
+    for (int s = 0; s < S; ++s)
+      if (P::S.lns[s] > 0) c += P::E / P::S.rad[s];
+    return c;
+  }
+  static constexpr int NW = offset(P::S.nst) > 0 ? offset(P::S.nst) : 1;
+  cx<double> w[NW];
+
+  template <int S>
+  __device__ __forceinline__ void load_stage(int t, const cx<double> *__restrict__ tw64) {
+    if constexpr (S < P::S.nst) {
+      constexpr int R = P::S.rad[S], LNS = P::S.lns[S];
+      if constexpr (LNS > 0) {
+        constexpr int NB = P::E / R, NS = 1 << LNS, STEP = (M / R) >> LNS;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int k = (t + b * P::TPF) & (NS - 1);
+          w[offset(S) + b] = tw64[k * STEP];
+        }
+      }
+      load_stage<S + 1>(t, tw64);
+    }
+  }
+  __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64) {
+    load_stage<0>(t, tw64);
+  }
+};
+
 // One in-place Stockham stage of radix R: for j < M/R, k = j mod NS, the
 // inputs j + r M/R are twiddled by exp(-2 pi i r k / (NS R)), transformed,
 // and written to (j - k) R + k + r NS.
-template <typename T, int M, int E, int R, int LNS>
-__device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const cx<T> *__restrict__ tw) {
-  constexpr int TPF = M / E;
+template <typename T, int M, int S>
+__device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const TwBase<T, M> &tb) {
+  using P = FftPlan<T, M>;
+  constexpr int E = P::E, TPF = P::TPF, R = P::S.rad[S], LNS = P::S.lns[S];
   constexpr int NB = E / R;
   constexpr int NR = M / R;
   constexpr int NS = 1 << LNS;
-  constexpr int STEP = NR >> LNS;  // M / (NS R)
   cx<T> v[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -192,8 +229,16 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const cx<T> *__rest
     const int j = t + b * TPF;
     const int k = j & (NS - 1);
     if constexpr (LNS > 0) {
+      cx<double> w1 = tb.w[TwBase<T, M>::offset(S) + b];
+      // opaque copy: keeps the compiler from keeping every power of w alive
+      // between two transforms of one kernel (register spills)
+      asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+      cx<double> wr = w1;
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * k * STEP]);
+      for (int r = 1; r < R; ++r) {
+        v[b][r] = cmul(v[b][r], cx<T>{T(wr.x), T(wr.y)});
+        if (r + 1 < R) wr = cmul(wr, w1);
+      }
     }
     dftR<T, R>(v[b]);
     const int base = (j - k) * R + k;
@@ -204,11 +249,11 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const cx<T> *__rest
 }
 
 template <typename T, int M, int S>
-__device__ __forceinline__ void fft_stages(cx<T> *buf, int t, const cx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_stages(cx<T> *buf, int t, const TwBase<T, M> &tb) {
   using P = FftPlan<T, M>;
   if constexpr (S < P::S.nst) {
-    fft_stage<T, M, P::E, P::S.rad[S], P::S.lns[S]>(buf, t, tw);
-    fft_stages<T, M, S + 1>(buf, t, tw);
+    fft_stage<T, M, S>(buf, t, tb);
+    fft_stages<T, M, S + 1>(buf, t, tb);
   }
 }
 
@@ -277,11 +322,24 @@ template <typename T_, int N_> struct DstSint {
   static constexpr int Region = 2 * RS;
   static constexpr int ScanTmp = TPF > 64 ? 2 * (TPF / 64) : 0;
 
+  static constexpr int H = E / 2;  // TPF * H = N / 2
+
+  // Per-thread constants, loaded once per kernel: base twiddles and the
+  // pre-twiddle weights was[k] = 2 sin(k pi / N) of the thread's k.
+  struct Ctx {
+    TwBase<T, M> tw;
+    T was[H];
+    __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64,
+                                         const T *__restrict__ was_g) {
+      tw.load(t, tw64);
+#pragma unroll
+      for (int e = 0; e < H; ++e) was[e] = was_g[t + e * TPF];
+    }
+  };
+
   template <typename Loader>
-  __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp,
-                                             const cx<T> *__restrict__ tw,
-                                             const T *__restrict__ was, Loader ld) {
-    constexpr int H = E / 2;  // TPF * H = N / 2
+  __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                             Loader ld) {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
     cx<T> ylo[H], yhi[H];
 #pragma unroll
@@ -296,7 +354,7 @@ template <typename T_, int N_> struct DstSint {
         T xka, xkb, xca, xcb;
         ld(k, xka, xkb);
         ld(N - k, xca, xcb);
-        const T w = was[k];
+        const T w = cx_.was[e];
         const T t1a = xka - xca, t2a = w * (xka + xca);
         const T t1b = xkb - xcb, t2b = w * (xkb + xcb);
         ylo[e] = {t1a + t2a, t1b + t2b};
@@ -311,7 +369,7 @@ template <typename T_, int N_> struct DstSint {
       buf[pad16(k == 0 ? N / 2 : N - k)] = yhi[e];
     }
     group_sync<TPF>();
-    fft_stages<T, M, 0>(buf, t, tw);
+    fft_stages<T, M, 0>(buf, t, cx_.tw);
     T ra[H], ia[H], rb[H], ib[H];
 #pragma unroll
     for (int e = 0; e < H; ++e) {
@@ -360,11 +418,18 @@ template <typename T_, int N_> struct DstExt {
   static constexpr int Region = 2 * (M + M / 16 + 1);
   static constexpr int ScanTmp = 0;
 
+  static constexpr int H = E / 2;  // TPF * H = N
+
+  struct Ctx {
+    TwBase<T, M> tw;
+    __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64,
+                                         const T *__restrict__) {
+      tw.load(t, tw64);
+    }
+  };
+
   template <typename Loader>
-  __device__ __forceinline__ static void run(T *region, int t, T *,
-                                             const cx<T> *__restrict__ tw,
-                                             const T *__restrict__, Loader ld) {
-    constexpr int H = E / 2;  // TPF * H = N
+  __device__ __forceinline__ static void run(T *region, int t, T *, const Ctx &cx_, Loader ld) {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
     cx<T> v[H];
 #pragma unroll
@@ -383,7 +448,7 @@ template <typename T_, int N_> struct DstExt {
       buf[pad16(k == 0 ? N : M - k)] = k == 0 ? cx<T>{T(0), T(0)} : cx<T>{-v[e].x, -v[e].y};
     }
     group_sync<TPF>();
-    fft_stages<T, M, 0>(buf, t, tw);
+    fft_stages<T, M, 0>(buf, t, cx_.tw);
 #pragma unroll
     for (int e = 0; e < H; ++e) v[e] = buf[pad16(t + e * TPF)];
     group_sync<TPF>();

@@ -6,17 +6,6 @@
 
 namespace pq {
 
-// Per-member descriptor (static part, built when a handle is created).
-struct MemDesc {
-  const double *coef;  // coefficient block, CoefLayout
-  void *scr;           // L x N x N scratch in the compute type, tile-blocked
-  double *part;        // psiC partials of the column pass: (N / C) x L
-  double *psic;        // L (final psiC of the last run)
-  int *status;         // sticky: 1 NaN psiC, 2 |psiC| > 1e100
-  int L;
-  int mode;            // 0 pv_inversion, 1 helm
-};
-
 // Coefficient block of one member (doubles):
 //   D[L*L]   z_to_modes, D(m,l) at m + L*l
 //   M[L*L]   modes_to_z, M(l,m) at l + L*m
@@ -45,16 +34,25 @@ struct PtrTab {
   long long ldz1, ldz2, ldp1, ldp2;
 };
 
-// Launch-wide arguments of the three passes.
+// Launch-wide arguments of the three passes.  Member b of a launch (b < nb)
+// uses the scratch at scr + b * scr_stride bytes (L x N x N elements of the
+// compute type, tile-blocked), the column pass's psiC partials at
+// part + b * part_stride ((N / C) x L), psic + b * L and status + b.
 struct InvArgs {
-  const MemDesc *mems;   // member descriptors, indexed by work.x
-  const int4 *work;      // one entry per workgroup
-  const void *tw;        // FFT twiddles of the engine: cx<T>[M], exp(-2 pi i k / M)
-  const void *was;       // DstSint pre-twiddle weights: T[N/2], 2 sin(k pi / N)
-  const void *lam4;      // T[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
-  const double *shat;    // double[N]: sum_i 2 sin(pi p i / N) (0 for even p)
-  int R, lgR;            // rows per tile of the row passes (scratch block height)
-  int C, lgC;            // columns per workgroup of the column pass
+  char *scr;
+  long long scr_stride;
+  double *part;
+  int part_stride;
+  double *psic;      // L per member: psiC of the last run
+  int *status;       // sticky per member: 1 NaN psiC, 2 |psiC| > 1e100
+  const double *coef;  // coefficient block shared by the members (CoefLayout)
+  const void *tw64;  // cx<double>[M]: exp(-2 pi i n / M), M the engine's FFT length
+  const void *was;   // T[N/2]: 2 sin(k pi / N) (DstSint)
+  const void *lam4;  // T[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
+  const double *shat;  // double[N]: sum_i 2 sin(pi p i / N) (0 for even p)
+  int L, mode;       // mode 0 pv_inversion, 1 helm
+  int R, lgR;        // rows per tile of the row passes (scratch block height)
+  int C, lgC;        // columns per workgroup of the column pass
 };
 
 // Static properties of a transform engine (per precision and length).
@@ -86,8 +84,8 @@ PQ_INV_SIZES(PQ_INV_DECL)
 inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups) {
   size_t b = (size_t)e.elem * groups * (e.region + e.scan);
   b = (b + 15) & ~size_t(15);
-  if (pass == 1) b += 512 * 8 + (size_t)e.elem * (64 + 64 + 8);
-  if (pass == 2) b += 512 * 8 + 8 * 8;
+  if (pass == 1) b += 64 * 8 + (size_t)e.elem * (64 + 64 + 8);
+  if (pass == 2) b += 64 * 8 + 8 * 8;
   return b;
 }
 

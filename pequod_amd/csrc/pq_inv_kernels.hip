@@ -58,91 +58,140 @@ __device__ __forceinline__ typename Eng::T *row_ptr(typename Eng::T *lds, int r)
   return lds + (r >> 1) * Eng::Region + (r & 1) * Eng::RS;
 }
 
+// Sum of v[0..n) over the workgroup in a fixed order (shuffle tree inside
+// each wavefront, then the wavefronts in order); every thread receives the
+// sums.  red: >= 64 doubles of LDS.  All threads of the workgroup call it.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], int n, double *red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nact = min(64, (int)blockDim.x - 64 * w);
+#pragma unroll
+  for (int m = 0; m < NV; ++m)
+    if (m < n) {
+      double x = v[m];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const double y = __shfl_down(x, d, 64);
+        if (lane + d < nact) x += y;
+      }
+      v[m] = x;
+    }
+  if (lane == 0)
+    for (int m = 0; m < n; ++m) red[w * 8 + m] = v[m];
+  __syncthreads();
+  const int nw = ((int)blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int m = 0; m < NV; ++m)
+    if (m < n) {
+      double x = 0.0;
+      for (int q = 0; q < nw; ++q) x += red[q * 8 + m];
+      v[m] = x;
+    }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------ K1
+// Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
 template <typename Eng, typename Tin>
 __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
-  constexpr int N = Eng::N, TPF = Eng::TPF;
+  using VT = typename V16<T>::type;
+  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<T>::n;
+  constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int4 w = a.work[blockIdx.x];
-  const MemDesc md = a.mems[w.x];
-  const int R = a.R, lgR = a.lgR;
-  const int l = w.y, jb = w.z, j0 = jb << lgR;
+  const int R = a.R, lgR = a.lgR, NB = N >> lgR;
+  const int per_member = a.L * NB;
+  const int b = blockIdx.x / per_member, rem = blockIdx.x - b * per_member;
+  const int l = rem / NB, jb = rem - l * NB, j0 = jb << lgR;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
-  const int ngrp = R >> 1;
+  const int ngrp = R >> 1, nt = ngrp * TPF;
+  typename Eng::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
-  const Tin *z = static_cast<const Tin *>(ptrs.zeta[w.w]) + (size_t)l * ptrs.ldz1 * ptrs.ldz2;
+  const Tin *z = static_cast<const Tin *>(ptrs.zeta[b]) + (size_t)l * ptrs.ldz1 * ptrs.ldz2;
   const int ja = j0 + 2 * f;
   const Tin *za = z + (size_t)ja * ptrs.ldz1;
   const Tin *zb = za + ptrs.ldz1;
   const bool va = ja > 0;  // node row 0 is the boundary: not transformed
-  Eng::run(region, t, scan_tmp, static_cast<const cx<T> *>(a.tw), static_cast<const T *>(a.was),
-           [&](int k, T &xa, T &xb) {
-             xa = va ? T(za[k]) : T(0);
-             xb = T(zb[k]);
-           });
+  Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
+    xa = va ? T(za[k]) : T(0);
+    xb = T(zb[k]);
+  });
   __syncthreads();
   // Block (l, jb): S[p][r] = row r at p, r fastest; one contiguous N*R run.
-  using VT = typename V16<T>::type;
-  constexpr int V = V16<T>::n;
-  T *out = static_cast<T *>(md.scr) + ((size_t)(l * (N >> lgR) + jb) * N << lgR);
-  const int nvec = (N << lgR) / V;
-  for (int iv = threadIdx.x; iv < nvec; iv += blockDim.x) {
-    const int e0 = iv * V, r0 = e0 & (R - 1), p = e0 >> lgR;
+  T *out = reinterpret_cast<T *>(a.scr + b * a.scr_stride) + ((size_t)(l * NB + jb) * N << lgR);
+#pragma unroll
+  for (int s = 0; s < NVT; ++s) {
+    const int e0 = (threadIdx.x + s * nt) * V, r0 = e0 & (R - 1), p = e0 >> lgR;
     VT v;
 #pragma unroll
-    for (int s = 0; s < V; ++s) v[s] = row_ptr<Eng>(lds, r0 + s)[pad16(p)];
+    for (int q = 0; q < V; ++q) v[q] = row_ptr<Eng>(lds, r0 + q)[pad16(p)];
     *reinterpret_cast<VT *>(out + e0) = v;
   }
 }
 
 // ------------------------------------------------------------------ K2
+// Grid: nb * (N / C) workgroups of ceil(C L / 2) * TPF threads.
 template <typename Eng>
 __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   using T = typename Eng::T;
-  constexpr int N = Eng::N, TPF = Eng::TPF;
+  using VT = typename V16<T>::type;
+  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<T>::n;
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int4 w = a.work[blockIdx.x];
-  const MemDesc md = a.mems[w.x];
-  const int L = md.L, C = a.C, lgC = a.lgC, R = a.R, lgR = a.lgR;
-  const int p0 = w.y << lgC;
-  const int ngrp = blockDim.x / TPF;
+  const int L = a.L, C = a.C, lgC = a.lgC, R = a.R, lgR = a.lgR;
+  const int per_member = N >> lgC;
+  const int b = blockIdx.x / per_member, pblk = blockIdx.x - b * per_member;
+  const int p0 = pblk << lgC;
+  const int ngrp = blockDim.x / TPF, nt = blockDim.x;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int nrows = L << lgC;  // row rho = l * C + c
+  typename Eng::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
   const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
   double *red = reinterpret_cast<double *>(smem + red_off);
-  T *cD = reinterpret_cast<T *>(red + 512);  // D, M, ssN in T (L <= 8)
+  T *cD = reinterpret_cast<T *>(red + 64);  // D, M, ssN in T (L <= 8)
   T *cM = cD + 64;
   T *cS = cM + 64;
   const CoefLayout CL{L, N};
-  for (int i = threadIdx.x; i < L * L; i += blockDim.x) {
-    cD[i] = T(md.coef[CL.D() + i]);
-    cM[i] = T(md.coef[CL.M() + i]);
+  // coefficients: loads issued now, stored to LDS after the forward transform
+  double cd0 = 0.0, cm0 = 0.0, cs0 = 0.0;
+  if ((int)threadIdx.x < L * L) {
+    cd0 = a.coef[CL.D() + threadIdx.x];
+    cm0 = a.coef[CL.M() + threadIdx.x];
   }
-  for (int i = threadIdx.x; i < L; i += blockDim.x) cS[i] = T(md.coef[CL.ssN() + i]);
+  if ((int)threadIdx.x < L) cs0 = a.coef[CL.ssN() + threadIdx.x];
 
-  // Element j of row (l, p) sits at ((l * NB + j / R) * N + p) * R + j % R.
-  // Layer l is N * N elements: N / R blocks of N * R.
-  T *scr = static_cast<T *>(md.scr);
+  // Element j of row (l, p) sits at l N^2 + (j / R) N R + p R + j % R.
+  T *scr = reinterpret_cast<T *>(a.scr + b * a.scr_stride);
   const int ra = 2 * f, rb = ra + 1;
   const int la = ra >> lgC, pa = p0 + (ra & (C - 1));
   const int lb = rb >> lgC, pb = p0 + (rb & (C - 1));
   const bool va = ra < nrows && pa >= 1;
   const bool vb = rb < nrows && pb >= 1;
-  const T *sa = scr + (size_t)la * N * N + (size_t)pa * R;  // + (j / R) * N * R + j % R
+  const T *sa = scr + (size_t)la * N * N + (size_t)pa * R;
   const T *sb = scr + (size_t)lb * N * N + (size_t)pb * R;
   const int blk = N << lgR;
-  Eng::run(region, t, scan_tmp, static_cast<const cx<T> *>(a.tw), static_cast<const T *>(a.was),
-           [&](int k, T &xa, T &xb) {
-             const int off = (k >> lgR) * blk + (k & (R - 1));
-             xa = va ? sa[off] : T(0);
-             xb = vb ? sb[off] : T(0);
-           });
+  Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
+    const int off = (k >> lgR) * blk + (k & (R - 1));
+    xa = va ? sa[off] : T(0);
+    xb = vb ? sb[off] : T(0);
+  });
+  if ((int)threadIdx.x < L * L) {
+    cD[threadIdx.x] = T(cd0);
+    cM[threadIdx.x] = T(cm0);
+  }
+  if ((int)threadIdx.x < L) cS[threadIdx.x] = T(cs0);
+  // small workgroups (fewer threads than coefficients)
+  for (int i = threadIdx.x + nt; i < L * L; i += nt) {
+    cD[i] = T(a.coef[CL.D() + i]);
+    cM[i] = T(a.coef[CL.M() + i]);
+  }
+  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = T(a.coef[CL.ssN() + i]);
   __syncthreads();
 
   // Spectral step over the points (p0 + c, q).
@@ -155,9 +204,13 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
     if (p < 1) continue;
     const T lp = lam4[p];
     const double sp = a.shat[p];
-    for (int q = 1 + threadIdx.x; q < N; q += blockDim.x) {
+    for (int q = 1 + threadIdx.x; q < N; q += nt) {
+      // keep the L x L coefficient reads inside the loop (hoisted into
+      // registers they would spill for large L)
+      asm volatile("" ::: "memory");
       const T lpq = lp + lam4[q];
-      if (md.mode == 0) {
+      if (a.mode == 0) {
+        const double wpq = sp * a.shat[q];
         T col[8], phi[8];
 #pragma unroll
         for (int ll = 0; ll < 8; ++ll)
@@ -170,13 +223,8 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
             for (int ll = 0; ll < 8; ++ll)
               if (ll < L) s += cD[m + L * ll] * col[ll];
             phi[m] = s / (lpq - cS[m]);
+            acc[m] += double(phi[m]) * wpq;  // wpq = 0 unless p, q odd
           }
-        if ((p & 1) && (q & 1)) {
-          const double wpq = sp * a.shat[q];
-#pragma unroll
-          for (int m = 0; m < 8; ++m)
-            if (m < L) acc[m] += double(phi[m]) * wpq;
-        }
 #pragma unroll
         for (int ll = 0; ll < 8; ++ll)
           if (ll < L) {
@@ -196,138 +244,131 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
       }
     }
   }
-  __syncthreads();
+  // psiC partial of this workgroup (fixed-order FP64 reduction, which also
+  // publishes the spectral step's LDS writes); the row inverse pass reduces
+  // the partials of a member after the kernel boundary.
+  if (a.mode == 0) {
+    block_sum(acc, L, red);
+    if (threadIdx.x == 0) {
+      double *part = a.part + (size_t)b * a.part_stride + (size_t)pblk * L;
+      for (int m = 0; m < L; ++m) part[m] = acc[m];
+    }
+  } else {
+    __syncthreads();
+  }
 
   // Inverse DST along j, in place.
   {
     const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::run(region, t, scan_tmp, static_cast<const cx<T> *>(a.tw), static_cast<const T *>(a.was),
-             [&](int k, T &xa, T &xb) {
-               xa = rowa[pad16(k)];
-               xb = rowb[pad16(k)];
-             });
+    Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
+      xa = rowa[pad16(k)];
+      xb = rowb[pad16(k)];
+    });
   }
   __syncthreads();
   // Write back: per layer and block, C*R contiguous elements.
-  using VT = typename V16<T>::type;
-  constexpr int V = V16<T>::n;
   const int per_layer = (N << lgC) / V;  // vectors per layer: N/R blocks x C x R
   for (int ll = 0; ll < L; ++ll) {
-    for (int iv = threadIdx.x; iv < per_layer; iv += blockDim.x) {
+    for (int iv = threadIdx.x; iv < per_layer; iv += nt) {
       const int e = iv * V;
       const int r0 = e & (R - 1), c = (e >> lgR) & (C - 1), jb = e >> (lgR + lgC);
       const T *src = row_ptr<Eng>(lds, (ll << lgC) + c);
       VT v;
 #pragma unroll
-      for (int s = 0; s < V; ++s) v[s] = src[pad16((jb << lgR) + r0 + s)];
+      for (int q = 0; q < V; ++q) v[q] = src[pad16((jb << lgR) + r0 + q)];
       T *dst = scr + (size_t)ll * N * N + ((size_t)jb * N + p0 + c) * R + r0;
       *reinterpret_cast<VT *>(dst) = v;
     }
   }
-  if (md.mode != 0) return;
-
-  // Workgroup reduction of the psiC partials (FP64) by a fixed-order tree in
-  // LDS.  The row-inverse pass reduces the per-workgroup partials of a member;
-  // the kernel boundary makes them visible without device-scope fences.
-  const int nt = blockDim.x, tid = threadIdx.x;
-  int pw = 1;
-  while (pw < nt) pw <<= 1;
-  for (int m = 0; m < L; ++m) {
-    red[tid] = acc[m];
-    __syncthreads();
-    for (int st = pw >> 1; st > 0; st >>= 1) {
-      if (tid < st && tid + st < nt) red[tid] += red[tid + st];
-      __syncthreads();
-    }
-    if (tid == 0) md.part[(size_t)w.y * L + m] = red[0];
-    __syncthreads();
-  }
 }
 
 // ------------------------------------------------------------------ K3
+// Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
 template <typename Eng, typename Tout>
 __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
-  constexpr int N = Eng::N, TPF = Eng::TPF;
+  using VT = typename V16<T>::type;
+  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<T>::n;
+  constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int4 w = a.work[blockIdx.x];
-  const MemDesc md = a.mems[w.x];
-  const int L = md.L, R = a.R, lgR = a.lgR;
-  const int l = w.y, jb = w.z, j0 = jb << lgR;
+  const int L = a.L, R = a.R, lgR = a.lgR, NB = N >> lgR;
+  const int per_member = L * NB;
+  const int b = blockIdx.x / per_member, rem = blockIdx.x - b * per_member;
+  const int l = rem / NB, jb = rem - l * NB, j0 = jb << lgR;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
-  const int ngrp = R >> 1;
+  const int ngrp = R >> 1, nt = ngrp * TPF;
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
+  const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
+  double *red = reinterpret_cast<double *>(smem + red_off);
   const CoefLayout CL{L, N};
 
-  // psiC_m: reduce the column pass's partials in a fixed order (every
-  // workgroup of the member does the same reduction), then the correction
-  // constants (FP64).
+  // Issue the tile block and psiC partial loads first.
+  const T *in = reinterpret_cast<const T *>(a.scr + b * a.scr_stride) + ((size_t)(l * NB + jb) * N << lgR);
+  VT tile[NVT];
+#pragma unroll
+  for (int s = 0; s < NVT; ++s) tile[s] = *reinterpret_cast<const VT *>(in + (threadIdx.x + s * nt) * V);
+  double pc[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) pc[m] = 0.0;
+  if (a.mode == 0) {
+    const double *part = a.part + (size_t)b * a.part_stride;
+    const int nblk = N >> a.lgC;
+    for (int q = threadIdx.x; q < nblk; q += nt)
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+        if (m < L) pc[m] += part[(size_t)q * L + m];
+  }
+  typename Eng::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
+
+  // Tile block (l, jb): row r at p <- S[p * R + r].
+#pragma unroll
+  for (int s = 0; s < NVT; ++s) {
+    const int e0 = (threadIdx.x + s * nt) * V, r0 = e0 & (R - 1), p = e0 >> lgR;
+#pragma unroll
+    for (int q = 0; q < V; ++q) row_ptr<Eng>(lds, r0 + q)[pad16(p)] = tile[s][q];
+  }
+
+  // psiC_m of the member (the same fixed-order reduction in every
+  // workgroup), then the correction constants (FP64).
   double kappa = 0.0;
   double beta[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) beta[m] = 0.0;
-  if (md.mode == 0) {
-    const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
-    double *red = reinterpret_cast<double *>(smem + red_off);
-    double *s_psic = red + 512;
-    const int nt = blockDim.x, tid = threadIdx.x, nblk = N >> a.lgC;
-    int pw = 1;
-    while (pw < nt) pw <<= 1;
-    for (int m = 0; m < L; ++m) {
-      double part = 0.0;
-      for (int b = tid; b < nblk; b += nt) part += md.part[(size_t)b * L + m];
-      red[tid] = part;
-      __syncthreads();
-      for (int st = pw >> 1; st > 0; st >>= 1) {
-        if (tid < st && tid + st < nt) red[tid] += red[tid + st];
-        __syncthreads();
-      }
-      if (tid == 0) s_psic[m] = red[0] / (double(N) * double(N));
-      __syncthreads();
-    }
-    if (jb == 0 && l == 0 && tid == 0) {
+  if (a.mode == 0) {
+    block_sum(pc, L, red);  // contains the barrier that publishes the tile
+    const double inv_n2 = 1.0 / (double(N) * double(N));
+    if (jb == 0 && l == 0 && threadIdx.x == 0) {
       int st = 0;
       for (int m = 0; m < L; ++m) {
-        const double v = s_psic[m];
-        md.psic[m] = v;
+        const double v = pc[m] * inv_n2;
+        a.psic[(size_t)b * L + m] = v;
         if (v != v) st |= 1;
         else if (fabs(v) > 1.0e100) st |= 2;
       }
-      if (st) atomicOr(md.status, st);
+      if (st) atomicOr(a.status + b, st);
     }
-    const double *M = md.coef + CL.M();
-    const double *ca = md.coef + CL.a();
-    const double *cb = md.coef + CL.b();
-    kappa = s_psic[0];
-    for (int m = 1; m < L; ++m) {
-      const double pc = s_psic[m];
-      kappa += M[l + L * m] * pc * ca[m];
-      beta[m] = M[l + L * m] * pc * cb[m];
-    }
+    const double *M = a.coef + CL.M();
+    const double *ca = a.coef + CL.a();
+    const double *cb = a.coef + CL.b();
+    kappa = pc[0] * inv_n2;
+#pragma unroll
+    for (int m = 1; m < 8; ++m)
+      if (m < L) {
+        const double psc = pc[m] * inv_n2;
+        kappa += M[l + L * m] * psc * ca[m];
+        beta[m] = M[l + L * m] * psc * cb[m];
+      }
+  } else {
     __syncthreads();
   }
 
-  // Tile block (l, jb): row r at p <- S[p * R + r].
-  {
-    using VT = typename V16<T>::type;
-    constexpr int V = V16<T>::n;
-    const T *in = static_cast<const T *>(md.scr) + ((size_t)(l * (N >> lgR) + jb) * N << lgR);
-    const int nvec = (N << lgR) / V;
-    for (int iv = threadIdx.x; iv < nvec; iv += blockDim.x) {
-      const int e0 = iv * V, r0 = e0 & (R - 1), p = e0 >> lgR;
-      const VT v = *reinterpret_cast<const VT *>(in + e0);
-#pragma unroll
-      for (int s = 0; s < V; ++s) row_ptr<Eng>(lds, r0 + s)[pad16(p)] = v[s];
-    }
-  }
-  __syncthreads();
-
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
-  if (md.mode == 0 && L > 1) {
-    const double *tab = md.coef + CL.tab();
-    for (int h = threadIdx.x; h < N / 2; h += blockDim.x) {
+  if (a.mode == 0 && L > 1) {
+    const double *tab = a.coef + CL.tab();
+    for (int h = threadIdx.x; h < N / 2; h += nt) {
       const int p = 2 * h + 1;
       double acc[16];
 #pragma unroll
@@ -367,23 +408,22 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
 
   {
     const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::run(region, t, scan_tmp, static_cast<const cx<T> *>(a.tw), static_cast<const T *>(a.was),
-             [&](int k, T &xa, T &xb) {
-               xa = rowa[pad16(k)];
-               xb = rowb[pad16(k)];
-             });
+    Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
+      xa = rowa[pad16(k)];
+      xb = rowb[pad16(k)];
+    });
   }
   __syncthreads();
 
-  Tout *ps = static_cast<Tout *>(ptrs.psi[w.w]) + (size_t)l * ptrs.ldp1 * ptrs.ldp2;
+  Tout *ps = static_cast<Tout *>(ptrs.psi[b]) + (size_t)l * ptrs.ldp1 * ptrs.ldp2;
   const long long ld1 = ptrs.ldp1;
   constexpr int G = N + 1;
   const bool last = (j0 + R == N);
   const int nrow = R + (last ? 1 : 0);
-  for (int idx = threadIdx.x; idx < nrow * G; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < nrow * G; idx += nt) {
     const int r = idx / G, i = idx - r * G, j = j0 + r;
     Tout *o = ps + (size_t)j * ld1 + i;
-    if (md.mode == 0) {
+    if (a.mode == 0) {
       double val;
       if (j == 0 || j == N || i == 0 || i == N) val = -kappa;
       else val = double(row_ptr<Eng>(lds, r)[pad16(i)]) - kappa;

@@ -65,7 +65,7 @@ struct pq_inv {
   KernelSet ks;
   EngineInfo ei{};
   // tables
-  void *tw = nullptr, *was = nullptr, *lam4 = nullptr;
+  void *tw64 = nullptr, *was = nullptr, *lam4 = nullptr;
   double *shat = nullptr;
   std::vector<double> lam_h, shat_h;  // lam_h[p] = -4 sin^2(p pi / 2N)
   std::vector<double> coef_h;
@@ -73,9 +73,8 @@ struct pq_inv {
   void *scr = nullptr;
   double *part = nullptr, *psic = nullptr;
   int *status = nullptr;
-  MemDesc *mems = nullptr;
-  int4 *work_rows = nullptr, *work_cols = nullptr;
-  int part_cap = 0;  // psiC partial slots per member (N / C_min)
+  size_t scr_member = 0;  // scratch bytes per member
+  int part_cap = 0;       // psiC partial slots per member (N / C_min)
   hipStream_t stream = nullptr;
   // staging for the host drop-in
   void *dz = nullptr, *dp = nullptr;
@@ -88,11 +87,12 @@ struct pq_helm {
 template <typename T> static int upload_tables(pq_inv *h, int M) {
   const int N = h->N;
   const long double pi = 3.141592653589793238462643383279502884L;
-  std::vector<T> tw(2 * (size_t)M), was(N / 2), lam4(N);
+  std::vector<double> tw(2 * (size_t)M);
+  std::vector<T> was(N / 2), lam4(N);
   for (int k = 0; k < M; ++k) {
     const long double a = -2.0L * pi * k / M;
-    tw[2 * k] = T(cosl(a));
-    tw[2 * k + 1] = T(sinl(a));
+    tw[2 * k] = (double)cosl(a);
+    tw[2 * k + 1] = (double)sinl(a);
   }
   for (int k = 0; k < N / 2; ++k) was[k] = T(2.0L * sinl((long double)k * pi / N));
   const long double fourN2 = 4.0L * N * (long double)N;
@@ -106,11 +106,11 @@ template <typename T> static int upload_tables(pq_inv *h, int M) {
     lam4[p] = T(fourN2 * lam);
     if (p & 1) h->shat_h[p] = (double)(2.0L * cosl(x) / s);
   }
-  PQ_HIP(hipMalloc(&h->tw, sizeof(T) * 2 * M));
+  PQ_HIP(hipMalloc(&h->tw64, sizeof(double) * 2 * M));
   PQ_HIP(hipMalloc(&h->was, sizeof(T) * (N / 2)));
   PQ_HIP(hipMalloc(&h->lam4, sizeof(T) * N));
   PQ_HIP(hipMalloc(&h->shat, sizeof(double) * N));
-  PQ_HIP(hipMemcpy(h->tw, tw.data(), sizeof(T) * 2 * M, hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(h->tw64, tw.data(), sizeof(double) * 2 * M, hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(h->was, was.data(), sizeof(T) * (N / 2), hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(h->lam4, lam4.data(), sizeof(T) * N, hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(h->shat, h->shat_h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
@@ -179,27 +179,6 @@ static bool tiles_ok(const pq_inv *h, int R, int C, bool report) {
   return true;
 }
 
-// Work lists for a full chunk of kMaxBatchArg members: (member slot,
-// layer | column block, tile, pointer index).
-static int build_work(pq_inv *h) {
-  (void)hipFree(h->work_rows);
-  (void)hipFree(h->work_cols);
-  h->work_rows = h->work_cols = nullptr;
-  const int chunk = std::min(h->batch_max, kMaxBatchArg);
-  const int N = h->N, L = h->L, R = h->R, C = h->C;
-  std::vector<int4> w1, w2;
-  for (int b = 0; b < chunk; ++b)
-    for (int l = 0; l < L; ++l)
-      for (int t = 0; t < N / R; ++t) w1.push_back(make_int4(b, l, t, b));
-  for (int b = 0; b < chunk; ++b)
-    for (int pb = 0; pb < N / C; ++pb) w2.push_back(make_int4(b, pb, 0, b));
-  if (hipMalloc(&h->work_rows, sizeof(int4) * w1.size()) != hipSuccess) return PQ_ERR_NOMEM;
-  if (hipMalloc(&h->work_cols, sizeof(int4) * w2.size()) != hipSuccess) return PQ_ERR_NOMEM;
-  PQ_HIP(hipMemcpy(h->work_rows, w1.data(), sizeof(int4) * w1.size(), hipMemcpyHostToDevice));
-  PQ_HIP(hipMemcpy(h->work_cols, w2.data(), sizeof(int4) * w2.size(), hipMemcpyHostToDevice));
-  return PQ_OK;
-}
-
 static void default_tiles(const pq_inv *h, int &R, int &C) {
   // 64-byte (FP64) / 64-byte (FP32) transposed runs per column; the largest
   // R the limits allow, halving when a tile does not fit.
@@ -259,13 +238,13 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   if (rc) return fail(rc);
   compute_coefs(h, ss, m2z, z2m);
   const size_t el = h->ei.elem;
-  const size_t scr_member = (size_t)L * N * N * el;
+  h->scr_member = (size_t)L * N * N * el;
   if (hipMalloc(&h->coef, h->coef_h.size() * sizeof(double)) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMemcpy(h->coef, h->coef_h.data(), h->coef_h.size() * sizeof(double), hipMemcpyHostToDevice) !=
       hipSuccess)
     return fail(PQ_ERR_HIP);
-  if (hipMalloc(&h->scr, scr_member * batch_max) != hipSuccess) {
-    pq_set_error("pq_inv_create: cannot allocate %zu bytes of scratch", scr_member * batch_max);
+  if (hipMalloc(&h->scr, h->scr_member * batch_max) != hipSuccess) {
+    pq_set_error("pq_inv_create: cannot allocate %zu bytes of scratch", h->scr_member * batch_max);
     return fail(PQ_ERR_NOMEM);
   }
   if (hipMalloc(&h->part, sizeof(double) * (size_t)h->part_cap * L * batch_max) != hipSuccess)
@@ -274,21 +253,6 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   if (hipMalloc(&h->status, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMemset(h->status, 0, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
   if (hipMemset(h->psic, 0, sizeof(double) * (size_t)L * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
-  std::vector<MemDesc> md(batch_max);
-  for (int b = 0; b < batch_max; ++b) {
-    md[b].coef = h->coef;
-    md[b].scr = (char *)h->scr + scr_member * b;
-    md[b].part = h->part + (size_t)h->part_cap * L * b;
-    md[b].psic = h->psic + (size_t)L * b;
-    md[b].status = h->status + b;
-    md[b].L = L;
-    md[b].mode = mode;
-  }
-  if (hipMalloc(&h->mems, sizeof(MemDesc) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
-  if (hipMemcpy(h->mems, md.data(), sizeof(MemDesc) * batch_max, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(PQ_ERR_HIP);
-  rc = build_work(h);
-  if (rc) return fail(rc);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PQ_ERR_HIP);
   *out = h;
   return PQ_OK;
@@ -297,17 +261,25 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
 // pass_mask bit 0: row forward pass, bit 1: column pass, bit 2: row inverse.
 static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream_t s, int pass_mask) {
   const EngineInfo &e = h->ei;
+  const int N = h->N, L = h->L;
   InvArgs a{};
-  a.mems = h->mems + mem0;
-  a.tw = h->tw;
+  a.scr = static_cast<char *>(h->scr) + h->scr_member * mem0;
+  a.scr_stride = (long long)h->scr_member;
+  a.part = h->part + (size_t)h->part_cap * L * mem0;
+  a.part_stride = h->part_cap * L;
+  a.psic = h->psic + (size_t)L * mem0;
+  a.status = h->status + mem0;
+  a.coef = h->coef;
+  a.tw64 = h->tw64;
   a.was = h->was;
   a.lam4 = h->lam4;
   a.shat = h->shat;
+  a.L = L;
+  a.mode = h->mode;
   a.R = h->R;
   a.lgR = ilog2i(h->R);
   a.C = h->C;
   a.lgC = ilog2i(h->C);
-  const int N = h->N, L = h->L;
   const int rows_threads = (h->R / 2) * e.tpf;
   const int col_groups = (h->C * L + 1) / 2;
   for (int pass = 0; pass < 3; ++pass) {
@@ -315,12 +287,10 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
     int blocks, threads;
     size_t lds;
     if (pass == 1) {
-      a.work = h->work_cols;
       blocks = nb * (N / h->C);
       threads = col_groups * e.tpf;
       lds = inv_lds_bytes(e, 1, col_groups);
     } else {
-      a.work = h->work_rows;
       blocks = nb * L * (N / h->R);
       threads = rows_threads;
       lds = inv_lds_bytes(e, pass, h->R / 2);
@@ -371,7 +341,7 @@ int pq_inv_create(pq_inv **h, int g, int layers, const double *ss, const double 
 int pq_inv_destroy(pq_inv *h) {
   if (!h) return PQ_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  (void)hipFree(h->tw);
+  (void)hipFree(h->tw64);
   (void)hipFree(h->was);
   (void)hipFree(h->lam4);
   (void)hipFree(h->shat);
@@ -380,9 +350,6 @@ int pq_inv_destroy(pq_inv *h) {
   (void)hipFree(h->part);
   (void)hipFree(h->psic);
   (void)hipFree(h->status);
-  (void)hipFree(h->mems);
-  (void)hipFree(h->work_rows);
-  (void)hipFree(h->work_cols);
   (void)hipFree(h->dz);
   (void)hipFree(h->dp);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -395,18 +362,12 @@ void *pq_inv_stream(pq_inv *h) { return h ? (void *)h->stream : nullptr; }
 int pq_inv_set_tiles(pq_inv *h, int rows, int cols) {
   PQ_CHECK_ARG(h != nullptr, "pq_inv_set_tiles: null handle");
   if (!tiles_ok(h, rows, cols, true)) return PQ_ERR_ARG;
+  // the scratch layout depends on R: let running work finish first
   PQ_HIP(hipStreamSynchronize(h->stream));
   PQ_HIP(hipDeviceSynchronize());
-  const int R0 = h->R, C0 = h->C;
   h->R = rows;
   h->C = cols;
-  int rc = build_work(h);
-  if (rc) {
-    h->R = R0;
-    h->C = C0;
-    build_work(h);
-  }
-  return rc;
+  return PQ_OK;
 }
 
 int pq_inv_get_tiles(pq_inv *h, int *rows, int *cols) {
