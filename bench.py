@@ -40,12 +40,16 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--precision", default="fp64", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"],
+                    help="fp32 (default): the reduced-precision path that meets 1e-6; fp64: plain drop-in")
     ap.add_argument("--ring", type=int, default=24, help="distinct input/output buffers")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cabaret", action="store_true")
     ap.add_argument("--no-latency", action="store_true",
                     help="skip the B=1 latency leg (every inversion launch then has the bench's batch)")
+    ap.add_argument("--sweep-tiles", action="store_true",
+                    help="time every tile shape of the inversion passes, record the best and exit")
+    ap.add_argument("--tiles", default="auto", help="'auto' (tuned table), 'default', or R,C")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 10 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -76,9 +80,19 @@ def main():
     def dev_field(a, dtype):
         return torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(dev, dtype)
 
+    from pequod_amd import autotune
+
+    def make_cache(prec, batch):
+        if args.tiles == "auto":
+            return qg.HelmholtzCache(g, st, prec, batch_max=batch)
+        if args.tiles == "default":
+            return qg.HelmholtzCache(g, st, prec, batch_max=batch, tiles=None)
+        return qg.HelmholtzCache(g, st, prec, batch_max=batch,
+                                 tiles=tuple(int(x) for x in args.tiles.split(",")))
+
     def run_rate(prec, steps, warmup, batch):
         dt = torch.float32 if prec == "fp32" else torch.float64
-        hc = qg.HelmholtzCache(g, st, prec, batch_max=batch)
+        hc = make_cache(prec, batch)
         base = dev_field(q, dt)
         nring = max(args.ring, batch)
         # distinct inputs: the seeded field with a per-buffer amplitude
@@ -101,6 +115,23 @@ def main():
         rc, _ = hc.status()
         assert rc == 0, f"inversion status {rc}"
         return hc, zs, ps, (t1 - t0), base
+
+    if args.sweep_tiles:
+        for prec in ("fp64", "fp32"):
+            for bt in (1, B):
+                dt = torch.float32 if prec == "fp32" else torch.float64
+                hc = qg.HelmholtzCache(g, st, prec, batch_max=bt, tiles=None)
+                base = dev_field(q, dt)
+                zs = [base * (1.0 + 0.01 * k) for k in range(bt)]
+                ps = [torch.empty_like(base) for _ in range(bt)]
+                res = autotune.sweep(hc, zs, ps, reps=10)
+                for R, C, ms in res:
+                    log(f"[sweep] {prec} B={bt} R={R} C={C} passes {[round(x, 4) for x in ms]} "
+                        f"total {sum(ms):.4f} ms")
+                e = autotune.record(g, L, prec, bt, res)
+                log(f"[sweep] {prec} B={bt} best {e}")
+                hc.deallocate()
+        return
 
     log(f"[bench] C2 {args.precision} batch {B}: {args.steps} steps")
     hc, zs, ps, wall, base = run_rate(args.precision, args.steps, args.warmup, B)
@@ -206,7 +237,8 @@ def main():
         "dtype": {"fp32": "f32", "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
         "config": {"workload": "C2: 3-layer QG 1023^2 interior (G=1025), dx 5 km, PV inversion",
-                   "batch_per_step": B, "precision": args.precision, "ring_buffers": max(args.ring, B)},
+                   "batch_per_step": B, "precision": args.precision, "ring_buffers": max(args.ring, B),
+                   "tiles_RC": list(hc.tiles())},
         "p50_ms": None if p50 is None else round(p50, 4),
         "p99_ms": None if p99 is None else round(p99, 4),
         "max_rel_psi_err_vs_fp64": err,

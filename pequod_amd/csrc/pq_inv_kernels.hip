@@ -132,6 +132,82 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   }
 }
 
+// Spectral step of the column pass on the points (p0 + c, q), c < C,
+// 1 <= q < N, rows (l, c) at LDS row l C + c.  pv_inversion (mode 0):
+// phi_m = sum_l D(m,l) x_l / ((2N)^2 (lambda_p + lambda_q - ss_m)), FP64
+// psiC partial acc_m += phi_m shat_p shat_q, x_l = sum_m M(l,m) phi_m.
+// helm (mode 1): x_l /= (2N)^2 (lambda_p + lambda_q - ss_l).  LT is the
+// layer count when small enough to hold the projections in registers (1..4),
+// else 0 (coefficients read from LDS per point).
+template <typename Eng, int LT>
+__device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T *lds,
+                                              const typename Eng::T *cD, const typename Eng::T *cM,
+                                              const typename Eng::T *cS, int p0, int nt,
+                                              double (&acc)[8]) {
+  using T = typename Eng::T;
+  constexpr int N = Eng::N;
+  const int L = LT > 0 ? LT : a.L, C = a.C, lgC = a.lgC;
+  const T *lam4 = static_cast<const T *>(a.lam4);
+  constexpr int LR = LT > 0 ? LT : 1;
+  T d[LR * LR], mm[LR * LR], sv[LR];
+  if constexpr (LT > 0) {
+#pragma unroll
+    for (int i = 0; i < LT * LT; ++i) {
+      d[i] = cD[i];
+      mm[i] = cM[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LT; ++i) sv[i] = cS[i];
+  }
+  for (int c = 0; c < C; ++c) {
+    const int p = p0 + c;
+    if (p < 1) continue;
+    const T lp = lam4[p];
+    const double sp = a.shat[p];
+    for (int q = 1 + threadIdx.x; q < N; q += nt) {
+      if constexpr (LT == 0) {
+        // keep the L x L coefficient reads inside the loop (hoisted into
+        // registers they would spill for large L)
+        asm volatile("" ::: "memory");
+      }
+      const T lpq = lp + lam4[q];
+      if (a.mode == 0) {
+        const double wpq = sp * a.shat[q];
+        T col[8], phi[8];
+#pragma unroll
+        for (int ll = 0; ll < 8; ++ll)
+          if (ll < L) col[ll] = row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)];
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          if (m < L) {
+            T s = T(0);
+#pragma unroll
+            for (int ll = 0; ll < 8; ++ll)
+              if (ll < L) s += (LT > 0 ? d[(m + L * ll) % (LR * LR)] : cD[m + L * ll]) * col[ll];
+            phi[m] = s / (lpq - (LT > 0 ? sv[m % LR] : cS[m]));
+            acc[m] += double(phi[m]) * wpq;  // wpq = 0 unless p, q odd
+          }
+#pragma unroll
+        for (int ll = 0; ll < 8; ++ll)
+          if (ll < L) {
+            T s = T(0);
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+              if (m < L) s += (LT > 0 ? mm[(ll + L * m) % (LR * LR)] : cM[ll + L * m]) * phi[m];
+            row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = s;
+          }
+      } else {
+#pragma unroll
+        for (int ll = 0; ll < 8; ++ll)
+          if (ll < L) {
+            T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(q);
+            *x = *x / (lpq - cS[ll]);
+          }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ K2
 // Grid: nb * (N / C) workgroups of ceil(C L / 2) * TPF threads.
 template <typename Eng>
@@ -195,54 +271,15 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   __syncthreads();
 
   // Spectral step over the points (p0 + c, q).
-  const T *lam4 = static_cast<const T *>(a.lam4);
   double acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = 0.0;
-  for (int c = 0; c < C; ++c) {
-    const int p = p0 + c;
-    if (p < 1) continue;
-    const T lp = lam4[p];
-    const double sp = a.shat[p];
-    for (int q = 1 + threadIdx.x; q < N; q += nt) {
-      // keep the L x L coefficient reads inside the loop (hoisted into
-      // registers they would spill for large L)
-      asm volatile("" ::: "memory");
-      const T lpq = lp + lam4[q];
-      if (a.mode == 0) {
-        const double wpq = sp * a.shat[q];
-        T col[8], phi[8];
-#pragma unroll
-        for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) col[ll] = row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)];
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-          if (m < L) {
-            T s = T(0);
-#pragma unroll
-            for (int ll = 0; ll < 8; ++ll)
-              if (ll < L) s += cD[m + L * ll] * col[ll];
-            phi[m] = s / (lpq - cS[m]);
-            acc[m] += double(phi[m]) * wpq;  // wpq = 0 unless p, q odd
-          }
-#pragma unroll
-        for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) {
-            T s = T(0);
-#pragma unroll
-            for (int m = 0; m < 8; ++m)
-              if (m < L) s += cM[ll + L * m] * phi[m];
-            row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = s;
-          }
-      } else {
-#pragma unroll
-        for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) {
-            T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(q);
-            *x = *x / (lpq - cS[ll]);
-          }
-      }
-    }
+  switch (a.mode == 0 ? L : 0) {
+    case 1: spectral_step<Eng, 1>(a, lds, cD, cM, cS, p0, nt, acc); break;
+    case 2: spectral_step<Eng, 2>(a, lds, cD, cM, cS, p0, nt, acc); break;
+    case 3: spectral_step<Eng, 3>(a, lds, cD, cM, cS, p0, nt, acc); break;
+    case 4: spectral_step<Eng, 4>(a, lds, cD, cM, cS, p0, nt, acc); break;
+    default: spectral_step<Eng, 0>(a, lds, cD, cM, cS, p0, nt, acc); break;
   }
   // psiC partial of this workgroup (fixed-order FP64 reduction, which also
   // publishes the spectral step's LDS writes); the row inverse pass reduces

@@ -92,7 +92,7 @@ class HelmholtzCache:
     batch_max members per call."""
 
     def __init__(self, g: int, strat: Stratification, precision: str = "fp64",
-                 batch_max: int = 1):
+                 batch_max: int = 1, tiles="auto"):
         self.g = g
         self.strat = strat
         self.layers = strat.layers
@@ -105,6 +105,13 @@ class HelmholtzCache:
         check(load().pq_inv_create(ctypes.byref(h), g, self.layers, _np_ptr(ss), _np_ptr(m2z),
                                    _np_ptr(z2m), PREC[precision], batch_max))
         self._h = h
+        # tile shapes: "auto" = the autotuner's table (built-in defaults when
+        # the size class has no entry), None = built-in defaults, or (R, C)
+        if tiles == "auto":
+            from . import autotune
+            autotune.apply(self)
+        elif tiles is not None and not self.set_tiles(*tiles):
+            raise ValueError(f"tile shape {tiles} not supported for g={g}, L={self.layers}")
 
     @property
     def stream(self) -> int:
