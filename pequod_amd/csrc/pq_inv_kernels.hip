@@ -90,6 +90,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], int n, double *red) {
   __syncthreads();
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block i on XCD i % 8),
+// each XCD with its own L2.  Remapping block i to
+// (i % 8) * (n / 8) + i / 8 gives each XCD a contiguous range of logical
+// blocks, so neighbouring blocks that share cache lines meet in one L2.
+__device__ __forceinline__ int xcd_block(int i, int n) {
+  return (n & 7) ? i : (i & 7) * (n >> 3) + (i >> 3);
+}
+
 // ------------------------------------------------------------------ K1
 // Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
 template <typename Eng, typename Tin>
@@ -219,7 +227,9 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   T *lds = reinterpret_cast<T *>(smem);
   const int L = a.L, C = a.C, lgC = a.lgC, R = a.R, lgR = a.lgR;
   const int per_member = N >> lgC;
-  const int b = blockIdx.x / per_member, pblk = blockIdx.x - b * per_member;
+  // neighbouring column blocks share the cache lines of the C R runs
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int b = bid / per_member, pblk = bid - b * per_member;
   const int p0 = pblk << lgC;
   const int ngrp = blockDim.x / TPF, nt = blockDim.x;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
