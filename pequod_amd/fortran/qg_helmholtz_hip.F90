@@ -1,5 +1,3 @@
-#include "debug.h"
-
 ! qg_helmholtz_hip -- drop-in replacement of the reference's qg_helmholtz
 ! module (qg3p_helmholtz.F90) on one MI355X.  It exports the same four names
 ! that qg_aux imports (qg3p_aux.F90:13-17):
@@ -12,8 +10,9 @@
 ! the reference's signatures and semantics: helm solves
 ! (Delta_5 - ss_l) psi_l = zeta_l in place with homogeneous Dirichlet values
 ! and the reference's output boundary values (qg3p_helmholtz.F90:614-769).
-! Errors abort through the reference's labort (debug.h).  One domain, closed
-! basin (decomp%n == 1, not periodic in j), G - 1 = 2^a or 3 * 2^a.
+! Errors abort through the reference's pabort (qg3p_base.F90:383-403), as its
+! labort macro does.  One domain, closed basin (decomp%n == 1, not periodic
+! in j), G - 1 = 2^a or 3 * 2^a.
 module qg_helmholtz_hip
 
   use iso_c_binding
@@ -54,16 +53,17 @@ contains
     integer(c_int) :: rc
 
     if (decomp%n /= 1) then
-      labort("qg_helmholtz_hip: one domain only (run with one MPI process)")
+      call pabort("qg_helmholtz_hip: one domain only (run with one MPI process)", &
+        & __FILE__, __LINE__)
     end if
     if (decomp%periodic_j) then
-      labort("qg_helmholtz_hip: closed basin only")
+      call pabort("qg_helmholtz_hip: closed basin only", __FILE__, __LINE__)
     end if
     helm_c%decomp => decomp
     rc = pq_helm_create(helm_c%h, int(decomp%ii, c_int), int(decomp%jj, c_int), &
       & int(decomp%layers, c_int), ss, PQ_PREC_FP64)
     if (rc /= PQ_OK) then
-      labort("qg_helmholtz_hip: " // pq_error_message())
+      call pabort("qg_helmholtz_hip: " // pq_error_message(), __FILE__, __LINE__)
     end if
   end subroutine allocate_helm_c
 
@@ -85,14 +85,16 @@ contains
 
     integer(c_int) :: rc
 
-    assert(size(zeta, 1) == helm_c%decomp%ii)
-    assert(size(zeta, 2) == helm_c%decomp%jj)
-    assert(size(zeta, 3) == helm_c%decomp%layers)
+    if (size(zeta, 1) /= helm_c%decomp%ii .or. size(zeta, 2) /= helm_c%decomp%jj &
+      & .or. size(zeta, 3) /= helm_c%decomp%layers) then
+      call pabort("qg_helmholtz_hip: zeta shape differs from the cache's decomposition", &
+        & __FILE__, __LINE__)
+    end if
 
     rc = pq_helm_solve(helm_c%h, zeta, int(size(zeta, 1), c_int), &
       & int(size(zeta, 2), c_int))
     if (rc /= PQ_OK) then
-      labort("qg_helmholtz_hip: " // pq_error_message())
+      call pabort("qg_helmholtz_hip: " // pq_error_message(), __FILE__, __LINE__)
     end if
   end subroutine helm
 

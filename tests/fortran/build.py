@@ -1,0 +1,48 @@
+"""Build the Fortran drop-in test program (tests/fortran/_build/helm_drop_in):
+the product modules pequod_amd/fortran/{pequod_hip_c,qg_helmholtz_hip}.F90
+against the qg_parallel test scaffold, linked to pequod_amd/libpequod_hip.so
+(rpath relative to the binary, so the tree can move).  Returns the binary's
+path, or None when no Fortran compiler is installed."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+OUT = os.path.join(HERE, "_build")
+EXE = os.path.join(OUT, "helm_drop_in")
+
+
+def _fc():
+    for c in ("/opt/rocm/llvm/bin/amdflang", shutil.which("amdflang"), shutil.which("flang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build() -> str | None:
+    fc = _fc()
+    if fc is None:
+        return None
+    os.makedirs(OUT, exist_ok=True)
+    srcs = [os.path.join(ROOT, "pequod_amd", "fortran", "pequod_hip_c.F90"),
+            os.path.join(HERE, "qg_parallel_stub.F90"),
+            os.path.join(ROOT, "pequod_amd", "fortran", "qg_helmholtz_hip.F90"),
+            os.path.join(HERE, "helm_drop_in.F90")]
+    objs = []
+    for s in srcs:
+        o = os.path.join(OUT, os.path.splitext(os.path.basename(s))[0] + ".o")
+        subprocess.run([fc, "-cpp", "-O2", "-c", s, "-module-dir", OUT, "-I", OUT, "-o", o],
+                       check=True, capture_output=True, text=True)
+        objs.append(o)
+    libdir = os.path.join(ROOT, "pequod_amd")
+    rel = os.path.relpath(libdir, OUT)
+    subprocess.run([fc, "-o", EXE, *objs[::-1], f"-L{libdir}", "-lpequod_hip",
+                    f"-Wl,-rpath,$ORIGIN/{rel}"], check=True, capture_output=True, text=True)
+    return EXE
+
+
+if __name__ == "__main__":
+    print(build())
