@@ -40,8 +40,9 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"],
-                    help="fp32 (default): the reduced-precision path that meets 1e-6; fp64: plain drop-in")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32s", "mixed", "fp64"],
+                    help="fp32 (default): FP32 storage and transforms; fp32s: FP32 storage, FP64 "
+                         "transforms; mixed: FP64 storage, FP32 transforms; fp64: plain drop-in")
     ap.add_argument("--ring", type=int, default=24, help="distinct input/output buffers")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cabaret", action="store_true")
@@ -90,8 +91,13 @@ def main():
         return qg.HelmholtzCache(g, st, prec, batch_max=batch,
                                  tiles=tuple(int(x) for x in args.tiles.split(",")))
 
+    from pequod_amd._lib import IO_BYTES
+
+    def io_dtype(prec):
+        return torch.float32 if IO_BYTES[prec] == 4 else torch.float64
+
     def run_rate(prec, steps, warmup, batch):
-        dt = torch.float32 if prec == "fp32" else torch.float64
+        dt = io_dtype(prec)
         hc = make_cache(prec, batch)
         base = dev_field(q, dt)
         nring = max(args.ring, batch)
@@ -117,9 +123,9 @@ def main():
         return hc, zs, ps, (t1 - t0), base
 
     if args.sweep_tiles:
-        for prec in ("fp64", "fp32"):
+        for prec in ("fp64", "fp32", "fp32s"):
             for bt in (1, B):
-                dt = torch.float32 if prec == "fp32" else torch.float64
+                dt = io_dtype(prec)
                 hc = qg.HelmholtzCache(g, st, prec, batch_max=bt, tiles=None)
                 base = dev_field(q, dt)
                 zs = [base * (1.0 + 0.01 * k) for k in range(bt)]
@@ -141,8 +147,8 @@ def main():
     # per-pass kernel durations (HIP events on the handle's stream)
     torch.cuda.synchronize()
     passes = hc.pass_times(zs[:B], ps[:B], reps=20)
-    el = 4 if args.precision == "fp32" else 8
-    el_t = 4 if args.precision != "fp64" else 8  # transform / scratch precision
+    el = IO_BYTES[args.precision]                 # zeta / psi element size
+    el_t = 8 if args.precision == "fp64" else 4   # scratch element size
     names = ["k_rows_fwd", "k_cols", "k_rows_inv"]
     dom = int(np.argmax(passes))
     # algorithmic bytes per launch: one read + one write of the L x G x G
@@ -234,7 +240,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
+        "dtype": {"fp32": "f32", "fp32s": "f32 storage, f64 transforms",
+                  "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
         "config": {"workload": "C2: 3-layer QG 1023^2 interior (G=1025), dx 5 km, PV inversion",
                    "batch_per_step": B, "precision": args.precision, "ring_buffers": max(args.ring, B),

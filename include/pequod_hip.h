@@ -51,6 +51,7 @@ extern "C" {
 #define PQ_PREC_FP64 0    /* FP64 storage and transforms (plain drop-in)   */
 #define PQ_PREC_FP32 1    /* FP32 storage and FP32 transforms              */
 #define PQ_PREC_MIXED 2   /* FP64 storage, FP32 transforms                 */
+#define PQ_PREC_FP32_STORAGE 3 /* FP32 storage, FP64 transforms             */
 
 const char *pq_last_error(void);
 /* Library/runtime information: device name and CU count into buf.         */

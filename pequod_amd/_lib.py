@@ -10,7 +10,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpequod_hip.so")
 
 PQ_OK, PQ_ERR_ARG, PQ_ERR_HIP, PQ_ERR_NAN, PQ_WARN_LARGE, PQ_ERR_NOMEM = range(6)
-PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
+# fp64: FP64 storage and transforms; fp32: FP32 storage and transforms;
+# mixed: FP64 storage, FP32 transforms; fp32s: FP32 storage, FP64 transforms
+PREC = {"fp64": 0, "fp32": 1, "mixed": 2, "fp32s": 3}
+# element type of the zeta / psi arrays of each precision
+IO_BYTES = {"fp64": 8, "fp32": 4, "mixed": 8, "fp32s": 4}
 
 _i, _d, _vp, _sz, _l = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_long
 _dp = ctypes.POINTER(ctypes.c_double)

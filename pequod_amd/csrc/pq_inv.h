@@ -61,7 +61,8 @@ struct EngineInfo {
   int tpf;           // threads per FFT group (one group = two rows)
   int region;        // LDS elements of T per group
   int scan;          // LDS elements of T of scan scratch per group
-  int elem;          // sizeof(T)
+  int elem;          // sizeof(T), the compute type
+  int selem;         // sizeof of the scratch type
 };
 
 // Supported transform lengths N = 2^a (8..4096) and 3 * 2^a (12..3072); one

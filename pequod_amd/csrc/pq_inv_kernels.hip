@@ -100,11 +100,11 @@ __device__ __forceinline__ int xcd_block(int i, int n) {
 
 // ------------------------------------------------------------------ K1
 // Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
-template <typename Eng, typename Tin>
+template <typename Eng, typename Tin, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
-  using VT = typename V16<T>::type;
-  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<T>::n;
+  using VT = typename V16<Ts>::type;
+  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
@@ -129,13 +129,13 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   });
   __syncthreads();
   // Block (l, jb): S[p][r] = row r at p, r fastest; one contiguous N*R run.
-  T *out = reinterpret_cast<T *>(a.scr + b * a.scr_stride) + ((size_t)(l * NB + jb) * N << lgR);
+  Ts *out = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride) + ((size_t)(l * NB + jb) * N << lgR);
 #pragma unroll
   for (int s = 0; s < NVT; ++s) {
     const int e0 = (threadIdx.x + s * nt) * V, r0 = e0 & (R - 1), p = e0 >> lgR;
     VT v;
 #pragma unroll
-    for (int q = 0; q < V; ++q) v[q] = row_ptr<Eng>(lds, r0 + q)[pad16(p)];
+    for (int q = 0; q < V; ++q) v[q] = Ts(row_ptr<Eng>(lds, r0 + q)[pad16(p)]);
     *reinterpret_cast<VT *>(out + e0) = v;
   }
 }
@@ -218,11 +218,11 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
 
 // ------------------------------------------------------------------ K2
 // Grid: nb * (N / C) workgroups of ceil(C L / 2) * TPF threads.
-template <typename Eng>
+template <typename Eng, typename Ts>
 __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   using T = typename Eng::T;
-  using VT = typename V16<T>::type;
-  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<T>::n;
+  using VT = typename V16<Ts>::type;
+  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
   const int L = a.L, C = a.C, lgC = a.lgC, R = a.R, lgR = a.lgR;
@@ -253,19 +253,19 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   if ((int)threadIdx.x < L) cs0 = a.coef[CL.ssN() + threadIdx.x];
 
   // Element j of row (l, p) sits at l N^2 + (j / R) N R + p R + j % R.
-  T *scr = reinterpret_cast<T *>(a.scr + b * a.scr_stride);
+  Ts *scr = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride);
   const int ra = 2 * f, rb = ra + 1;
   const int la = ra >> lgC, pa = p0 + (ra & (C - 1));
   const int lb = rb >> lgC, pb = p0 + (rb & (C - 1));
   const bool va = ra < nrows && pa >= 1;
   const bool vb = rb < nrows && pb >= 1;
-  const T *sa = scr + (size_t)la * N * N + (size_t)pa * R;
-  const T *sb = scr + (size_t)lb * N * N + (size_t)pb * R;
+  const Ts *sa = scr + (size_t)la * N * N + (size_t)pa * R;
+  const Ts *sb = scr + (size_t)lb * N * N + (size_t)pb * R;
   const int blk = N << lgR;
   Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
     const int off = (k >> lgR) * blk + (k & (R - 1));
-    xa = va ? sa[off] : T(0);
-    xb = vb ? sb[off] : T(0);
+    xa = va ? T(sa[off]) : T(0);
+    xb = vb ? T(sb[off]) : T(0);
   });
   if ((int)threadIdx.x < L * L) {
     cD[threadIdx.x] = T(cd0);
@@ -322,8 +322,8 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
       const T *src = row_ptr<Eng>(lds, (ll << lgC) + c);
       VT v;
 #pragma unroll
-      for (int q = 0; q < V; ++q) v[q] = src[pad16((jb << lgR) + r0 + q)];
-      T *dst = scr + (size_t)ll * N * N + ((size_t)jb * N + p0 + c) * R + r0;
+      for (int q = 0; q < V; ++q) v[q] = Ts(src[pad16((jb << lgR) + r0 + q)]);
+      Ts *dst = scr + (size_t)ll * N * N + ((size_t)jb * N + p0 + c) * R + r0;
       *reinterpret_cast<VT *>(dst) = v;
     }
   }
@@ -331,11 +331,11 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
 
 // ------------------------------------------------------------------ K3
 // Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
-template <typename Eng, typename Tout>
+template <typename Eng, typename Tout, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
-  using VT = typename V16<T>::type;
-  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<T>::n;
+  using VT = typename V16<Ts>::type;
+  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   const CoefLayout CL{L, N};
 
   // Issue the tile block and psiC partial loads first.
-  const T *in = reinterpret_cast<const T *>(a.scr + b * a.scr_stride) + ((size_t)(l * NB + jb) * N << lgR);
+  const Ts *in = reinterpret_cast<const Ts *>(a.scr + b * a.scr_stride) + ((size_t)(l * NB + jb) * N << lgR);
   VT tile[NVT];
 #pragma unroll
   for (int s = 0; s < NVT; ++s) tile[s] = *reinterpret_cast<const VT *>(in + (threadIdx.x + s * nt) * V);
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   for (int s = 0; s < NVT; ++s) {
     const int e0 = (threadIdx.x + s * nt) * V, r0 = e0 & (R - 1), p = e0 >> lgR;
 #pragma unroll
-    for (int q = 0; q < V; ++q) row_ptr<Eng>(lds, r0 + q)[pad16(p)] = tile[s][q];
+    for (int q = 0; q < V; ++q) row_ptr<Eng>(lds, r0 + q)[pad16(p)] = T(tile[s][q]);
   }
 
   // psiC_m of the member (the same fixed-order reduction in every
@@ -488,8 +488,9 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
 }
 
 // ------------------------------------------------------------ dispatch
-template <typename Eng> static EngineInfo info_of() {
-  return EngineInfo{Eng::M, Eng::TPF, Eng::Region, Eng::ScanTmp, (int)sizeof(typename Eng::T)};
+template <typename Eng, typename Ts> static EngineInfo info_of() {
+  return EngineInfo{Eng::M, Eng::TPF, Eng::Region, Eng::ScanTmp, (int)sizeof(typename Eng::T),
+                    (int)sizeof(Ts)};
 }
 
 // Dynamic LDS above the 64 KiB default needs a per-kernel limit (gfx950 has
@@ -504,18 +505,19 @@ template <typename K> static void lds_limit(K kernel, size_t bytes) {
   if (hipGetLastError() == hipSuccess) granted = bytes;
 }
 
-template <typename Eng, typename Tio>
+// Eng: transform engine (compute type), Tio: zeta/psi type, Ts: scratch type.
+template <typename Eng, typename Tio, typename Ts>
 static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int blocks, int threads,
                               size_t lds, hipStream_t s) {
   if (pass == 0) {
-    lds_limit(k_rows_fwd<Eng, Tio>, lds);
-    hipLaunchKernelGGL((k_rows_fwd<Eng, Tio>), dim3(blocks), dim3(threads), lds, s, a, pt);
+    lds_limit(k_rows_fwd<Eng, Tio, Ts>, lds);
+    hipLaunchKernelGGL((k_rows_fwd<Eng, Tio, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   } else if (pass == 1) {
-    lds_limit(k_cols<Eng>, lds);
-    hipLaunchKernelGGL((k_cols<Eng>), dim3(blocks), dim3(threads), lds, s, a);
+    lds_limit(k_cols<Eng, Ts>, lds);
+    hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a);
   } else {
-    lds_limit(k_rows_inv<Eng, Tio>, lds);
-    hipLaunchKernelGGL((k_rows_inv<Eng, Tio>), dim3(blocks), dim3(threads), lds, s, a, pt);
+    lds_limit(k_rows_inv<Eng, Tio, Ts>, lds);
+    hipLaunchKernelGGL((k_rows_inv<Eng, Tio, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   }
   return hipGetLastError();
 }
@@ -523,19 +525,29 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
 #define PQ_CAT2(a, b) a##b
 #define PQ_CAT(a, b) PQ_CAT2(a, b)
 
+// PQ_PREC_FP64: FP64 everywhere (FFTPACK-style sint engine).
+// PQ_PREC_FP32: FP32 fields, scratch and transforms (odd-extension engine).
+// PQ_PREC_MIXED: FP64 fields, FP32 scratch and transforms.
+// PQ_PREC_FP32_STORAGE: FP32 fields and scratch, FP64 transforms.
 EngineInfo PQ_CAT(inv_info_, PQ_N)(int prec) {
-  return prec == PQ_PREC_FP64 ? info_of<DstSint<double, PQ_N>>() : info_of<DstExt<float, PQ_N>>();
+  switch (prec) {
+    case PQ_PREC_FP64: return info_of<DstSint<double, PQ_N>, double>();
+    case PQ_PREC_FP32_STORAGE: return info_of<DstSint<double, PQ_N>, float>();
+    default: return info_of<DstExt<float, PQ_N>, float>();
+  }
 }
 
 hipError_t PQ_CAT(inv_launch_, PQ_N)(int prec, int pass, const InvArgs &a, const PtrTab &pt,
                                      int blocks, int threads, size_t lds, hipStream_t s) {
   switch (prec) {
     case PQ_PREC_FP64:
-      return launch_pass<DstSint<double, PQ_N>, double>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<DstSint<double, PQ_N>, double, double>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32:
-      return launch_pass<DstExt<float, PQ_N>, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<DstExt<float, PQ_N>, float, float>(pass, a, pt, blocks, threads, lds, s);
+    case PQ_PREC_FP32_STORAGE:
+      return launch_pass<DstSint<double, PQ_N>, float, float>(pass, a, pt, blocks, threads, lds, s);
     default:
-      return launch_pass<DstExt<float, PQ_N>, double>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<DstExt<float, PQ_N>, double, float>(pass, a, pt, blocks, threads, lds, s);
   }
 }
 

@@ -159,7 +159,7 @@ static void compute_coefs(pq_inv *h, const double *ss, const double *m2z, const 
 // Whether rows-per-tile R and columns-per-workgroup C fit the kernels.
 static bool tiles_ok(const pq_inv *h, int R, int C, bool report) {
   const EngineInfo &e = h->ei;
-  const int V = 16 / e.elem;
+  const int V = 16 / e.selem;  // 16-byte vectors of the scratch type
   auto fail = [&](const char *why) {
     if (report)
       pq_set_error("pq_inv tiles R=%d C=%d unsupported for N=%d, L=%d, precision %d: %s", R, C,
@@ -184,7 +184,7 @@ static void default_tiles(const pq_inv *h, int &R, int &C) {
   // R the limits allow, halving when a tile does not fit.
   R = 16;
   while (R > 2 && !tiles_ok(h, R, 1, false)) R /= 2;
-  if (h->prec == PQ_PREC_FP64 && R > 8) R = 8;
+  if (h->ei.selem == 8 && R > 8) R = 8;  // FP64 scratch: 64-byte runs at R = 8
   C = 2;
   while (C > 1 && !tiles_ok(h, R, C, false)) C /= 2;
 }
@@ -195,7 +195,7 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   PQ_CHECK_ARG(g >= 9 && pq_supported_n(g - 2),
                "pq_inv_create: unsupported grid size g=%d (need g-1 = 2^a or 3*2^a, 8 <= g-1 <= 4096)", g);
   PQ_CHECK_ARG(L >= 1 && L <= 8, "pq_inv_create: layers must be in 1..8 (got %d)", L);
-  PQ_CHECK_ARG(precision >= 0 && precision <= 2, "pq_inv_create: bad precision %d", precision);
+  PQ_CHECK_ARG(precision >= 0 && precision <= 3, "pq_inv_create: bad precision %d", precision);
   PQ_CHECK_ARG(batch_max >= 1, "pq_inv_create: batch_max must be >= 1");
   PQ_CHECK_ARG(ss != nullptr, "pq_inv_create: ss is null");
   if (mode == 0) {
@@ -233,12 +233,10 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
     pq_inv_destroy(h);
     return code;
   };
-  int rc = precision == PQ_PREC_FP64 ? upload_tables<double>(h, h->ei.M)
-                                     : upload_tables<float>(h, h->ei.M);
+  int rc = h->ei.elem == 8 ? upload_tables<double>(h, h->ei.M) : upload_tables<float>(h, h->ei.M);
   if (rc) return fail(rc);
   compute_coefs(h, ss, m2z, z2m);
-  const size_t el = h->ei.elem;
-  h->scr_member = (size_t)L * N * N * el;
+  h->scr_member = (size_t)L * N * N * h->ei.selem;
   if (hipMalloc(&h->coef, h->coef_h.size() * sizeof(double)) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMemcpy(h->coef, h->coef_h.data(), h->coef_h.size() * sizeof(double), hipMemcpyHostToDevice) !=
       hipSuccess)
@@ -454,7 +452,7 @@ static int run_host(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *p
                "pq_inv_run: leading dimensions smaller than g=%d", h->g);
   const int g = h->g, L = h->L;
   // FP64 storage for FP64 and mixed handles, FP32 for FP32 handles
-  const size_t el = h->prec == PQ_PREC_FP32 ? 4 : 8;
+  const size_t el = (h->prec == PQ_PREC_FP32 || h->prec == PQ_PREC_FP32_STORAGE) ? 4 : 8;
   void *mz = pq_mirror_device_ptr(zeta);
   void *mp = pq_mirror_device_ptr(psi);
   if (mz && mp && el == 8) {

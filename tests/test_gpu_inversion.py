@@ -13,7 +13,7 @@ import pytest
 
 from oracle import oracle as O
 from pequod_amd import qg, strat as S, synth
-from pequod_amd._lib import PQ_ERR_NAN, PqError
+from pequod_amd._lib import IO_BYTES, PQ_ERR_NAN, PqError
 
 from helpers import from_dev, random_strat, rel_err, to_dev
 
@@ -151,7 +151,7 @@ def test_inversion_sdata_views_and_batch(dev):
         assert np.all(out[0, :, :] == 0) and np.all(out[-1, :, :] == 0)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed", "fp32s"])
 @pytest.mark.parametrize("cfg", ["C1", "C2"])
 def test_inversion_reduced_precision_k3(dev, prec, cfg):
     """Seeded k^-3 PV on the C1 / C2 grids: max relative psi error <= 1e-6
@@ -165,7 +165,7 @@ def test_inversion_reduced_precision_k3(dev, prec, cfg):
     print(f"pv_inversion {prec} {cfg} g={g}", flush=True)
     ref = _oracle_inv(g, st, q)
     hc = qg.HelmholtzCache(g, st, prec)
-    dt = torch.float32 if prec == "fp32" else torch.float64
+    dt = torch.float32 if IO_BYTES[prec] == 4 else torch.float64
     zt = to_dev(q, dev, dt)
     pt = torch.zeros_like(zt)
     hc.run_dev([zt], [pt])
@@ -174,25 +174,29 @@ def test_inversion_reduced_precision_k3(dev, prec, cfg):
     err = rel_err(out, ref)
     print(f"{prec} rel err {err:.3e}")
     assert err < 1e-6
+    if prec == "fp32s":
+        # FP64 transforms: only the FP32 roundings of input, scratch and output remain
+        assert err < 3e-7
     for l in range(st.layers):
         assert abs(O.mean(out[:, :, l])) < 1e-6 * np.max(np.abs(ref[:, :, l]))
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp32s"])
 @pytest.mark.parametrize("g", [9, 13, 17, 25, 33, 49, 65, 97, 129, 193, 257, 385, 513, 769, 1025])
-def test_inversion_fp32_all_sizes(dev, g):
+def test_inversion_fp32_all_sizes(dev, g, prec):
     """FP32 (odd-extension transforms) on every kernel unit up to 1024:
     random white-noise PV with 3 layers, error vs the FP64 oracle.  White
     noise is the hardest input for relative error (flat spectrum), so the
     bound is 2e-6 here."""
     import torch
-    print(f"pv_inversion fp32 g={g}", flush=True)
+    print(f"pv_inversion {prec} g={g}", flush=True)
     rng = np.random.default_rng(77 + g)
     L = 3
     s1, s2 = random_strat(L, rng)
     st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
     z = np.asfortranarray(rng.standard_normal((g, g, L)))
     ref = _oracle_inv(g, st, z)
-    hc = qg.HelmholtzCache(g, st, "fp32")
+    hc = qg.HelmholtzCache(g, st, prec)
     zt = to_dev(z, dev, torch.float32)
     pt = torch.zeros_like(zt)
     hc.run_dev([zt], [pt])
@@ -200,7 +204,7 @@ def test_inversion_fp32_all_sizes(dev, g):
     assert rel_err(from_dev(pt), ref) < 2e-6
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32s"])
 def test_inversion_tiles_do_not_change_results(dev, prec):
     """Every accepted tile shape gives the same psi (to the psiC summation
     order, which depends on the column tile)."""
@@ -210,8 +214,8 @@ def test_inversion_tiles_do_not_change_results(dev, prec):
     s1, s2 = random_strat(L, rng)
     st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
     z = np.asfortranarray(rng.standard_normal((g, g, L)))
-    dt = torch.float32 if prec == "fp32" else torch.float64
-    hc = qg.HelmholtzCache(g, st, prec)
+    dt = torch.float32 if IO_BYTES[prec] == 4 else torch.float64
+    hc = qg.HelmholtzCache(g, st, prec, tiles=None)
     zt = to_dev(z, dev, dt)
     base = torch.zeros_like(zt)
     hc.run_dev([zt], [base])
