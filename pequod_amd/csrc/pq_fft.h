@@ -208,52 +208,73 @@ template <typename T, int M> struct TwBase {
 
 // One in-place Stockham stage of radix R: for j < M/R, k = j mod NS, the
 // inputs j + r M/R are twiddled by exp(-2 pi i r k / (NS R)), transformed,
-// and written to (j - k) R + k + r NS.
-template <typename T, int M, int S>
-__device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const TwBase<T, M> &tb) {
+// and written to (j - k) R + k + r NS.  Split into load / compute / store so
+// that the first stage can take its inputs from registers and the last stage
+// can leave its outputs there.
+template <typename T, int M, int S> struct Stage {
   using P = FftPlan<T, M>;
-  constexpr int E = P::E, TPF = P::TPF, R = P::S.rad[S], LNS = P::S.lns[S];
-  constexpr int NB = E / R;
-  constexpr int NR = M / R;
-  constexpr int NS = 1 << LNS;
-  cx<T> v[NB][R];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = t + b * TPF;
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(j + r * NR)];
+  static constexpr int E = P::E, TPF = P::TPF, R = P::S.rad[S], LNS = P::S.lns[S];
+  static constexpr int NB = E / R, NR = M / R, NS = 1 << LNS;
+  // element index of input r of block b of thread t
+  __device__ __forceinline__ static int in_index(int t, int b, int r) { return t + b * TPF + r * NR; }
+  // element index of output r of block b of thread t
+  __device__ __forceinline__ static int out_index(int t, int b, int r) {
+    const int j = t + b * TPF, k = j & (NS - 1);
+    return (j - k) * R + k + r * NS;
   }
-  group_sync<TPF>();
+  __device__ __forceinline__ static void load(const cx<T> *buf, int t, cx<T> (&v)[NB][R]) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int j = t + b * TPF;
-    const int k = j & (NS - 1);
-    if constexpr (LNS > 0) {
-      cx<double> w1 = tb.w[TwBase<T, M>::offset(S) + b];
-      // opaque copy: keeps the compiler from keeping every power of w alive
-      // between two transforms of one kernel (register spills)
-      asm volatile("" : "+v"(w1.x), "+v"(w1.y));
-      cx<double> wr = w1;
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        v[b][r] = cmul(v[b][r], cx<T>{T(wr.x), T(wr.y)});
-        if (r + 1 < R) wr = cmul(wr, w1);
+      for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(in_index(t, b, r))];
+  }
+  __device__ __forceinline__ static void compute(int t, const TwBase<T, M> &tb, cx<T> (&v)[NB][R]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if constexpr (LNS > 0) {
+        cx<double> w1 = tb.w[TwBase<T, M>::offset(S) + b];
+        // opaque copy: keeps the compiler from keeping every power of w
+        // alive between two transforms of one kernel (register spills)
+        asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+        cx<double> wr = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[b][r] = cmul(v[b][r], cx<T>{T(wr.x), T(wr.y)});
+          if (r + 1 < R) wr = cmul(wr, w1);
+        }
       }
+      dftR<T, R>(v[b]);
     }
-    dftR<T, R>(v[b]);
-    const int base = (j - k) * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) buf[pad16(base + r * NS)] = v[b][r];
   }
-  group_sync<TPF>();
-}
+  __device__ __forceinline__ static void store(cx<T> *buf, int t, const cx<T> (&v)[NB][R]) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int base = out_index(t, b, 0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad16(base + r * NS)] = v[b][r];
+    }
+  }
+};
 
 template <typename T, int M, int S>
+__device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const TwBase<T, M> &tb) {
+  using St = Stage<T, M, S>;
+  cx<T> v[St::NB][St::R];
+  St::load(buf, t, v);
+  group_sync<St::TPF>();
+  St::compute(t, tb, v);
+  St::store(buf, t, v);
+  group_sync<St::TPF>();
+}
+
+// Stages S .. SEND - 1 (SEND = -1: all remaining stages).
+template <typename T, int M, int S, int SEND = -1>
 __device__ __forceinline__ void fft_stages(cx<T> *buf, int t, const TwBase<T, M> &tb) {
   using P = FftPlan<T, M>;
-  if constexpr (S < P::S.nst) {
+  constexpr int END = SEND < 0 ? P::S.nst : SEND;
+  if constexpr (S < END) {
     fft_stage<T, M, S>(buf, t, tb);
-    fft_stages<T, M, S + 1>(buf, t, tb);
+    fft_stages<T, M, S + 1, SEND>(buf, t, tb);
   }
 }
 
@@ -428,37 +449,52 @@ template <typename T_, int N_> struct DstExt {
     }
   };
 
+  // The first stage takes its inputs straight from the rows (w_e = x_e for
+  // e < N, w_N = w_0 = 0, w_e = -x_{2N-e} for e > N; the loader is called
+  // once per element it feeds) and the last stage's outputs go straight to
+  // the rows (only the outputs k < N are needed), which saves two LDS round
+  // trips per transform.
   template <typename Loader>
   __device__ __forceinline__ static void run(T *region, int t, T *, const Ctx &cx_, Loader ld) {
+    static_assert(P::S.nst >= 2, "DstExt needs at least two FFT stages");
+    constexpr int LAST = P::S.nst - 1;
+    using S0 = Stage<T, M, 0>;
+    using SL = Stage<T, M, LAST>;
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
-    cx<T> v[H];
+    {
+      cx<T> v[S0::NB][S0::R];
 #pragma unroll
-    for (int e = 0; e < H; ++e) {
-      const int k = t + e * TPF;
-      T xa = T(0), xb = T(0);
-      if (k != 0) ld(k, xa, xb);
-      v[e] = {xa, xb};
+      for (int b = 0; b < S0::NB; ++b)
+#pragma unroll
+        for (int r = 0; r < S0::R; ++r) {
+          const int e = S0::in_index(t, b, r);
+          const bool lo = e < N;
+          const int k = lo ? e : M - e;
+          T xa = T(0), xb = T(0);
+          if (k != 0 && k != N) ld(k, xa, xb);
+          v[b][r] = lo ? cx<T>{xa, xb} : cx<T>{-xa, -xb};
+        }
+      group_sync<TPF>();  // loads that read the region are done
+      S0::compute(t, cx_.tw, v);
+      S0::store(buf, t, v);
+      group_sync<TPF>();
     }
+    fft_stages<T, M, 1, LAST>(buf, t, cx_.tw);
+    cx<T> v[SL::NB][SL::R];
+    SL::load(buf, t, v);
     group_sync<TPF>();
-#pragma unroll
-    for (int e = 0; e < H; ++e) {
-      const int k = t + e * TPF;
-      buf[pad16(k)] = v[e];  // w_k (w_0 = 0)
-      // w_{2N-k} = -w_k; the k = 0 thread writes w_N = 0 instead
-      buf[pad16(k == 0 ? N : M - k)] = k == 0 ? cx<T>{T(0), T(0)} : cx<T>{-v[e].x, -v[e].y};
-    }
-    group_sync<TPF>();
-    fft_stages<T, M, 0>(buf, t, cx_.tw);
-#pragma unroll
-    for (int e = 0; e < H; ++e) v[e] = buf[pad16(t + e * TPF)];
-    group_sync<TPF>();
+    SL::compute(t, cx_.tw, v);
     T *rowa = region, *rowb = region + RS;
 #pragma unroll
-    for (int e = 0; e < H; ++e) {
-      const int k = t + e * TPF;
-      rowa[pad16(k)] = k == 0 ? T(0) : -v[e].y;
-      rowb[pad16(k)] = k == 0 ? T(0) : v[e].x;
-    }
+    for (int b = 0; b < SL::NB; ++b)
+#pragma unroll
+      for (int r = 0; r < SL::R; ++r) {
+        const int k = SL::out_index(t, b, r);
+        if (k < N) {
+          rowa[pad16(k)] = k == 0 ? T(0) : -v[b][r].y;
+          rowb[pad16(k)] = k == 0 ? T(0) : v[b][r].x;
+        }
+      }
   }
 };
 
