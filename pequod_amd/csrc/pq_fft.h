@@ -206,32 +206,69 @@ template <typename T, int M> struct TwBase {
   }
 };
 
+// Padded LDS index of x + c for a per-thread x and a compile-time c that is
+// a multiple of 16: pad16(x + c) = pad16(x) + c + c / 16, so the constant
+// part folds into the LDS instruction's immediate offset.
+__device__ __forceinline__ int padx(int x) { return x + (x >> 4); }
+constexpr int padc(int c) { return c + c / 16; }
+
 // One in-place Stockham stage of radix R: for j < M/R, k = j mod NS, the
 // inputs j + r M/R are twiddled by exp(-2 pi i r k / (NS R)), transformed,
-// and written to (j - k) R + k + r NS.  Split into load / compute / store so
-// that the first stage can take its inputs from registers and the last stage
-// can leave its outputs there.
+// and written to (j - k) R + k + r NS.  Split into LDS loads, compute and
+// store, so that the first stage can take its inputs from registers and the
+// last stage can leave its outputs there.
 template <typename T, int M, int S> struct Stage {
   using P = FftPlan<T, M>;
   static constexpr int E = P::E, TPF = P::TPF, R = P::S.rad[S], LNS = P::S.lns[S];
   static constexpr int NB = E / R, NR = M / R, NS = 1 << LNS;
-  // element index of input r of block b of thread t
+  static constexpr int STEP = NR >> LNS;  // M / (NS R)
+  static constexpr bool LAST = NS * R == M;
+  static constexpr bool TW = LNS > 0;
+  // index shapes that allow per-thread base + constant offsets
+  static constexpr bool IN_FOLD = TPF % 16 == 0 && NR % 16 == 0;
+  static constexpr bool OUT_LAST_FOLD = LAST && TPF % 16 == 0 && NS % 16 == 0;
+  static constexpr bool OUT_MID_FOLD = !LAST && NS % 16 == 0 && TPF % NS == 0;
+  static constexpr bool OUT_FIRST_FOLD = NS == 1 && 16 % R == 0 && (TPF * R) % 16 == 0;
+
+
+  // element index of input r of block b of thread t, and its padded index
   __device__ __forceinline__ static int in_index(int t, int b, int r) { return t + b * TPF + r * NR; }
+  __device__ __forceinline__ static int in_pad(int t, int pt, int b, int r) {
+    if constexpr (IN_FOLD) return pt + padc(b * TPF + r * NR);
+    else return pad16(in_index(t, b, r));
+  }
   // element index of output r of block b of thread t
   __device__ __forceinline__ static int out_index(int t, int b, int r) {
     const int j = t + b * TPF, k = j & (NS - 1);
     return (j - k) * R + k + r * NS;
   }
+  // padded output index; ob is the per-thread base of out_base()
+  __device__ __forceinline__ static int out_base(int t) {
+    if constexpr (OUT_LAST_FOLD) return padx(t);
+    else if constexpr (OUT_MID_FOLD) {
+      const int k = t & (NS - 1);
+      return padx((t - k) * R + k);
+    } else if constexpr (OUT_FIRST_FOLD) return padx(t * R);
+    else return 0;
+  }
+  __device__ __forceinline__ static int out_pad(int t, int ob, int b, int r) {
+    if constexpr (OUT_LAST_FOLD) return ob + padc(b * TPF + r * NS);
+    else if constexpr (OUT_MID_FOLD) return ob + padc(b * TPF * R + r * NS);
+    else if constexpr (OUT_FIRST_FOLD) return ob + padc(b * TPF * R) + r;
+    else return pad16(out_index(t, b, r));
+  }
+
   __device__ __forceinline__ static void load(const cx<T> *buf, int t, cx<T> (&v)[NB][R]) {
+    const int pt = padx(t);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[pad16(in_index(t, b, r))];
+      for (int r = 0; r < R; ++r) v[b][r] = buf[in_pad(t, pt, b, r)];
   }
-  __device__ __forceinline__ static void compute(int t, const TwBase<T, M> &tb, cx<T> (&v)[NB][R]) {
+  __device__ __forceinline__ static void compute(const TwBase<T, M> &tb, cx<T> (&v)[NB][R]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      if constexpr (LNS > 0) {
+      if constexpr (TW) {
         cx<double> w1 = tb.w[TwBase<T, M>::offset(S) + b];
         // opaque copy: keeps the compiler from keeping every power of w
         // alive between two transforms of one kernel (register spills)
@@ -247,12 +284,11 @@ template <typename T, int M, int S> struct Stage {
     }
   }
   __device__ __forceinline__ static void store(cx<T> *buf, int t, const cx<T> (&v)[NB][R]) {
+    const int ob = out_base(t);
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int base = out_index(t, b, 0);
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[pad16(base + r * NS)] = v[b][r];
-    }
+      for (int r = 0; r < R; ++r) buf[out_pad(t, ob, b, r)] = v[b][r];
   }
 };
 
@@ -262,7 +298,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const TwBase<T, M> 
   cx<T> v[St::NB][St::R];
   St::load(buf, t, v);
   group_sync<St::TPF>();
-  St::compute(t, tb, v);
+  St::compute(tb, v);
   St::store(buf, t, v);
   group_sync<St::TPF>();
 }
@@ -323,7 +359,8 @@ __device__ __forceinline__ void group_excl_scan2(T &a, T &b, int t, T *tmp) {
 // ------------------------------------------------------------ DST engines
 // Interface: a group's LDS region of Region elements of T holds the two rows
 // a = region[0 .. RS) and b = region[RS .. 2 RS) (padded slots pad16(k),
-// k < N) on output: slot k = X_k, slot 0 = 0.  The loader ld(k, xa, xb),
+// k < N) on output: slot k = X_k, slot 0 = 0.  The loader ld(k, kp, xa, xb)
+// (kp = pad16(k), for loaders that read padded LDS rows),
 // called for 1 <= k <= N - 1 (and k = N/2 by DstSint), may read LDS that
 // aliases the region: all loads happen before the first store.  Only
 // group-level synchronisation happens inside; callers synchronise the
@@ -368,13 +405,13 @@ template <typename T_, int N_> struct DstSint {
       const int k = t + e * TPF;
       if (k == 0) {
         T xa, xb;
-        ld(N / 2, xa, xb);
+        ld(N / 2, pad16(N / 2), xa, xb);
         ylo[e] = {T(0), T(0)};
         yhi[e] = {T(4) * xa, T(4) * xb};
       } else {
         T xka, xkb, xca, xcb;
-        ld(k, xka, xkb);
-        ld(N - k, xca, xcb);
+        ld(k, pad16(k), xka, xkb);
+        ld(N - k, pad16(N - k), xca, xcb);
         const T w = cx_.was[e];
         const T t1a = xka - xca, t2a = w * (xka + xca);
         const T t1b = xkb - xcb, t2b = w * (xkb + xcb);
@@ -463,6 +500,7 @@ template <typename T_, int N_> struct DstExt {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
     {
       cx<T> v[S0::NB][S0::R];
+      const int pt = padx(t);
 #pragma unroll
       for (int b = 0; b < S0::NB; ++b)
 #pragma unroll
@@ -470,12 +508,13 @@ template <typename T_, int N_> struct DstExt {
           const int e = S0::in_index(t, b, r);
           const bool lo = e < N;
           const int k = lo ? e : M - e;
+          const int kp = lo ? S0::in_pad(t, pt, b, r) : pad16(k);
           T xa = T(0), xb = T(0);
-          if (k != 0 && k != N) ld(k, xa, xb);
+          if (k != 0 && k != N) ld(k, kp, xa, xb);
           v[b][r] = lo ? cx<T>{xa, xb} : cx<T>{-xa, -xb};
         }
       group_sync<TPF>();  // loads that read the region are done
-      S0::compute(t, cx_.tw, v);
+      S0::compute(cx_.tw, v);
       S0::store(buf, t, v);
       group_sync<TPF>();
     }
@@ -483,16 +522,18 @@ template <typename T_, int N_> struct DstExt {
     cx<T> v[SL::NB][SL::R];
     SL::load(buf, t, v);
     group_sync<TPF>();
-    SL::compute(t, cx_.tw, v);
+    SL::compute(cx_.tw, v);
     T *rowa = region, *rowb = region + RS;
+    const int ob = SL::out_base(t);
 #pragma unroll
     for (int b = 0; b < SL::NB; ++b)
 #pragma unroll
       for (int r = 0; r < SL::R; ++r) {
         const int k = SL::out_index(t, b, r);
         if (k < N) {
-          rowa[pad16(k)] = k == 0 ? T(0) : -v[b][r].y;
-          rowb[pad16(k)] = k == 0 ? T(0) : v[b][r].x;
+          const int kp = SL::out_pad(t, ob, b, r);
+          rowa[kp] = k == 0 ? T(0) : -v[b][r].y;
+          rowb[kp] = k == 0 ? T(0) : v[b][r].x;
         }
       }
   }

@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   const Tin *za = z + (size_t)ja * ptrs.ldz1;
   const Tin *zb = za + ptrs.ldz1;
   const bool va = ja > 0;  // node row 0 is the boundary: not transformed
-  Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
+  Eng::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
     xa = va ? T(za[k]) : T(0);
     xb = T(zb[k]);
   });
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   const Ts *sa = scr + (size_t)la * N * N + (size_t)pa * R;
   const Ts *sb = scr + (size_t)lb * N * N + (size_t)pb * R;
   const int blk = N << lgR;
-  Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
+  Eng::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
     const int off = (k >> lgR) * blk + (k & (R - 1));
     xa = va ? T(sa[off]) : T(0);
     xb = vb ? T(sb[off]) : T(0);
@@ -307,9 +307,9 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   // Inverse DST along j, in place.
   {
     const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
-      xa = rowa[pad16(k)];
-      xb = rowb[pad16(k)];
+    Eng::run(region, t, scan_tmp, ctx, [&](int, int kp, T &xa, T &xb) {
+      xa = rowa[kp];
+      xb = rowb[kp];
     });
   }
   __syncthreads();
@@ -455,9 +455,9 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
 
   {
     const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::run(region, t, scan_tmp, ctx, [&](int k, T &xa, T &xb) {
-      xa = rowa[pad16(k)];
-      xb = rowb[pad16(k)];
+    Eng::run(region, t, scan_tmp, ctx, [&](int, int kp, T &xa, T &xb) {
+      xa = rowa[kp];
+      xb = rowb[kp];
     });
   }
   __syncthreads();
