@@ -44,6 +44,8 @@ struct InvArgs {
   double *part;
   int part_stride;
   double *psic;      // L per member: psiC of the last run
+  double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m
+  int kb_stride;
   int *status;       // sticky per member: 1 NaN psiC, 2 |psiC| > 1e100
   const double *coef;  // coefficient block shared by the members (CoefLayout)
   const void *tw64;  // cx<double>[M]: exp(-2 pi i n / M), M the engine's FFT length
@@ -86,7 +88,7 @@ inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups) {
   size_t b = (size_t)e.elem * groups * (e.region + e.scan);
   b = (b + 15) & ~size_t(15);
   if (pass == 1) b += 64 * 8 + (size_t)e.elem * (64 + 64 + 8);
-  if (pass == 2) b += 64 * 8 + 8 * 8;
+  if (pass == 2) b += 16;
   return b;
 }
 

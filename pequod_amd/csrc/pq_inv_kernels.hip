@@ -10,8 +10,9 @@
 //               (2N)^2 (lambda_p + lambda_q - ss_m), accumulate the FP64
 //               partial of the domain mean psiC_m, project modes -> layers
 //               (modes_to_z), inverse DST-I along j, write back in place.
-//   k_rows_inv  per (member, layer, tile): reduce the psiC partials in a
-//               fixed order, read the tile block, subtract the mass
+//   k_psic      (host file) per member: psiC from the column pass's partials,
+//               kappa_l and beta_lm.
+//   k_rows_inv  per (member, layer, tile): read the tile block, subtract the mass
 //               correction in the i-spectral domain (closed form of
 //               helm(1)), inverse DST-I along i, subtract the constant part,
 //               write psi rows including the boundary.
@@ -347,8 +348,6 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   const int ngrp = R >> 1, nt = ngrp * TPF;
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
-  const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
-  double *red = reinterpret_cast<double *>(smem + red_off);
   const CoefLayout CL{L, N};
 
   // Issue the tile block and psiC partial loads first.
@@ -356,16 +355,17 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   VT tile[NVT];
 #pragma unroll
   for (int s = 0; s < NVT; ++s) tile[s] = *reinterpret_cast<const VT *>(in + (threadIdx.x + s * nt) * V);
-  double pc[8];
+  // kappa_l and beta_lm of this member and layer (written by k_psic)
+  double kappa = 0.0;
+  double beta[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) pc[m] = 0.0;
+  for (int m = 0; m < 8; ++m) beta[m] = 0.0;
   if (a.mode == 0) {
-    const double *part = a.part + (size_t)b * a.part_stride;
-    const int nblk = N >> a.lgC;
-    for (int q = threadIdx.x; q < nblk; q += nt)
+    const double *kb = a.kb + (size_t)b * a.kb_stride;
+    kappa = kb[l];
 #pragma unroll
-      for (int m = 0; m < 8; ++m)
-        if (m < L) pc[m] += part[(size_t)q * L + m];
+    for (int m = 1; m < 8; ++m)
+      if (m < L) beta[m] = kb[L + l * L + m];
   }
   typename Eng::Ctx ctx;
   ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
@@ -378,39 +378,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
     for (int q = 0; q < V; ++q) row_ptr<Eng>(lds, r0 + q)[pad16(p)] = T(tile[s][q]);
   }
 
-  // psiC_m of the member (the same fixed-order reduction in every
-  // workgroup), then the correction constants (FP64).
-  double kappa = 0.0;
-  double beta[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) beta[m] = 0.0;
-  if (a.mode == 0) {
-    block_sum(pc, L, red);  // contains the barrier that publishes the tile
-    const double inv_n2 = 1.0 / (double(N) * double(N));
-    if (jb == 0 && l == 0 && threadIdx.x == 0) {
-      int st = 0;
-      for (int m = 0; m < L; ++m) {
-        const double v = pc[m] * inv_n2;
-        a.psic[(size_t)b * L + m] = v;
-        if (v != v) st |= 1;
-        else if (fabs(v) > 1.0e100) st |= 2;
-      }
-      if (st) atomicOr(a.status + b, st);
-    }
-    const double *M = a.coef + CL.M();
-    const double *ca = a.coef + CL.a();
-    const double *cb = a.coef + CL.b();
-    kappa = pc[0] * inv_n2;
-#pragma unroll
-    for (int m = 1; m < 8; ++m)
-      if (m < L) {
-        const double psc = pc[m] * inv_n2;
-        kappa += M[l + L * m] * psc * ca[m];
-        beta[m] = M[l + L * m] * psc * cb[m];
-      }
-  } else {
-    __syncthreads();
-  }
+  __syncthreads();  // publishes the tile
 
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
   if (a.mode == 0 && L > 1) {
@@ -423,6 +391,15 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
       for (int m = 1; m < L; ++m) {
         const double *e = tab + ((size_t)(m - 1) * (N / 2) + h) * 3;
         const double rho = e[0], gam = beta[m] * e[1], eta = e[2];
+        // r^j for the tile's rows is below e^-40 (relative 4e-18) far from
+        // both edges: then only the constant part remains
+        const int jmin = min(j0, N - j0 - R + 1);
+        if (double(jmin) * -rho > 40.0) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            if (r < R) acc[r] += gam;
+          continue;
+        }
         const double rr = exp(rho);
         double hh[16];
         double u = exp(double(j0) * rho);  // r^(j0 + r), going up
@@ -467,8 +444,10 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   constexpr int G = N + 1;
   const bool last = (j0 + R == N);
   const int nrow = R + (last ? 1 : 0);
+  // (r, i) of idx = r G + i, advanced by nt per iteration without a division
+  int r = (int)threadIdx.x / G, i = (int)threadIdx.x - r * G;
   for (int idx = threadIdx.x; idx < nrow * G; idx += nt) {
-    const int r = idx / G, i = idx - r * G, j = j0 + r;
+    const int j = j0 + r;
     Tout *o = ps + (size_t)j * ld1 + i;
     if (a.mode == 0) {
       double val;
@@ -477,12 +456,18 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
       *o = Tout(val);
     } else {
       // helm's boundary semantics (qg3p_helmholtz.F90:757-761, 1055-1057)
-      if (j == N || i == 0) continue;
-      if (j == 0) {
-        if (i <= N - 1) *o = Tout(0);
-      } else {
-        *o = i == N ? Tout(0) : Tout(row_ptr<Eng>(lds, r)[pad16(i)]);
+      if (j != N && i != 0) {
+        if (j == 0) {
+          if (i <= N - 1) *o = Tout(0);
+        } else {
+          *o = i == N ? Tout(0) : Tout(row_ptr<Eng>(lds, r)[pad16(i)]);
+        }
       }
+    }
+    i += nt;
+    while (i >= G) {
+      i -= G;
+      ++r;
     }
   }
 }

@@ -51,6 +51,61 @@ static int ilog2i(int x) {
   return r;
 }
 
+// psiC of each member from the column pass's per-workgroup partials, summed
+// in a fixed order (FP64), then the constants of the row-inverse pass:
+// kappa_l = psiC_1 + sum_{m>=2} M(l,m) psiC_m a_m and
+// beta_lm = M(l,m) psiC_m b_m (qg3p_diagnostics.F90:615-635 rewritten for
+// the closed-form correction).  Status bits: 1 NaN psiC, 2 |psiC| > 1e100
+// (qg3p_aux.F90:504-517).  Grid: one 256-thread workgroup per member.
+__global__ void __launch_bounds__(256) k_psic(InvArgs a, int nblk, int N) {
+  __shared__ double red[4 * 8];
+  __shared__ double s_psic[8];
+  const int b = blockIdx.x, L = a.L, tid = threadIdx.x;
+  const double *part = a.part + (size_t)b * a.part_stride;
+  double acc[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) acc[m] = 0.0;
+  for (int q = tid; q < nblk; q += 256)
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      if (m < L) acc[m] += part[(size_t)q * L + m];
+  const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+    if (m < L) {
+      double x = acc[m];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
+      if (lane == 0) red[w * 8 + m] = x;
+    }
+  __syncthreads();
+  if (tid == 0) {
+    const double inv_n2 = 1.0 / (double(N) * double(N));
+    int st = 0;
+    for (int m = 0; m < L; ++m) {
+      const double v = (red[m] + red[8 + m] + red[16 + m] + red[24 + m]) * inv_n2;
+      s_psic[m] = v;
+      a.psic[(size_t)b * L + m] = v;
+      if (v != v) st |= 1;
+      else if (fabs(v) > 1.0e100) st |= 2;
+    }
+    if (st) atomicOr(a.status + b, st);
+  }
+  __syncthreads();
+  const CoefLayout CL{L, N};
+  const double *M = a.coef + CL.M(), *ca = a.coef + CL.a(), *cb = a.coef + CL.b();
+  double *kb = a.kb + (size_t)b * a.kb_stride;
+  if (tid < L) {
+    const int l = tid;
+    double kap = s_psic[0];
+    for (int m = 1; m < L; ++m) {
+      kap += M[l + L * m] * s_psic[m] * ca[m];
+      kb[L + l * L + m] = M[l + L * m] * s_psic[m] * cb[m];
+    }
+    kb[l] = kap;
+  }
+}
+
 constexpr size_t kLdsMax = 160 * 1024;
 constexpr int kThreadsMax = 512;
 
@@ -71,7 +126,7 @@ struct pq_inv {
   std::vector<double> coef_h;
   double *coef = nullptr;
   void *scr = nullptr;
-  double *part = nullptr, *psic = nullptr;
+  double *part = nullptr, *psic = nullptr, *kb = nullptr;
   int *status = nullptr;
   size_t scr_member = 0;  // scratch bytes per member
   int part_cap = 0;       // psiC partial slots per member (N / C_min)
@@ -248,6 +303,8 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   if (hipMalloc(&h->part, sizeof(double) * (size_t)h->part_cap * L * batch_max) != hipSuccess)
     return fail(PQ_ERR_NOMEM);
   if (hipMalloc(&h->psic, sizeof(double) * (size_t)L * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
+  if (hipMalloc(&h->kb, sizeof(double) * (size_t)(L + L * L) * batch_max) != hipSuccess)
+    return fail(PQ_ERR_NOMEM);
   if (hipMalloc(&h->status, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMemset(h->status, 0, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
   if (hipMemset(h->psic, 0, sizeof(double) * (size_t)L * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
@@ -267,6 +324,8 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.part_stride = h->part_cap * L;
   a.psic = h->psic + (size_t)L * mem0;
   a.status = h->status + mem0;
+  a.kb = h->kb + (size_t)(L + L * L) * mem0;
+  a.kb_stride = L + L * L;
   a.coef = h->coef;
   a.tw64 = h->tw64;
   a.was = h->was;
@@ -292,6 +351,14 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
       blocks = nb * L * (N / h->R);
       threads = rows_threads;
       lds = inv_lds_bytes(e, pass, h->R / 2);
+    }
+    if (pass == 2 && h->mode == 0) {
+      hipLaunchKernelGGL(k_psic, dim3(nb), dim3(256), 0, s, a, N / h->C, N);
+      hipError_t e2 = hipGetLastError();
+      if (e2 != hipSuccess) {
+        pq_set_error("pq_inv: launch of the psiC reduction failed: %s", hipGetErrorString(e2));
+        return PQ_ERR_HIP;
+      }
     }
     hipError_t err = h->ks.launch(h->prec, pass, a, pt, blocks, threads, lds, s);
     if (err != hipSuccess) {
@@ -347,6 +414,7 @@ int pq_inv_destroy(pq_inv *h) {
   (void)hipFree(h->scr);
   (void)hipFree(h->part);
   (void)hipFree(h->psic);
+  (void)hipFree(h->kb);
   (void)hipFree(h->status);
   (void)hipFree(h->dz);
   (void)hipFree(h->dp);
