@@ -395,23 +395,49 @@ template <typename T_, int N_> struct DstSint {
     }
   };
 
+  // Raw inputs of a thread: x_k and x_{N-k} of both rows for k = t + e TPF
+  // (x_{N/2} in ka/kb for k = 0).
+  struct In {
+    T ka[H], kb[H], ca[H], cb[H];
+  };
+
+  template <typename Loader>
+  __device__ __forceinline__ static void load_in(int t, Loader ld, In &in) {
+#pragma unroll
+    for (int e = 0; e < H; ++e) {
+      const int k = t + e * TPF;
+      if (k == 0) {
+        ld(N / 2, pad16(N / 2), in.ka[e], in.kb[e]);
+        in.ca[e] = in.cb[e] = T(0);
+      } else {
+        ld(k, pad16(k), in.ka[e], in.kb[e]);
+        ld(N - k, pad16(N - k), in.ca[e], in.cb[e]);
+      }
+    }
+  }
+
   template <typename Loader>
   __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
                                              Loader ld) {
+    In in;
+    load_in(t, ld, in);
+    run_in(region, t, scan_tmp, cx_, in);
+  }
+
+  // The transform of loaded inputs; the group's region may still be read by
+  // the loader until this call (it synchronises the group first).
+  __device__ __forceinline__ static void run_in(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                                const In &in) {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
     cx<T> ylo[H], yhi[H];
 #pragma unroll
     for (int e = 0; e < H; ++e) {
       const int k = t + e * TPF;
       if (k == 0) {
-        T xa, xb;
-        ld(N / 2, pad16(N / 2), xa, xb);
         ylo[e] = {T(0), T(0)};
-        yhi[e] = {T(4) * xa, T(4) * xb};
+        yhi[e] = {T(4) * in.ka[e], T(4) * in.kb[e]};
       } else {
-        T xka, xkb, xca, xcb;
-        ld(k, pad16(k), xka, xkb);
-        ld(N - k, pad16(N - k), xca, xcb);
+        const T xka = in.ka[e], xkb = in.kb[e], xca = in.ca[e], xcb = in.cb[e];
         const T w = cx_.was[e];
         const T t1a = xka - xca, t2a = w * (xka + xca);
         const T t1b = xkb - xcb, t2b = w * (xkb + xcb);
@@ -491,33 +517,49 @@ template <typename T_, int N_> struct DstExt {
   // once per element it feeds) and the last stage's outputs go straight to
   // the rows (only the outputs k < N are needed), which saves two LDS round
   // trips per transform.
+  static_assert(P::S.nst >= 2, "DstExt needs at least two FFT stages");
+  static constexpr int LAST = P::S.nst - 1;
+  using S0 = Stage<T, M, 0>;
+  using SL = Stage<T, M, LAST>;
+
+  // Raw inputs of a thread: the first stage's elements of the extension.
+  struct In {
+    cx<T> v[S0::NB][S0::R];
+  };
+
   template <typename Loader>
-  __device__ __forceinline__ static void run(T *region, int t, T *, const Ctx &cx_, Loader ld) {
-    static_assert(P::S.nst >= 2, "DstExt needs at least two FFT stages");
-    constexpr int LAST = P::S.nst - 1;
-    using S0 = Stage<T, M, 0>;
-    using SL = Stage<T, M, LAST>;
+  __device__ __forceinline__ static void load_in(int t, Loader ld, In &in) {
+    const int pt = padx(t);
+#pragma unroll
+    for (int b = 0; b < S0::NB; ++b)
+#pragma unroll
+      for (int r = 0; r < S0::R; ++r) {
+        const int e = S0::in_index(t, b, r);
+        const bool lo = e < N;
+        const int k = lo ? e : M - e;
+        const int kp = lo ? S0::in_pad(t, pt, b, r) : pad16(k);
+        T xa = T(0), xb = T(0);
+        if (k != 0 && k != N) ld(k, kp, xa, xb);
+        in.v[b][r] = lo ? cx<T>{xa, xb} : cx<T>{-xa, -xb};
+      }
+  }
+
+  template <typename Loader>
+  __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                             Loader ld) {
+    In in;
+    load_in(t, ld, in);
+    run_in(region, t, scan_tmp, cx_, in);
+  }
+
+  // The transform of loaded inputs; the group's region may still be read by
+  // the loader until this call (it synchronises the group first).
+  __device__ __forceinline__ static void run_in(T *region, int t, T *, const Ctx &cx_, In in) {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
-    {
-      cx<T> v[S0::NB][S0::R];
-      const int pt = padx(t);
-#pragma unroll
-      for (int b = 0; b < S0::NB; ++b)
-#pragma unroll
-        for (int r = 0; r < S0::R; ++r) {
-          const int e = S0::in_index(t, b, r);
-          const bool lo = e < N;
-          const int k = lo ? e : M - e;
-          const int kp = lo ? S0::in_pad(t, pt, b, r) : pad16(k);
-          T xa = T(0), xb = T(0);
-          if (k != 0 && k != N) ld(k, kp, xa, xb);
-          v[b][r] = lo ? cx<T>{xa, xb} : cx<T>{-xa, -xb};
-        }
-      group_sync<TPF>();  // loads that read the region are done
-      S0::compute(cx_.tw, v);
-      S0::store(buf, t, v);
-      group_sync<TPF>();
-    }
+    group_sync<TPF>();
+    S0::compute(cx_.tw, in.v);
+    S0::store(buf, t, in.v);
+    group_sync<TPF>();
     fft_stages<T, M, 1, LAST>(buf, t, cx_.tw);
     cx<T> v[SL::NB][SL::R];
     SL::load(buf, t, v);
