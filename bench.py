@@ -1,7 +1,7 @@
 """Benchmark of the PV (modal Helmholtz) inversion on one MI355X.
 
 Metric: 3-layer 1023^2 Helmholtz inversions/s at <= 1e-6 relative psi error
-vs FP64, with p50/p99 latency.  Workload (C2 grid): 3-layer QG, interior
+vs FP64, with p50/p99 latency.  Workload: 3-layer QG, interior
 1023 x 1023 (G = 1025), dx = 5 km, H = [300, 700, 3000] m,
 g' = [0.02, 0.01] m/s^2, seeded k^-3 PV with RMS 1e-5 s^-1.
 
@@ -73,7 +73,7 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copy probe: 10 copies of {args.copy_probe} MiB")
         return
-    c = synth.config("C2")
+    c = synth.config("qg3_1023")
     st = c.stratification()
     g, L, B = c.g, st.layers, args.batch
     q = c.q()  # (G, G, L) Fortran order, non-dimensional
@@ -139,7 +139,7 @@ def main():
                 hc.deallocate()
         return
 
-    log(f"[bench] C2 {args.precision} batch {B}: {args.steps} steps")
+    log(f"[bench] 3-layer 1023^2 {args.precision} batch {B}: {args.steps} steps")
     hc, zs, ps, wall, base = run_rate(args.precision, args.steps, args.warmup, B)
     ms_per_step = 1000.0 * wall / args.steps
     value = B * args.steps / wall
@@ -188,7 +188,7 @@ def main():
         _, _, _, w64, _ = run_rate("fp64", max(20, args.steps // 4), 5, B)
         fp64_rate = B * max(20, args.steps // 4) / w64
 
-    # device-resident FP64 CABARET model (C2 grid): steps/s
+    # device-resident FP64 CABARET model (same grid): steps/s
     cab = None
     if not args.no_cabaret:
         nd = c.nondim()
@@ -220,7 +220,7 @@ def main():
             n_cpu += 1
         cpu_rate = n_cpu / (time.perf_counter() - t0)
         cpu = {"value": round(cpu_rate, 3), "unit": "inversions/s", "cores": 1, "kind": "port",
-               "sample": f"{n_cpu} C2 inversions (1023^2 x 3, FP64) in ~10 s on one host core"}
+               "sample": f"{n_cpu} inversions (3-layer 1023^2, FP64) in ~10 s on one host core"}
         # error of the measured path on the unscaled seeded field
         out = torch.empty_like(base)
         hc.run_dev([base], [out])
@@ -243,7 +243,7 @@ def main():
         "dtype": {"fp32": "f32", "fp32s": "f32 storage, f64 transforms",
                   "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
-        "config": {"workload": "C2: 3-layer QG 1023^2 interior (G=1025), dx 5 km, PV inversion",
+        "config": {"workload": "3-layer QG, 1023^2 interior (G=1025), dx 5 km, PV inversion",
                    "batch_per_step": B, "precision": args.precision, "ring_buffers": max(args.ring, B),
                    "tiles_RC": list(hc.tiles())},
         "p50_ms": None if p50 is None else round(p50, 4),

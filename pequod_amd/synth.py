@@ -118,24 +118,26 @@ class QGConfig:
 
 
 def config(name: str) -> QGConfig:
-    """The hot path's workloads: C1 (CPU-runnable 255^2 x 3), C2 (1023^2 x 3
-    FP64 drop-in), C3 (2047^2 x 6 mixed precision)."""
-    if name == "C1":
-        return QGConfig("C1: 3-layer 255^2, dx 10 km", 255, 10e3, [300.0, 700.0, 3000.0],
+    """The hot path's workloads by name: "qg3_255" (3-layer, 255^2 interior,
+    dx 10 km; CPU-runnable), "qg3_1023" (3-layer, 1023^2, dx 5 km) and
+    "qg6_2047" (6-layer, 2047^2, dx 2.5 km)."""
+    if name == "qg3_255":
+        return QGConfig("3-layer 255^2, dx 10 km", 255, 10e3, [300.0, 700.0, 3000.0],
                         [0.02, 0.01], config_id=0)
-    if name == "C2":
-        return QGConfig("C2: 3-layer 1023^2, dx 5 km", 1023, 5e3, [300.0, 700.0, 3000.0],
+    if name == "qg3_1023":
+        return QGConfig("3-layer 1023^2, dx 5 km", 1023, 5e3, [300.0, 700.0, 3000.0],
                         [0.02, 0.01], config_id=1)
-    if name == "C3":
-        return QGConfig("C3: 6-layer 2047^2, dx 2.5 km", 2047, 2.5e3,
+    if name == "qg6_2047":
+        return QGConfig("6-layer 2047^2, dx 2.5 km", 2047, 2.5e3,
                         [150.0, 250.0, 400.0, 700.0, 1200.0, 2300.0],
                         list(np.linspace(0.03, 0.005, 5)), config_id=2)
     raise KeyError(name)
 
 
 def ensemble_member(member: int, seed_base: int = 3):
-    """C4 member: interior size from {255, 511, 767, 1023}, 2..8 layers, H
-    geometric summing to 4000 m, g' linear 0.03 -> 0.005, dx 5 km."""
+    """Member of the 64-member mixed ensemble: interior size from
+    {255, 511, 767, 1023}, 2..8 layers, H geometric summing to 4000 m,
+    g' linear 0.03 -> 0.005, dx 5 km, PV amplitude 1e-5 s^-1 +-10 %."""
     rng = np.random.Generator(np.random.Philox(seed_for(seed_base, member)))
     n = int(rng.choice([255, 511, 767, 1023]))
     L = int(rng.integers(2, 9))
@@ -144,5 +146,5 @@ def ensemble_member(member: int, seed_base: int = 3):
     H = list(4000.0 * H / H.sum())
     gp = list(np.linspace(0.03, 0.005, L - 1))
     amp = float(rng.uniform(0.9, 1.1))
-    return QGConfig(f"C4 member {member}", n, 5e3, H, gp, rms=1e-5 * amp,
+    return QGConfig(f"ensemble member {member}", n, 5e3, H, gp, rms=1e-5 * amp,
                     config_id=seed_base, extra={"member": member})

@@ -175,9 +175,9 @@ def _models(cfg_name, g_override=None, adapt=False):
 
 @pytest.mark.parametrize("adapt", [False, True])
 def test_model_steps_match_oracle(dev, adapt):
-    """10 full steps + a final partial step of the C1 physics on a 65^2 grid
+    """10 full steps + a final partial step of the 3-layer 255^2 physics on a 65^2 grid
     (Courant number about 0.3), FP64, vs the oracle."""
-    c, gm, om, q = _models("C1", g_override=65, adapt=adapt)
+    c, gm, om, q = _models("qg3_255", g_override=65, adapt=adapt)
     gm.init(q)
     om.init(q)
     for name in ("psi", "zeta", "rel"):

@@ -253,7 +253,7 @@ def test_cabaret_cfl_known_answers():
 
 
 def _small_model(g=33, adapt=False, wind=True):
-    c = synth.config("C1")
+    c = synth.config("qg3_255")
     c.n = g - 2
     nd = c.nondim()
     st = nd["strat"]
