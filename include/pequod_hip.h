@@ -118,6 +118,12 @@ int pq_inv_run(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *psi,
 int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1,
                    long ldz2, void *const *psi, long ldp1, long ldp2,
                    void *stream);
+/* Per-member stratification (no counterpart in the reference, whose cache
+ * holds one): member slot `slot` of later runs uses these ss / modes_to_z /
+ * z_to_modes instead of the creation ones.  For ensembles whose members
+ * share a grid size and layer count but not a stratification.  Synchronous. */
+int pq_inv_set_member(pq_inv *h, int slot, const double *ss,
+                      const double *modes_to_z, const double *z_to_modes);
 /* Waits for the handle's last run and returns its status (NaN / large
  * psiC); psic (may be NULL) receives batch*L values of the last run.       */
 int pq_inv_status(pq_inv *h, double *psic);
@@ -227,6 +233,48 @@ int pq_model_set(pq_model *m, int which, const double *host, size_t count);
 /* dt, tweight_a, tweight_u, model_time */
 int pq_model_scalars(pq_model *m, double *out4);
 void *pq_model_stream(pq_model *m);
+
+/* ======================================================================
+ * Ensembles (no counterpart in the reference, which runs one member per
+ * MPI job): independent members of mixed grid sizes, layer counts and
+ * stratifications on one GPU.  Members with the same grid, layer count and
+ * leading dimensions share a batched inversion handle (a size class) with
+ * per-member stratifications; classes run concurrently.  The inversion of
+ * all members, or one time step of all CABARET model members, can be
+ * captured into a HIP graph and replayed.
+ * ==================================================================== */
+typedef struct pq_ens pq_ens;
+/* precision: PQ_PREC_* of every class (FP64 storage, i.e. PQ_PREC_FP64 or
+ * PQ_PREC_MIXED, when model members are added).                          */
+int pq_ens_create(pq_ens **e, int precision);
+/* A member whose zeta / psi are device g x g x layers views (types as for
+ * pq_inv_run_dev); index receives its number.                            */
+int pq_ens_add(pq_ens *e, int g, int layers, const double *ss,
+               const double *modes_to_z, const double *z_to_modes,
+               const void *zeta, long ldz1, long ldz2, void *psi, long ldp1,
+               long ldp2, int *index);
+/* A CABARET model member (fixed dt); the ensemble inverts its zeta into its
+ * psi and steps it with pq_ens_step.                                      */
+int pq_ens_add_model(pq_ens *e, pq_model *m, const double *ss,
+                     const double *modes_to_z, const double *z_to_modes,
+                     int *index);
+/* Groups the members into size classes and creates their handles.        */
+int pq_ens_finalize(pq_ens *e);
+int pq_ens_class_count(pq_ens *e);
+pq_inv *pq_ens_class_handle(pq_ens *e, int k);  /* e.g. for pq_inv_set_tiles */
+int pq_ens_member_class(pq_ens *e, int member, int *cls, int *slot);
+void *pq_ens_stream(pq_ens *e);
+/* All members' inversions (asynchronous on the ensemble's stream).        */
+int pq_ens_invert(pq_ens *e);
+/* nsteps time steps of the model members (asynchronous).                  */
+int pq_ens_step(pq_ens *e, int nsteps);
+/* what = 0: capture the inversion, 1: capture one model step; later
+ * pq_ens_invert / pq_ens_step replay the graph.                           */
+int pq_ens_capture(pq_ens *e, int what);
+int pq_ens_synchronize(pq_ens *e);
+/* Waits and returns the worst status of the last inversions.              */
+int pq_ens_status(pq_ens *e);
+int pq_ens_destroy(pq_ens *e);
 
 #ifdef __cplusplus
 }

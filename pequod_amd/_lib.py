@@ -44,6 +44,7 @@ SIGNATURES = {
     "pq_inv_status": (_i, [_vp, _dp]),
     "pq_inv_stream": (_vp, [_vp]),
     "pq_inv_set_tiles": (_i, [_vp, _i, _i]),
+    "pq_inv_set_member": (_i, [_vp, _i, _dp, _dp, _dp]),
     "pq_inv_get_tiles": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "pq_inv_pass_times": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _i,
                                ctypes.POINTER(ctypes.c_float)]),
@@ -66,6 +67,20 @@ SIGNATURES = {
     "pq_model_set": (_i, [_vp, _i, _dp, _sz]),
     "pq_model_scalars": (_i, [_vp, _dp]),
     "pq_model_stream": (_vp, [_vp]),
+    "pq_ens_create": (_i, [ctypes.POINTER(_vp), _i]),
+    "pq_ens_add": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp, _l, _l, _vp, _l, _l, ctypes.POINTER(_i)]),
+    "pq_ens_add_model": (_i, [_vp, _vp, _dp, _dp, _dp, ctypes.POINTER(_i)]),
+    "pq_ens_finalize": (_i, [_vp]),
+    "pq_ens_class_count": (_i, [_vp]),
+    "pq_ens_class_handle": (_vp, [_vp, _i]),
+    "pq_ens_member_class": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "pq_ens_stream": (_vp, [_vp]),
+    "pq_ens_invert": (_i, [_vp]),
+    "pq_ens_step": (_i, [_vp, _i]),
+    "pq_ens_capture": (_i, [_vp, _i]),
+    "pq_ens_synchronize": (_i, [_vp]),
+    "pq_ens_status": (_i, [_vp]),
+    "pq_ens_destroy": (_i, [_vp]),
 }
 
 

@@ -23,6 +23,7 @@ SOURCES = [
     "pq_runtime.cpp",
     "pq_inversion.hip",
     "pq_cabaret.hip",
+    "pq_ensemble.hip",
 ]
 
 # Transform lengths N = g - 1 with a kernel unit each (pq_inv_kernels.hip

@@ -540,8 +540,7 @@ int pq_model_destroy(pq_model *m) {
     (void)hipFree(p);
   (void)hipFree(m->maxu);
   (void)hipHostFree(m->maxu_host);
-  pq_inv_destroy(m->inv);
-  if (m->stream) (void)hipStreamDestroy(m->stream);
+  pq_inv_destroy(m->inv);  // also destroys m->stream, which is the handle's stream
   delete m;
   return PQ_OK;
 }
@@ -666,52 +665,92 @@ static int model_adapt_dt(pq_model *m) {
   return PQ_OK;
 }
 
-int pq_model_step(pq_model *m, int nsteps, int final_partial) {
-  PQ_CHECK_ARG(m != nullptr && nsteps >= 0, "pq_model_step: bad arguments");
+// First half of a step: adapt_dt (host round trip when enabled), predictor
+// and beta term (qg3.F90:782-790).  The PV inversion follows.
+static int model_pre(pq_model *m, hipStream_t s) {
+  if (m->adapt) {
+    int rc = model_adapt_dt(m);
+    if (rc) return rc;
+  }
+  const int g = m->g, L = m->L;
+  hipLaunchKernelGGL(k_predictor_beta, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d, m->dt,
+                     m->tw_a, m->beta, m->zeta_old, m->res, m->ui, m->zeta, m->res_beta);
+  return PQ_OK;
+}
+
+// Second half, after psi is up to date: relative vorticity and boundary
+// values, forcing, cell boundaries, extrapolator, fluxes and (unless this is
+// a final partial step) the corrector with zeta_old <- zeta (qg3.F90:790-960).
+static int model_post(pq_model *m, hipStream_t s, bool last) {
   const int g = m->g, L = m->L;
   const Dims &d = m->d;
-  hipStream_t s = m->stream;
-  for (int it = 0; it < nsteps; ++it) {
-    if (m->adapt) {
-      int rc = model_adapt_dt(m);
-      if (rc) return rc;
-    }
-    const double dt = m->dt;
-    hipLaunchKernelGGL(k_predictor_beta, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->tw_a,
-                       m->beta, m->zeta_old, m->res, m->ui, m->zeta, m->res_beta);
-    int rc = model_invert(m);
-    if (rc) return rc;
-    if (L == 1)
-      PQ_HIP(hipMemcpyAsync(m->rel, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->bc_type,
-                       m->alpha, m->zeta, m->psi, m->rel);
-    hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,
-                       m->psi, m->zeta);
+  const double dt = m->dt;
+  if (L == 1)
+    PQ_HIP(hipMemcpyAsync(m->rel, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->bc_type,
+                     m->alpha, m->zeta, m->psi, m->rel);
+  hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,
+                     m->psi, m->zeta);
+  m->model_time += 0.5 * dt;
+  hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw, m->visc,
+                     m->visc_bot, m->rel, m->zeta);
+  hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->tw_u,
+                     m->psi, m->ui, m->uj, m->uimo, m->ujmo);
+  hipLaunchKernelGGL(k_extrapolator, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, dt, m->zeta,
+                     m->zeta_old, m->ui, m->uj, m->zfi, m->zfj, m->fin, m->fjn);
+  hipLaunchKernelGGL(k_face_copy, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->fin, m->fjn,
+                     m->zfi, m->zfj);
+  hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, m->ui, m->uj, m->zfi,
+                     m->zfj, m->flux_i, m->flux_j);
+  if (!last) {
+    // corrector into zeta_old, so that zeta_old <- zeta needs no copy; the
+    // next predictor rewrites the interior of zeta and keeps its boundary.
+    hipLaunchKernelGGL(k_corrector_to, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
+                       m->flux_i, m->flux_j, m->res, m->zeta, m->zeta_old);
+    PQ_HIP(hipMemcpyAsync(m->zeta, m->zeta_old, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
     m->model_time += 0.5 * dt;
-    hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw, m->visc,
-                       m->visc_bot, m->rel, m->zeta);
-    hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->tw_u,
-                       m->psi, m->ui, m->uj, m->uimo, m->ujmo);
-    hipLaunchKernelGGL(k_extrapolator, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, dt, m->zeta,
-                       m->zeta_old, m->ui, m->uj, m->zfi, m->zfj, m->fin, m->fjn);
-    hipLaunchKernelGGL(k_face_copy, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->fin, m->fjn,
-                       m->zfi, m->zfj);
-    hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, m->ui, m->uj, m->zfi,
-                       m->zfj, m->flux_i, m->flux_j);
-    const bool last = (it == nsteps - 1) && final_partial;
-    if (!last) {
-      // corrector into zeta_old, so that zeta_old <- zeta needs no copy; the
-      // next predictor rewrites the interior of zeta and keeps its boundary.
-      hipLaunchKernelGGL(k_corrector_to, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
-                         m->flux_i, m->flux_j, m->res, m->zeta, m->zeta_old);
-      PQ_HIP(hipMemcpyAsync(m->zeta, m->zeta_old, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
-      m->model_time += 0.5 * dt;
-    }
-    rc = check_launch("pq_model_step");
+  }
+  return check_launch("pq_model_step");
+}
+
+int pq_model_step(pq_model *m, int nsteps, int final_partial) {
+  PQ_CHECK_ARG(m != nullptr && nsteps >= 0, "pq_model_step: bad arguments");
+  for (int it = 0; it < nsteps; ++it) {
+    int rc = model_pre(m, m->stream);
+    if (rc) return rc;
+    rc = model_invert(m);
+    if (rc) return rc;
+    rc = model_post(m, m->stream, (it == nsteps - 1) && final_partial);
     if (rc) return rc;
   }
   return PQ_OK;
 }
+
+}  // extern "C"
+
+// ---- hooks for the ensemble runtime (pq_ensemble.hip) --------------------
+namespace pq {
+int model_step_pre(pq_model *m, hipStream_t s) { return model_pre(m, s); }
+int model_step_post(pq_model *m, hipStream_t s, bool last) { return model_post(m, s, last); }
+hipStream_t model_stream(pq_model *m) { return m->stream; }
+bool model_adaptive(const pq_model *m) { return m->adapt != 0; }
+void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld) {
+  *g = m->g;
+  *L = m->L;
+  *ld = m->g + 2;
+  *zeta_sdata = m->zeta + 1 + (m->g + 2);
+  *psi_sdata = m->psi + 1 + (m->g + 2);
+}
+double model_time(const pq_model *m) { return m->model_time; }
+void model_set_time(pq_model *m, double t) { m->model_time = t; }
+// the same two half-step additions as model_post, so times round alike
+void model_account_step(pq_model *m) {
+  m->model_time += 0.5 * m->dt;
+  m->model_time += 0.5 * m->dt;
+}
+}  // namespace pq
+
+extern "C" {
 
 int pq_model_get(pq_model *m, int which, double *host, size_t count) {
   PQ_CHECK_ARG(m && host, "pq_model_get: null argument");

@@ -47,7 +47,8 @@ struct InvArgs {
   double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m
   int kb_stride;
   int *status;       // sticky per member: 1 NaN psiC, 2 |psiC| > 1e100
-  const double *coef;  // coefficient block shared by the members (CoefLayout)
+  const double *coef;  // coefficient blocks (CoefLayout): member b's at coef + b * coef_stride
+  long long coef_stride;
   const void *tw64;  // cx<double>[M]: exp(-2 pi i n / M), M the engine's FFT length
   const void *was;   // T[N/2]: 2 sin(k pi / N) (DstSint)
   const void *lam4;  // T[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))

@@ -245,13 +245,14 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   T *cM = cD + 64;
   T *cS = cM + 64;
   const CoefLayout CL{L, N};
+  const double *coef = a.coef + b * a.coef_stride;
   // coefficients: loads issued now, stored to LDS after the forward transform
   double cd0 = 0.0, cm0 = 0.0, cs0 = 0.0;
   if ((int)threadIdx.x < L * L) {
-    cd0 = a.coef[CL.D() + threadIdx.x];
-    cm0 = a.coef[CL.M() + threadIdx.x];
+    cd0 = coef[CL.D() + threadIdx.x];
+    cm0 = coef[CL.M() + threadIdx.x];
   }
-  if ((int)threadIdx.x < L) cs0 = a.coef[CL.ssN() + threadIdx.x];
+  if ((int)threadIdx.x < L) cs0 = coef[CL.ssN() + threadIdx.x];
 
   // Element j of row (l, p) sits at l N^2 + (j / R) N R + p R + j % R.
   Ts *scr = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride);
@@ -275,10 +276,10 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   if ((int)threadIdx.x < L) cS[threadIdx.x] = T(cs0);
   // small workgroups (fewer threads than coefficients)
   for (int i = threadIdx.x + nt; i < L * L; i += nt) {
-    cD[i] = T(a.coef[CL.D() + i]);
-    cM[i] = T(a.coef[CL.M() + i]);
+    cD[i] = T(coef[CL.D() + i]);
+    cM[i] = T(coef[CL.M() + i]);
   }
-  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = T(a.coef[CL.ssN() + i]);
+  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = T(coef[CL.ssN() + i]);
   __syncthreads();
 
   // Spectral step over the points (p0 + c, q).
@@ -382,7 +383,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
 
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
   if (a.mode == 0 && L > 1) {
-    const double *tab = a.coef + CL.tab();
+    const double *tab = a.coef + b * a.coef_stride + CL.tab();
     for (int h = threadIdx.x; h < N / 2; h += nt) {
       const int p = 2 * h + 1;
       double acc[16];

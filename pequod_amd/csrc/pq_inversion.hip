@@ -93,7 +93,8 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, int nblk, int N) {
   }
   __syncthreads();
   const CoefLayout CL{L, N};
-  const double *M = a.coef + CL.M(), *ca = a.coef + CL.a(), *cb = a.coef + CL.b();
+  const double *coef = a.coef + b * a.coef_stride;
+  const double *M = coef + CL.M(), *ca = coef + CL.a(), *cb = coef + CL.b();
   double *kb = a.kb + (size_t)b * a.kb_stride;
   if (tid < L) {
     const int l = tid;
@@ -123,7 +124,7 @@ struct pq_inv {
   void *tw64 = nullptr, *was = nullptr, *lam4 = nullptr;
   double *shat = nullptr;
   std::vector<double> lam_h, shat_h;  // lam_h[p] = -4 sin^2(p pi / 2N)
-  std::vector<double> coef_h;
+  size_t coef_size = 0;  // doubles per member
   double *coef = nullptr;
   void *scr = nullptr;
   double *part = nullptr, *psic = nullptr, *kb = nullptr;
@@ -172,11 +173,12 @@ template <typename T> static int upload_tables(pq_inv *h, int M) {
   return PQ_OK;
 }
 
-static void compute_coefs(pq_inv *h, const double *ss, const double *m2z, const double *z2m) {
+// Coefficient block of one member (CoefLayout) into c[0 .. CL.size()).
+static void compute_coefs(const pq_inv *h, const double *ss, const double *m2z, const double *z2m,
+                          double *c) {
   const int L = h->L, N = h->N;
   CoefLayout CL{L, N};
-  h->coef_h.assign(CL.size(), 0.0);
-  double *c = h->coef_h.data();
+  std::fill(c, c + CL.size(), 0.0);
   const double fourN2 = 4.0 * N * (double)N;
   for (int i = 0; i < L * L; ++i) {
     c[CL.D() + i] = z2m ? z2m[i] : (i % (L + 1) == 0 ? 1.0 : 0.0);
@@ -290,12 +292,16 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   };
   int rc = h->ei.elem == 8 ? upload_tables<double>(h, h->ei.M) : upload_tables<float>(h, h->ei.M);
   if (rc) return fail(rc);
-  compute_coefs(h, ss, m2z, z2m);
+  h->coef_size = CoefLayout{L, N}.size();
+  std::vector<double> coef_h(h->coef_size);
+  compute_coefs(h, ss, m2z, z2m, coef_h.data());
   h->scr_member = (size_t)L * N * N * h->ei.selem;
-  if (hipMalloc(&h->coef, h->coef_h.size() * sizeof(double)) != hipSuccess) return fail(PQ_ERR_NOMEM);
-  if (hipMemcpy(h->coef, h->coef_h.data(), h->coef_h.size() * sizeof(double), hipMemcpyHostToDevice) !=
-      hipSuccess)
-    return fail(PQ_ERR_HIP);
+  // every member slot starts with the creation stratification
+  if (hipMalloc(&h->coef, h->coef_size * sizeof(double) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
+  for (int b = 0; b < batch_max; ++b)
+    if (hipMemcpy(h->coef + h->coef_size * b, coef_h.data(), h->coef_size * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return fail(PQ_ERR_HIP);
   if (hipMalloc(&h->scr, h->scr_member * batch_max) != hipSuccess) {
     pq_set_error("pq_inv_create: cannot allocate %zu bytes of scratch", h->scr_member * batch_max);
     return fail(PQ_ERR_NOMEM);
@@ -326,7 +332,8 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.status = h->status + mem0;
   a.kb = h->kb + (size_t)(L + L * L) * mem0;
   a.kb_stride = L + L * L;
-  a.coef = h->coef;
+  a.coef = h->coef + h->coef_size * mem0;
+  a.coef_stride = (long long)h->coef_size;
   a.tw64 = h->tw64;
   a.was = h->was;
   a.lam4 = h->lam4;
@@ -440,6 +447,28 @@ int pq_inv_get_tiles(pq_inv *h, int *rows, int *cols) {
   PQ_CHECK_ARG(h && rows && cols, "pq_inv_get_tiles: null argument");
   *rows = h->R;
   *cols = h->C;
+  return PQ_OK;
+}
+
+int pq_inv_set_member(pq_inv *h, int slot, const double *ss, const double *modes_to_z,
+                      const double *z_to_modes) {
+  PQ_CHECK_ARG(h != nullptr, "pq_inv_set_member: null handle");
+  PQ_CHECK_ARG(h->mode == 0, "pq_inv_set_member: only for PV inversion handles");
+  PQ_CHECK_ARG(slot >= 0 && slot < h->batch_max, "pq_inv_set_member: slot %d outside 0..%d", slot,
+               h->batch_max - 1);
+  PQ_CHECK_ARG(ss && modes_to_z && z_to_modes, "pq_inv_set_member: null stratification");
+  const int L = h->L;
+  PQ_CHECK_ARG(ss[0] == 0.0, "pq_inv_set_member: ss(1) must be 0 (barotropic mode)");
+  for (int l = 0; l < L; ++l) {
+    PQ_CHECK_ARG(modes_to_z[l] == 1.0, "pq_inv_set_member: modes_to_z(:,1) must be all ones");
+    PQ_CHECK_ARG(ss[l] >= 0.0 && std::isfinite(ss[l]), "pq_inv_set_member: ss must be finite and >= 0");
+  }
+  std::vector<double> c(h->coef_size);
+  compute_coefs(h, ss, modes_to_z, z_to_modes, c.data());
+  // ordered after the handle's earlier work on its stream
+  PQ_HIP(hipMemcpyAsync(h->coef + h->coef_size * slot, c.data(), h->coef_size * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
+  PQ_HIP(hipStreamSynchronize(h->stream));
   return PQ_OK;
 }
 
