@@ -34,6 +34,64 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def ensemble_rates(torch, dev, precision, log, members=64, reps=20):
+    """Inversions/s of the 64-member mixed ensemble: the batched, graph-
+    replayed ensemble in `precision` and the FP64 drop-in (one handle and
+    one call per member), with the max relative psi error of the batched
+    run against the drop-in over all members."""
+    from pequod_amd import qg, synth
+    from pequod_amd._lib import IO_BYTES
+    from pequod_amd.ensemble import Ensemble
+
+    cfgs = [synth.ensemble_member(k) for k in range(members)]
+    dt = torch.float32 if IO_BYTES[precision] == 4 else torch.float64
+    fields = []
+    for c in cfgs:
+        q = c.q(member=c.extra["member"])
+        fields.append(torch.from_numpy(np.ascontiguousarray(q.transpose(2, 1, 0))).to(dev))
+    log(f"[bench] ensemble: {members} members, "
+        f"{len(set((c.n, c.layers) for c in cfgs))} size classes")
+    ens = Ensemble(precision)
+    zs = [f.to(dt) for f in fields]
+    ps = [torch.zeros_like(z) for z in zs]
+    for c, z, p in zip(cfgs, zs, ps):
+        ens.add(c.stratification(), z, p)
+    ens.finalize()
+    ens.capture(Ensemble.INVERT)
+    ens.invert()
+    ens.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ens.invert()
+    ens.synchronize()
+    t_ens = (time.perf_counter() - t0) / reps
+    assert ens.status() == 0
+    # plain FP64 drop-in, member by member
+    hcs = [qg.HelmholtzCache(c.g, c.stratification(), "fp64") for c in cfgs]
+    outs = [torch.zeros_like(f) for f in fields]
+    for h, f, o in zip(hcs, fields, outs):
+        h.run_dev([f], [o])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for h, f, o in zip(hcs, fields, outs):
+            h.run_dev([f], [o])
+    torch.cuda.synchronize()
+    t_64 = (time.perf_counter() - t0) / reps
+    err = 0.0
+    for p, o in zip(ps, outs):
+        pd, od = p.double(), o
+        for l in range(o.shape[0]):
+            err = max(err, float((pd[l] - od[l]).abs().max() / od[l].abs().max()))
+    chosen = sorted({ens.class_precision(k) for k in range(ens.class_count)})
+    ens.close()
+    return {"members": members, "precision": precision, "class_precisions": chosen,
+            "batched_graph_inversions_per_s": round(members / t_ens, 1),
+            "fp64_dropin_inversions_per_s": round(members / t_64, 1),
+            "speedup_vs_fp64_dropin": round(t_64 / t_ens, 3),
+            "max_rel_psi_err_vs_fp64": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +104,8 @@ def main():
     ap.add_argument("--ring", type=int, default=24, help="distinct input/output buffers")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cabaret", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-member mixed-size ensemble leg")
     ap.add_argument("--no-latency", action="store_true",
                     help="skip the B=1 latency leg (every inversion launch then has the bench's batch)")
     ap.add_argument("--sweep-tiles", action="store_true",
@@ -204,6 +264,13 @@ def main():
         cab = nst / (time.perf_counter() - t0)
         m.close()
 
+    # 64-member mixed ensemble (sizes {255, 511, 767, 1023}^2, 2..8 layers,
+    # per-member stratification): batched per size class and replayed from a
+    # HIP graph, against the plain FP64 drop-in called member by member.
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_rates(torch, dev, "fp32auto" if args.precision == "fp32" else args.precision, log)
+
     # CPU baseline (oracle, FP64, one core) on a bounded sample, and the
     # error of the measured path against it on the same input.
     cpu = None
@@ -252,6 +319,7 @@ def main():
         "fp64_dropin_inversions_per_s": None if fp64_rate is None else round(fp64_rate, 1),
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
+        "ensemble_64_mixed": ens,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic},

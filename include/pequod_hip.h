@@ -52,6 +52,15 @@ extern "C" {
 #define PQ_PREC_FP32 1    /* FP32 storage and FP32 transforms              */
 #define PQ_PREC_MIXED 2   /* FP64 storage, FP32 transforms                 */
 #define PQ_PREC_FP32_STORAGE 3 /* FP32 storage, FP64 transforms             */
+/* FP32 storage; FP32 transforms when the layer <-> mode projection is well
+ * conditioned (pq_prec_fp32_choice), else FP64 transforms.                */
+#define PQ_PREC_FP32_AUTO 4
+
+/* PQ_PREC_FP32 or PQ_PREC_FP32_STORAGE for an L-layer stratification: FP32
+ * transforms when kappa_inf(modes_to_z) = |modes_to_z|_inf |z_to_modes|_inf
+ * <= 8 (the FP32 transform error reaches psi amplified by it).            */
+int pq_prec_fp32_choice(int layers, const double *modes_to_z,
+                        const double *z_to_modes);
 
 const char *pq_last_error(void);
 /* Library/runtime information: device name and CU count into buf.         */
@@ -263,6 +272,8 @@ int pq_ens_finalize(pq_ens *e);
 int pq_ens_class_count(pq_ens *e);
 pq_inv *pq_ens_class_handle(pq_ens *e, int k);  /* e.g. for pq_inv_set_tiles */
 int pq_ens_member_class(pq_ens *e, int member, int *cls, int *slot);
+/* Precision chosen for class k (resolves PQ_PREC_FP32_AUTO), or -1.      */
+int pq_ens_class_precision(pq_ens *e, int k);
 void *pq_ens_stream(pq_ens *e);
 /* All members' inversions (asynchronous on the ensemble's stream).        */
 int pq_ens_invert(pq_ens *e);

@@ -12,9 +12,12 @@ LIB_PATH = os.path.join(HERE, "libpequod_hip.so")
 PQ_OK, PQ_ERR_ARG, PQ_ERR_HIP, PQ_ERR_NAN, PQ_WARN_LARGE, PQ_ERR_NOMEM = range(6)
 # fp64: FP64 storage and transforms; fp32: FP32 storage and transforms;
 # mixed: FP64 storage, FP32 transforms; fp32s: FP32 storage, FP64 transforms
-PREC = {"fp64": 0, "fp32": 1, "mixed": 2, "fp32s": 3}
+# fp32auto: FP32 storage, FP32 transforms when the projection is well
+# conditioned (pq_prec_fp32_choice), else FP64 transforms
+PREC = {"fp64": 0, "fp32": 1, "mixed": 2, "fp32s": 3, "fp32auto": 4}
+PREC_NAME = {v: k for k, v in PREC.items()}
 # element type of the zeta / psi arrays of each precision
-IO_BYTES = {"fp64": 8, "fp32": 4, "mixed": 8, "fp32s": 4}
+IO_BYTES = {"fp64": 8, "fp32": 4, "mixed": 8, "fp32s": 4, "fp32auto": 4}
 
 _i, _d, _vp, _sz, _l = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_long
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -74,6 +77,8 @@ SIGNATURES = {
     "pq_ens_class_count": (_i, [_vp]),
     "pq_ens_class_handle": (_vp, [_vp, _i]),
     "pq_ens_member_class": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "pq_ens_class_precision": (_i, [_vp, _i]),
+    "pq_prec_fp32_choice": (_i, [_i, _dp, _dp]),
     "pq_ens_stream": (_vp, [_vp]),
     "pq_ens_invert": (_i, [_vp]),
     "pq_ens_step": (_i, [_vp, _i]),

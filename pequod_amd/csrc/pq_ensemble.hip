@@ -50,6 +50,7 @@ struct pq_ens {
     int g, L;
     long ldz1, ldz2, ldp1, ldp2;
     pq_inv *inv = nullptr;
+    int prec = 0;
     hipStream_t stream = nullptr;
     std::vector<int> members;
     std::vector<const void *> z;
@@ -119,7 +120,7 @@ extern "C" {
 
 int pq_ens_create(pq_ens **out, int precision) {
   PQ_CHECK_ARG(out != nullptr, "pq_ens_create: null pointer");
-  PQ_CHECK_ARG(precision >= 0 && precision <= 3, "pq_ens_create: bad precision %d", precision);
+  PQ_CHECK_ARG(precision >= 0 && precision <= 4, "pq_ens_create: bad precision %d", precision);
   pq_ens *e = new pq_ens();
   e->prec = precision;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -203,7 +204,16 @@ int pq_ens_finalize(pq_ens *e) {
   }
   for (auto &c : e->cls) {
     const auto &m0 = e->mem[c.members[0]];
-    int rc = pq_inv_create(&c.inv, c.g, c.L, m0.ss.data(), m0.m2z.data(), m0.z2m.data(), e->prec,
+    // FP32_AUTO: FP64 transforms for the class when any member needs them
+    int prec = e->prec;
+    if (prec == PQ_PREC_FP32_AUTO) {
+      prec = PQ_PREC_FP32;
+      for (int q : c.members)
+        if (pq_prec_fp32_choice(c.L, e->mem[q].m2z.data(), e->mem[q].z2m.data()) != PQ_PREC_FP32)
+          prec = PQ_PREC_FP32_STORAGE;
+    }
+    c.prec = prec;
+    int rc = pq_inv_create(&c.inv, c.g, c.L, m0.ss.data(), m0.m2z.data(), m0.z2m.data(), prec,
                            (int)c.members.size());
     if (rc) return rc;
     for (size_t q = 1; q < c.members.size(); ++q) {
@@ -224,6 +234,11 @@ int pq_ens_class_count(pq_ens *e) { return e && e->finalized ? (int)e->cls.size(
 pq_inv *pq_ens_class_handle(pq_ens *e, int k) {
   if (!e || !e->finalized || k < 0 || k >= (int)e->cls.size()) return nullptr;
   return e->cls[k].inv;
+}
+
+int pq_ens_class_precision(pq_ens *e, int k) {
+  if (!e || !e->finalized || k < 0 || k >= (int)e->cls.size()) return -1;
+  return e->cls[k].prec;
 }
 
 int pq_ens_member_class(pq_ens *e, int member, int *cls, int *slot) {

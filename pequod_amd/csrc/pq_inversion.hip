@@ -246,13 +246,33 @@ static void default_tiles(const pq_inv *h, int &R, int &C) {
   while (C > 1 && !tiles_ok(h, R, C, false)) C /= 2;
 }
 
+// FP32-storage precision for a stratification: FP32 transforms when the
+// layer <-> mode projection is well conditioned (kappa_inf of modes_to_z
+// <= 8), else FP64 transforms.  The FP32 transforms' ~2.5 eps max error per
+// transform reaches psi amplified by that condition (1023^2 x 3 layers,
+// kappa_inf 6.8: 8e-7; 2047^2 x 6 layers, kappa_inf 16: 1.2e-6).
+extern "C" int pq_prec_fp32_choice(int L, const double *m2z, const double *z2m) {
+  if (!m2z || !z2m) return PQ_PREC_FP32;
+  auto norm_inf = [L](const double *a) {  // column-major L x L
+    double n = 0.0;
+    for (int i = 0; i < L; ++i) {
+      double r = 0.0;
+      for (int j = 0; j < L; ++j) r += std::fabs(a[i + L * j]);
+      n = std::max(n, r);
+    }
+    return n;
+  };
+  return norm_inf(m2z) * norm_inf(z2m) <= 8.0 ? PQ_PREC_FP32 : PQ_PREC_FP32_STORAGE;
+}
+
 static int inv_create(pq_inv **out, int g, int L, const double *ss, const double *m2z,
                       const double *z2m, int precision, int batch_max, int mode) {
   PQ_CHECK_ARG(out != nullptr, "pq_inv_create: null handle pointer");
   PQ_CHECK_ARG(g >= 9 && pq_supported_n(g - 2),
                "pq_inv_create: unsupported grid size g=%d (need g-1 = 2^a or 3*2^a, 8 <= g-1 <= 4096)", g);
   PQ_CHECK_ARG(L >= 1 && L <= 8, "pq_inv_create: layers must be in 1..8 (got %d)", L);
-  PQ_CHECK_ARG(precision >= 0 && precision <= 3, "pq_inv_create: bad precision %d", precision);
+  PQ_CHECK_ARG(precision >= 0 && precision <= 4, "pq_inv_create: bad precision %d", precision);
+  if (precision == PQ_PREC_FP32_AUTO) precision = pq_prec_fp32_choice(L, m2z, z2m);
   PQ_CHECK_ARG(batch_max >= 1, "pq_inv_create: batch_max must be >= 1");
   PQ_CHECK_ARG(ss != nullptr, "pq_inv_create: ss is null");
   if (mode == 0) {

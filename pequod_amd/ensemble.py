@@ -79,13 +79,17 @@ class Ensemble:
             for k in range(self.class_count):
                 h = load().pq_ens_class_handle(self._h, k)
                 g, L, n = self.class_shape(k)
-                t = autotune.lookup(g, L, self.precision, n)
+                t = autotune.lookup(g, L, self.class_precision(k), n)
                 if t:
                     load().pq_inv_set_tiles(h, *t)
 
     @property
     def class_count(self) -> int:
         return load().pq_ens_class_count(self._h)
+
+    def class_precision(self, k: int) -> str:
+        """Precision of size class k (fp32auto resolved per class)."""
+        return _lib.PREC_NAME[load().pq_ens_class_precision(self._h, k)]
 
     def member_class(self, member: int):
         c, s = ctypes.c_int(), ctypes.c_int()

@@ -96,12 +96,14 @@ class HelmholtzCache:
         self.g = g
         self.strat = strat
         self.layers = strat.layers
-        self.precision = precision
         self.batch_max = batch_max
         h = _vp()
         ss = np.ascontiguousarray(strat.ss, dtype=np.float64)
         m2z = np.asfortranarray(strat.modes_to_z, dtype=np.float64)
         z2m = np.asfortranarray(strat.z_to_modes, dtype=np.float64)
+        if precision == "fp32auto":
+            precision = _lib.PREC_NAME[load().pq_prec_fp32_choice(self.layers, _np_ptr(m2z), _np_ptr(z2m))]
+        self.precision = precision
         check(load().pq_inv_create(ctypes.byref(h), g, self.layers, _np_ptr(ss), _np_ptr(m2z),
                                    _np_ptr(z2m), PREC[precision], batch_max))
         self._h = h

@@ -59,3 +59,23 @@ def test_supported_sizes_rejected_without_device():
     rc = L.pq_helm_create(ctypes.byref(h), 100, 100, 1, ss, 0)
     assert rc == _lib.PQ_ERR_ARG
     assert b"unsupported" in L.pq_last_error()
+
+
+def test_fp32_precision_choice_follows_projection_condition():
+    """FP32 transforms for the 3-layer stratification (kappa_inf 6.8), FP64
+    transforms for the 6-layer one (kappa_inf 16) and for 8 layers."""
+    import ctypes
+    import numpy as np
+    from pequod_amd import _lib, synth
+    L = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def choice(st):
+        m2z = np.asfortranarray(st.modes_to_z)
+        z2m = np.asfortranarray(st.z_to_modes)
+        return L.pq_prec_fp32_choice(st.layers, m2z.ctypes.data_as(dp), z2m.ctypes.data_as(dp))
+
+    assert choice(synth.config("qg3_1023").stratification()) == _lib.PREC["fp32"]
+    assert choice(synth.config("qg6_2047").stratification()) == _lib.PREC["fp32s"]
+    eight = [m for m in (synth.ensemble_member(k) for k in range(16)) if m.layers == 8][0]
+    assert choice(eight.stratification()) == _lib.PREC["fp32s"]

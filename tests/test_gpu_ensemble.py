@@ -107,3 +107,39 @@ def test_ensemble_model_step_matches_standalone(dev, graph):
         for name in ("zeta", "psi", "zeta_fi", "ui"):
             np.testing.assert_array_equal(m.get(name), r.get(name), err_msg=name)
         assert m.scalars()["model_time"] == r.scalars()["model_time"]
+
+
+def test_ensemble_fp32auto_within_1e6(dev):
+    """Mixed ensemble members with 2..8 layers (the ensemble workload's
+    stratifications, on small grids): the FP32-storage ensemble with the
+    per-class transform choice stays within 1e-6 max relative psi error of
+    the FP64 oracle for every member; well-conditioned classes use FP32
+    transforms, the others FP64 transforms."""
+    import torch
+    from oracle import oracle as O
+    from helpers import from_dev, rel_err
+    cfgs = []
+    for k in range(40):
+        c = synth.ensemble_member(k)
+        if c.layers not in [x.layers for x in cfgs]:
+            c.n = 127 if k % 2 else 191
+            cfgs.append(c)
+    assert sorted(c.layers for c in cfgs) == list(range(2, 9))
+    ens = Ensemble("fp32auto")
+    zs, ps, refs = [], [], []
+    for c in cfgs:
+        st = c.stratification()
+        q = c.q(member=c.extra["member"])
+        refs.append(O.Inversion(c.g, st.ss, st.modes_to_z, st.z_to_modes).run(q)[0])
+        zs.append(to_dev(q, "cuda", torch.float32))
+        ps.append(torch.zeros_like(zs[-1]))
+        ens.add(st, zs[-1], ps[-1])
+    ens.finalize()
+    precs = {ens.class_precision(k) for k in range(ens.class_count)}
+    assert precs == {"fp32", "fp32s"}, precs
+    ens.invert()
+    assert ens.status() == 0
+    for c, p, ref in zip(cfgs, ps, refs):
+        err = rel_err(from_dev(p), ref)
+        print(f"layers {c.layers} g {c.g}: {err:.2e}", flush=True)
+        assert err < 1e-6, (c.layers, err)
