@@ -98,9 +98,13 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32s", "mixed", "fp64"],
-                    help="fp32 (default): FP32 storage and transforms; fp32s: FP32 storage, FP64 "
-                         "transforms; mixed: FP64 storage, FP32 transforms; fp64: plain drop-in")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32w", "fp32s", "mixed", "fp64"],
+                    help="fp32 (default): FP32 storage and transforms; fp32w: FP32 storage, FP64 "
+                         "arithmetic; fp32s: FP32 storage, FP64 transforms; mixed: FP64 storage, "
+                         "FP32 transforms; fp64: plain drop-in")
+    ap.add_argument("--ensemble-precision", default="fp32auto",
+                    choices=["fp32auto", "fp32", "fp32w", "fp32s", "mixed", "fp64"],
+                    help="precision of the batched 64-member ensemble leg")
     ap.add_argument("--ring", type=int, default=24, help="distinct input/output buffers")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cabaret", action="store_true")
@@ -183,7 +187,7 @@ def main():
         return hc, zs, ps, (t1 - t0), base
 
     if args.sweep_tiles:
-        for prec in ("fp64", "fp32", "fp32s"):
+        for prec in ("fp64", "fp32", "fp32w", "fp32s"):
             for bt in (1, B):
                 dt = io_dtype(prec)
                 hc = qg.HelmholtzCache(g, st, prec, batch_max=bt, tiles=None)
@@ -269,7 +273,7 @@ def main():
     # HIP graph, against the plain FP64 drop-in called member by member.
     ens = None
     if not args.no_ensemble:
-        ens = ensemble_rates(torch, dev, "fp32auto" if args.precision == "fp32" else args.precision, log)
+        ens = ensemble_rates(torch, dev, args.ensemble_precision, log)
 
     # CPU baseline (oracle, FP64, one core) on a bounded sample, and the
     # error of the measured path against it on the same input.
@@ -308,6 +312,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp32s": "f32 storage, f64 transforms",
+                  "fp32w": "f32 storage, f64 arithmetic",
                   "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
         "config": {"workload": "3-layer QG, 1023^2 interior (G=1025), dx 5 km, PV inversion",

@@ -55,6 +55,9 @@ extern "C" {
 /* FP32 storage; FP32 transforms when the layer <-> mode projection is well
  * conditioned (pq_prec_fp32_choice), else FP64 transforms.                */
 #define PQ_PREC_FP32_AUTO 4
+/* FP32 storage (fields, scratch and on-chip buffers), FP64 arithmetic: the
+ * transforms round to FP32 only between FFT stages.                        */
+#define PQ_PREC_FP32_WIDE 5
 
 /* PQ_PREC_FP32 or PQ_PREC_FP32_STORAGE for an L-layer stratification: FP32
  * transforms when kappa_inf(modes_to_z) = |modes_to_z|_inf |z_to_modes|_inf

@@ -66,6 +66,10 @@ template <typename T> __device__ __forceinline__ cx<T> mul_mi(cx<T> a) {
 template <typename T> __device__ __forceinline__ cx<T> scale(cx<T> a, T s) {
   return {a.x * s, a.y * s};
 }
+// Component-wise conversion (rounds to nearest when narrowing).
+template <typename D, typename S> __device__ __forceinline__ cx<D> cx_cast(cx<S> a) {
+  return {D(a.x), D(a.y)};
+}
 
 // LDS padding: one pad element after every 16, so that a thread-blocked
 // access (lane t touching elements 16t..16t+15) spreads over the banks.

@@ -120,7 +120,8 @@ extern "C" {
 
 int pq_ens_create(pq_ens **out, int precision) {
   PQ_CHECK_ARG(out != nullptr, "pq_ens_create: null pointer");
-  PQ_CHECK_ARG(precision >= 0 && precision <= 4, "pq_ens_create: bad precision %d", precision);
+  PQ_CHECK_ARG(precision >= 0 && precision <= PQ_PREC_FP32_WIDE, "pq_ens_create: bad precision %d",
+               precision);
   pq_ens *e = new pq_ens();
   e->prec = precision;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -204,13 +205,15 @@ int pq_ens_finalize(pq_ens *e) {
   }
   for (auto &c : e->cls) {
     const auto &m0 = e->mem[c.members[0]];
-    // FP32_AUTO: FP64 transforms for the class when any member needs them
+    // FP32_AUTO: the more accurate FP32-storage mode for the class when any
+    // member needs it
     int prec = e->prec;
     if (prec == PQ_PREC_FP32_AUTO) {
       prec = PQ_PREC_FP32;
-      for (int q : c.members)
-        if (pq_prec_fp32_choice(c.L, e->mem[q].m2z.data(), e->mem[q].z2m.data()) != PQ_PREC_FP32)
-          prec = PQ_PREC_FP32_STORAGE;
+      for (int q : c.members) {
+        const int p = pq_prec_fp32_choice(c.L, e->mem[q].m2z.data(), e->mem[q].z2m.data());
+        if (p != PQ_PREC_FP32) prec = p;
+      }
     }
     c.prec = prec;
     int rc = pq_inv_create(&c.inv, c.g, c.L, m0.ss.data(), m0.m2z.data(), m0.z2m.data(), prec,

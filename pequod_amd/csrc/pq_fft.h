@@ -75,8 +75,9 @@ constexpr StageArr make_stages(int M, int E) {
   return a;
 }
 
-template <typename T, int M> struct FftPlan {
-  static constexpr int E = choose_E<T>(M);
+// EF > 0 overrides the elements per thread.
+template <typename T, int M, int EF = 0> struct FftPlan {
+  static constexpr int E = EF > 0 ? EF : choose_E<T>(M);
   static constexpr int TPF = M / E;
   static constexpr StageArr S = make_stages(M, E);
 };
@@ -175,8 +176,8 @@ template <typename T, int R> __device__ __forceinline__ void dftR(cx<T> *v) {
 // The stage multiplies input r by w^r, formed by FP64 recurrence (relative
 // error <= (R - 1) eps_64, then rounded once to T), so the stages issue no
 // memory loads for twiddles.
-template <typename T, int M> struct TwBase {
-  using P = FftPlan<T, M>;
+template <typename T, int M, int EF = 0> struct TwBase {
+  using P = FftPlan<T, M, EF>;
   static constexpr int offset(int S) {
     int c = 0;
     for (int s = 0; s < S; ++s)
@@ -217,8 +218,9 @@ constexpr int padc(int c) { return c + c / 16; }
 // and written to (j - k) R + k + r NS.  Split into LDS loads, compute and
 // store, so that the first stage can take its inputs from registers and the
 // last stage can leave its outputs there.
-template <typename T, int M, int S> struct Stage {
-  using P = FftPlan<T, M>;
+template <typename T, int M, int S, int EF = 0> struct Stage {
+  using P = FftPlan<T, M, EF>;
+  using TB = TwBase<T, M, EF>;
   static constexpr int E = P::E, TPF = P::TPF, R = P::S.rad[S], LNS = P::S.lns[S];
   static constexpr int NB = E / R, NR = M / R, NS = 1 << LNS;
   static constexpr int STEP = NR >> LNS;  // M / (NS R)
@@ -258,18 +260,21 @@ template <typename T, int M, int S> struct Stage {
     else return pad16(out_index(t, b, r));
   }
 
-  __device__ __forceinline__ static void load(const cx<T> *buf, int t, cx<T> (&v)[NB][R]) {
+  // buf holds elements of type B (B = T, or FP32 storage of an FP64
+  // computation: converted on load and rounded on store).
+  template <typename B>
+  __device__ __forceinline__ static void load(const cx<B> *buf, int t, cx<T> (&v)[NB][R]) {
     const int pt = padx(t);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[in_pad(t, pt, b, r)];
+      for (int r = 0; r < R; ++r) v[b][r] = cx_cast<T>(buf[in_pad(t, pt, b, r)]);
   }
-  __device__ __forceinline__ static void compute(const TwBase<T, M> &tb, cx<T> (&v)[NB][R]) {
+  __device__ __forceinline__ static void compute(const TB &tb, cx<T> (&v)[NB][R]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       if constexpr (TW) {
-        cx<double> w1 = tb.w[TwBase<T, M>::offset(S) + b];
+        cx<double> w1 = tb.w[TB::offset(S) + b];
         // opaque copy: keeps the compiler from keeping every power of w
         // alive between two transforms of one kernel (register spills)
         asm volatile("" : "+v"(w1.x), "+v"(w1.y));
@@ -283,18 +288,19 @@ template <typename T, int M, int S> struct Stage {
       dftR<T, R>(v[b]);
     }
   }
-  __device__ __forceinline__ static void store(cx<T> *buf, int t, const cx<T> (&v)[NB][R]) {
+  template <typename B>
+  __device__ __forceinline__ static void store(cx<B> *buf, int t, const cx<T> (&v)[NB][R]) {
     const int ob = out_base(t);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[out_pad(t, ob, b, r)] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[out_pad(t, ob, b, r)] = cx_cast<B>(v[b][r]);
   }
 };
 
-template <typename T, int M, int S>
-__device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const TwBase<T, M> &tb) {
-  using St = Stage<T, M, S>;
+template <typename T, int M, int S, int EF, typename B>
+__device__ __forceinline__ void fft_stage(cx<B> *buf, int t, const TwBase<T, M, EF> &tb) {
+  using St = Stage<T, M, S, EF>;
   cx<T> v[St::NB][St::R];
   St::load(buf, t, v);
   group_sync<St::TPF>();
@@ -304,13 +310,13 @@ __device__ __forceinline__ void fft_stage(cx<T> *buf, int t, const TwBase<T, M> 
 }
 
 // Stages S .. SEND - 1 (SEND = -1: all remaining stages).
-template <typename T, int M, int S, int SEND = -1>
-__device__ __forceinline__ void fft_stages(cx<T> *buf, int t, const TwBase<T, M> &tb) {
-  using P = FftPlan<T, M>;
+template <typename T, int M, int S, int SEND = -1, int EF = 0, typename B>
+__device__ __forceinline__ void fft_stages(cx<B> *buf, int t, const TwBase<T, M, EF> &tb) {
+  using P = FftPlan<T, M, EF>;
   constexpr int END = SEND < 0 ? P::S.nst : SEND;
   if constexpr (S < END) {
-    fft_stage<T, M, S>(buf, t, tb);
-    fft_stages<T, M, S + 1, SEND>(buf, t, tb);
+    fft_stage<T, M, S, EF>(buf, t, tb);
+    fft_stages<T, M, S + 1, SEND, EF>(buf, t, tb);
   }
 }
 
@@ -365,12 +371,28 @@ __device__ __forceinline__ void group_excl_scan2(T &a, T &b, int t, T *tmp) {
 // aliases the region: all loads happen before the first store.  Only
 // group-level synchronisation happens inside; callers synchronise the
 // workgroup around cross-group LDS traffic.
+//
+// Outputs go to a sink st(k, kp, xa, xb), called once for every
+// 0 <= k <= N - 1 with X_k of both rows in the compute type Tc (kp =
+// pad16(k)); the default sink stores them to the region's rows.  A sink
+// that writes elsewhere (global memory) leaves the region's contents
+// undefined.
+
+// Sink that stores the outputs to the padded rows a and b of a region.
+template <typename T> struct RowSink {
+  T *rowa, *rowb;
+  template <typename Tc> __device__ __forceinline__ void operator()(int, int kp, Tc xa, Tc xb) const {
+    rowa[kp] = T(xa);
+    rowb[kp] = T(xb);
+  }
+};
 
 // Row stride (elements of T) of a padded row of N slots.
 template <int N> constexpr int row_stride() { return N + N / 16 + 1; }
 
 template <typename T_, int N_> struct DstSint {
   using T = T_;
+  using Tc = T_;
   static constexpr int N = N_;
   static constexpr int M = N;
   using P = FftPlan<T, M>;
@@ -379,6 +401,8 @@ template <typename T_, int N_> struct DstSint {
   static constexpr int RS = row_stride<N>();
   static constexpr int Region = 2 * RS;
   static constexpr int ScanTmp = TPF > 64 ? 2 * (TPF / 64) : 0;
+  // outputs are thread-blocked (thread t holds k = 2 t H .. 2 t H + 2 H - 1)
+  static constexpr bool kLaneOut = false;
 
   static constexpr int H = E / 2;  // TPF * H = N / 2
 
@@ -416,18 +440,24 @@ template <typename T_, int N_> struct DstSint {
     }
   }
 
+  template <typename Loader, typename Sink>
+  __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                             Loader ld, Sink st) {
+    In in;
+    load_in(t, ld, in);
+    run_in(region, t, scan_tmp, cx_, in, st);
+  }
   template <typename Loader>
   __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
                                              Loader ld) {
-    In in;
-    load_in(t, ld, in);
-    run_in(region, t, scan_tmp, cx_, in);
+    run(region, t, scan_tmp, cx_, ld, RowSink<T>{region, region + RS});
   }
 
   // The transform of loaded inputs; the group's region may still be read by
   // the loader until this call (it synchronises the group first).
+  template <typename Sink>
   __device__ __forceinline__ static void run_in(T *region, int t, T *scan_tmp, const Ctx &cx_,
-                                                const In &in) {
+                                                const In &in, Sink st) {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
     cx<T> ylo[H], yhi[H];
 #pragma unroll
@@ -479,35 +509,43 @@ template <typename T_, int N_> struct DstSint {
     }
     group_excl_scan2<T, TPF>(sa, sb, t, scan_tmp);
     group_sync<TPF>();
-    T *rowa = region, *rowb = region + RS;
 #pragma unroll
     for (int e = 0; e < H; ++e) {
       const int m = t * H + e;
-      rowa[pad16(2 * m + 1)] = ra[e] + sa;
-      rowb[pad16(2 * m + 1)] = rb[e] + sb;
-      rowa[pad16(2 * m)] = m == 0 ? T(0) : -ia[e];
-      rowb[pad16(2 * m)] = m == 0 ? T(0) : -ib[e];
+      st(2 * m + 1, pad16(2 * m + 1), ra[e] + sa, rb[e] + sb);
+      st(2 * m, pad16(2 * m), m == 0 ? T(0) : -ia[e], m == 0 ? T(0) : -ib[e]);
     }
   }
 };
 
-template <typename T_, int N_> struct DstExt {
+// T: the rows' and the LDS buffer's type; Tc: the arithmetic type.  With
+// T = float and Tc = double every butterfly, twiddle and output is computed
+// in FP64 and only the stage results are rounded to FP32 in LDS (three
+// roundings per transform instead of one per operation).
+template <typename T_, int N_, typename Tc_ = T_> struct DstExt {
   using T = T_;
+  using Tc = Tc_;
   static constexpr int N = N_;
   static constexpr int M = 2 * N;
-  using P = FftPlan<T, M>;
+  // FP64 arithmetic at M >= 4096: 32 elements per thread (not 16), so that
+  // one group has at most 128 threads and a column workgroup of 8 layers
+  // fits in 512
+  static constexpr int EF = (sizeof(Tc) > sizeof(T) && M >= 4096) ? 32 : 0;
+  using P = FftPlan<Tc, M, EF>;
   static constexpr int TPF = P::TPF;
   static constexpr int E = P::E;
   static constexpr int RS = row_stride<N>();
   static constexpr int Region = 2 * (M + M / 16 + 1);
   static constexpr int ScanTmp = 0;
+  // the sink is called with consecutive k on consecutive lanes
+  static constexpr bool kLaneOut = true;
 
   static constexpr int H = E / 2;  // TPF * H = N
 
   struct Ctx {
-    TwBase<T, M> tw;
+    TwBase<Tc, M, EF> tw;
     __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64,
-                                         const T *__restrict__) {
+                                         const Tc *__restrict__) {
       tw.load(t, tw64);
     }
   };
@@ -519,12 +557,12 @@ template <typename T_, int N_> struct DstExt {
   // trips per transform.
   static_assert(P::S.nst >= 2, "DstExt needs at least two FFT stages");
   static constexpr int LAST = P::S.nst - 1;
-  using S0 = Stage<T, M, 0>;
-  using SL = Stage<T, M, LAST>;
+  using S0 = Stage<Tc, M, 0, EF>;
+  using SL = Stage<Tc, M, LAST, EF>;
 
   // Raw inputs of a thread: the first stage's elements of the extension.
   struct In {
-    cx<T> v[S0::NB][S0::R];
+    cx<Tc> v[S0::NB][S0::R];
   };
 
   template <typename Loader>
@@ -540,43 +578,46 @@ template <typename T_, int N_> struct DstExt {
         const int kp = lo ? S0::in_pad(t, pt, b, r) : pad16(k);
         T xa = T(0), xb = T(0);
         if (k != 0 && k != N) ld(k, kp, xa, xb);
-        in.v[b][r] = lo ? cx<T>{xa, xb} : cx<T>{-xa, -xb};
+        in.v[b][r] = lo ? cx<Tc>{Tc(xa), Tc(xb)} : cx<Tc>{-Tc(xa), -Tc(xb)};
       }
   }
 
+  template <typename Loader, typename Sink>
+  __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                             Loader ld, Sink st) {
+    In in;
+    load_in(t, ld, in);
+    run_in(region, t, scan_tmp, cx_, in, st);
+  }
   template <typename Loader>
   __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
                                              Loader ld) {
-    In in;
-    load_in(t, ld, in);
-    run_in(region, t, scan_tmp, cx_, in);
+    run(region, t, scan_tmp, cx_, ld, RowSink<T>{region, region + RS});
   }
 
   // The transform of loaded inputs; the group's region may still be read by
   // the loader until this call (it synchronises the group first).
-  __device__ __forceinline__ static void run_in(T *region, int t, T *, const Ctx &cx_, In in) {
+  template <typename Sink>
+  __device__ __forceinline__ static void run_in(T *region, int t, T *, const Ctx &cx_, In in,
+                                                Sink st) {
     cx<T> *buf = reinterpret_cast<cx<T> *>(region);
     group_sync<TPF>();
     S0::compute(cx_.tw, in.v);
     S0::store(buf, t, in.v);
     group_sync<TPF>();
-    fft_stages<T, M, 1, LAST>(buf, t, cx_.tw);
-    cx<T> v[SL::NB][SL::R];
+    fft_stages<Tc, M, 1, LAST, EF>(buf, t, cx_.tw);
+    cx<Tc> v[SL::NB][SL::R];
     SL::load(buf, t, v);
     group_sync<TPF>();
     SL::compute(cx_.tw, v);
-    T *rowa = region, *rowb = region + RS;
     const int ob = SL::out_base(t);
 #pragma unroll
     for (int b = 0; b < SL::NB; ++b)
 #pragma unroll
       for (int r = 0; r < SL::R; ++r) {
         const int k = SL::out_index(t, b, r);
-        if (k < N) {
-          const int kp = SL::out_pad(t, ob, b, r);
-          rowa[kp] = k == 0 ? T(0) : -v[b][r].y;
-          rowb[kp] = k == 0 ? T(0) : v[b][r].x;
-        }
+        if (k < N)
+          st(k, SL::out_pad(t, ob, b, r), k == 0 ? Tc(0) : -v[b][r].y, k == 0 ? Tc(0) : v[b][r].x);
       }
   }
 };

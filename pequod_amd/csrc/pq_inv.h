@@ -50,8 +50,8 @@ struct InvArgs {
   const double *coef;  // coefficient blocks (CoefLayout): member b's at coef + b * coef_stride
   long long coef_stride;
   const void *tw64;  // cx<double>[M]: exp(-2 pi i n / M), M the engine's FFT length
-  const void *was;   // T[N/2]: 2 sin(k pi / N) (DstSint)
-  const void *lam4;  // T[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
+  const void *was;   // Tc[N/2]: 2 sin(k pi / N) (DstSint), Tc the arithmetic type
+  const void *lam4;  // Tc[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
   const double *shat;  // double[N]: sum_i 2 sin(pi p i / N) (0 for even p)
   int L, mode;       // mode 0 pv_inversion, 1 helm
   int R, lgR;        // rows per tile of the row passes (scratch block height)
@@ -64,8 +64,9 @@ struct EngineInfo {
   int tpf;           // threads per FFT group (one group = two rows)
   int region;        // LDS elements of T per group
   int scan;          // LDS elements of T of scan scratch per group
-  int elem;          // sizeof(T), the compute type
+  int elem;          // sizeof(T), the LDS (row and FFT buffer) type
   int selem;         // sizeof of the scratch type
+  int celem;         // sizeof of the arithmetic type (tables lam4, was)
 };
 
 // Supported transform lengths N = 2^a (8..4096) and 3 * 2^a (12..3072); one
@@ -88,7 +89,7 @@ PQ_INV_SIZES(PQ_INV_DECL)
 inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups) {
   size_t b = (size_t)e.elem * groups * (e.region + e.scan);
   b = (b + 15) & ~size_t(15);
-  if (pass == 1) b += 64 * 8 + (size_t)e.elem * (64 + 64 + 8);
+  if (pass == 1) b += 64 * 8 + (size_t)e.celem * (64 + 64 + 8);
   if (pass == 2) b += 16;
   return b;
 }

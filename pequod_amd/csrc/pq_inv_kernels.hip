@@ -104,6 +104,7 @@ __device__ __forceinline__ int xcd_block(int i, int n) {
 template <typename Eng, typename Tin, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
+  using Tc = typename Eng::Tc;
   using VT = typename V16<Ts>::type;
   constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int ngrp = R >> 1, nt = ngrp * TPF;
   typename Eng::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was));
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
   const Tin *z = static_cast<const Tin *>(ptrs.zeta[b]) + (size_t)l * ptrs.ldz1 * ptrs.ldz2;
@@ -142,7 +143,8 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
 }
 
 // Spectral step of the column pass on the points (p0 + c, q), c < C,
-// 1 <= q < N, rows (l, c) at LDS row l C + c.  pv_inversion (mode 0):
+// 1 <= q < N, rows (l, c) at LDS row l C + c, computed in the engine's
+// arithmetic type Tc (the coefficients cD, cM, cS are in Tc).  pv_inversion (mode 0):
 // phi_m = sum_l D(m,l) x_l / ((2N)^2 (lambda_p + lambda_q - ss_m)), FP64
 // psiC partial acc_m += phi_m shat_p shat_q, x_l = sum_m M(l,m) phi_m.
 // helm (mode 1): x_l /= (2N)^2 (lambda_p + lambda_q - ss_l).  LT is the
@@ -150,15 +152,16 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
 // else 0 (coefficients read from LDS per point).
 template <typename Eng, int LT>
 __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T *lds,
-                                              const typename Eng::T *cD, const typename Eng::T *cM,
-                                              const typename Eng::T *cS, int p0, int nt,
+                                              const typename Eng::Tc *cD, const typename Eng::Tc *cM,
+                                              const typename Eng::Tc *cS, int p0, int nt,
                                               double (&acc)[8]) {
   using T = typename Eng::T;
+  using Tc = typename Eng::Tc;
   constexpr int N = Eng::N;
   const int L = LT > 0 ? LT : a.L, C = a.C, lgC = a.lgC;
-  const T *lam4 = static_cast<const T *>(a.lam4);
+  const Tc *lam4 = static_cast<const Tc *>(a.lam4);
   constexpr int LR = LT > 0 ? LT : 1;
-  T d[LR * LR], mm[LR * LR], sv[LR];
+  Tc d[LR * LR], mm[LR * LR], sv[LR];
   if constexpr (LT > 0) {
 #pragma unroll
     for (int i = 0; i < LT * LT; ++i) {
@@ -171,7 +174,7 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
   for (int c = 0; c < C; ++c) {
     const int p = p0 + c;
     if (p < 1) continue;
-    const T lp = lam4[p];
+    const Tc lp = lam4[p];
     const double sp = a.shat[p];
     for (int q = 1 + threadIdx.x; q < N; q += nt) {
       if constexpr (LT == 0) {
@@ -179,17 +182,17 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
         // registers they would spill for large L)
         asm volatile("" ::: "memory");
       }
-      const T lpq = lp + lam4[q];
+      const Tc lpq = lp + lam4[q];
       if (a.mode == 0) {
         const double wpq = sp * a.shat[q];
-        T col[8], phi[8];
+        Tc col[8], phi[8];
 #pragma unroll
         for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) col[ll] = row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)];
+          if (ll < L) col[ll] = Tc(row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)]);
 #pragma unroll
         for (int m = 0; m < 8; ++m)
           if (m < L) {
-            T s = T(0);
+            Tc s = Tc(0);
 #pragma unroll
             for (int ll = 0; ll < 8; ++ll)
               if (ll < L) s += (LT > 0 ? d[(m + L * ll) % (LR * LR)] : cD[m + L * ll]) * col[ll];
@@ -199,18 +202,18 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
 #pragma unroll
         for (int ll = 0; ll < 8; ++ll)
           if (ll < L) {
-            T s = T(0);
+            Tc s = Tc(0);
 #pragma unroll
             for (int m = 0; m < 8; ++m)
               if (m < L) s += (LT > 0 ? mm[(ll + L * m) % (LR * LR)] : cM[ll + L * m]) * phi[m];
-            row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = s;
+            row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = T(s);
           }
       } else {
 #pragma unroll
         for (int ll = 0; ll < 8; ++ll)
           if (ll < L) {
             T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(q);
-            *x = *x / (lpq - cS[ll]);
+            *x = T(Tc(*x) / (lpq - cS[ll]));
           }
       }
     }
@@ -222,6 +225,7 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
 template <typename Eng, typename Ts>
 __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   using T = typename Eng::T;
+  using Tc = typename Eng::Tc;
   using VT = typename V16<Ts>::type;
   constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -236,14 +240,14 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int nrows = L << lgC;  // row rho = l * C + c
   typename Eng::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was));
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
   const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
   double *red = reinterpret_cast<double *>(smem + red_off);
-  T *cD = reinterpret_cast<T *>(red + 64);  // D, M, ssN in T (L <= 8)
-  T *cM = cD + 64;
-  T *cS = cM + 64;
+  Tc *cD = reinterpret_cast<Tc *>(red + 64);  // D, M, ssN in Tc (L <= 8)
+  Tc *cM = cD + 64;
+  Tc *cS = cM + 64;
   const CoefLayout CL{L, N};
   const double *coef = a.coef + b * a.coef_stride;
   // coefficients: loads issued now, stored to LDS after the forward transform
@@ -270,16 +274,16 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
     xb = vb ? T(sb[off]) : T(0);
   });
   if ((int)threadIdx.x < L * L) {
-    cD[threadIdx.x] = T(cd0);
-    cM[threadIdx.x] = T(cm0);
+    cD[threadIdx.x] = Tc(cd0);
+    cM[threadIdx.x] = Tc(cm0);
   }
-  if ((int)threadIdx.x < L) cS[threadIdx.x] = T(cs0);
+  if ((int)threadIdx.x < L) cS[threadIdx.x] = Tc(cs0);
   // small workgroups (fewer threads than coefficients)
   for (int i = threadIdx.x + nt; i < L * L; i += nt) {
-    cD[i] = T(coef[CL.D() + i]);
-    cM[i] = T(coef[CL.M() + i]);
+    cD[i] = Tc(coef[CL.D() + i]);
+    cM[i] = Tc(coef[CL.M() + i]);
   }
-  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = T(coef[CL.ssN() + i]);
+  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = Tc(coef[CL.ssN() + i]);
   __syncthreads();
 
   // Spectral step over the points (p0 + c, q).
@@ -336,6 +340,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
 template <typename Eng, typename Tout, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
+  using Tc = typename Eng::Tc;
   using VT = typename V16<Ts>::type;
   constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
@@ -369,7 +374,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
       if (m < L) beta[m] = kb[L + l * L + m];
   }
   typename Eng::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const T *>(a.was));
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was));
 
   // Tile block (l, jb): row r at p <- S[p * R + r].
 #pragma unroll
@@ -431,52 +436,56 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
     __syncthreads();
   }
 
-  {
-    const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::run(region, t, scan_tmp, ctx, [&](int, int kp, T &xa, T &xb) {
-      xa = rowa[kp];
-      xb = rowb[kp];
-    });
-  }
-  __syncthreads();
-
+  // Output rows ja = j0 + 2f and ja + 1 of this group (ja + 1 <= N - 1).
+  // pv_inversion (mode 0): psi = X - kappa, and -kappa on the boundary
+  // (X_0 = 0 gives i = 0; row j = 0 is written as X = 0).  helm (mode 1),
+  // the reference's boundary semantics (qg3p_helmholtz.F90:757-761,
+  // 1055-1057): row 0 gets 0 at 1 <= i <= N - 1, rows 1..N-1 get X at
+  // 1 <= i <= N - 1 and 0 at i = N; column i = 0 and row N are not written.
   Tout *ps = static_cast<Tout *>(ptrs.psi[b]) + (size_t)l * ptrs.ldp1 * ptrs.ldp2;
   const long long ld1 = ptrs.ldp1;
   constexpr int G = N + 1;
-  const bool last = (j0 + R == N);
-  const int nrow = R + (last ? 1 : 0);
-  // (r, i) of idx = r G + i, advanced by nt per iteration without a division
-  int r = (int)threadIdx.x / G, i = (int)threadIdx.x - r * G;
-  for (int idx = threadIdx.x; idx < nrow * G; idx += nt) {
-    const int j = j0 + r;
-    Tout *o = ps + (size_t)j * ld1 + i;
-    if (a.mode == 0) {
-      double val;
-      if (j == 0 || j == N || i == 0 || i == N) val = -kappa;
-      else val = double(row_ptr<Eng>(lds, r)[pad16(i)]) - kappa;
-      *o = Tout(val);
+  const int ja = j0 + 2 * f;
+  Tout *pa = ps + (size_t)ja * ld1, *pb = pa + ld1;
+  const bool za = ja == 0;
+  const bool m0 = a.mode == 0;
+  const double sub = m0 ? kappa : 0.0;
+  auto sink = [&](int k, int, Tc xa, Tc xb) {
+    if (m0 || k != 0) {
+      pa[k] = Tout(double(za ? Tc(0) : xa) - sub);
+      pb[k] = Tout(double(xb) - sub);
+    }
+  };
+  {
+    const T *rowa = region, *rowb = region + Eng::RS;
+    auto ld = [&](int, int kp, T &xa, T &xb) {
+      xa = rowa[kp];
+      xb = rowb[kp];
+    };
+    if constexpr (Eng::kLaneOut) {
+      // outputs leave the last FFT stage with consecutive k across lanes
+      Eng::run(region, t, scan_tmp, ctx, ld, sink);
     } else {
-      // helm's boundary semantics (qg3p_helmholtz.F90:757-761, 1055-1057)
-      if (j != N && i != 0) {
-        if (j == 0) {
-          if (i <= N - 1) *o = Tout(0);
-        } else {
-          *o = i == N ? Tout(0) : Tout(row_ptr<Eng>(lds, r)[pad16(i)]);
-        }
-      }
+      Eng::run(region, t, scan_tmp, ctx, ld);
+      group_sync<TPF>();
+      for (int k = t; k < N; k += TPF) sink(k, 0, Tc(rowa[pad16(k)]), Tc(rowb[pad16(k)]));
     }
-    i += nt;
-    while (i >= G) {
-      i -= G;
-      ++r;
-    }
+  }
+  if (t == 0) {
+    const Tout edge = Tout(0.0 - sub);  // +0 for helm
+    if (m0 || !za) pa[N] = edge;
+    pb[N] = edge;
+  }
+  if (m0 && j0 + R == N) {
+    Tout *pn = ps + (size_t)N * ld1;
+    for (int i = threadIdx.x; i < G; i += nt) pn[i] = Tout(-kappa);
   }
 }
 
 // ------------------------------------------------------------ dispatch
 template <typename Eng, typename Ts> static EngineInfo info_of() {
   return EngineInfo{Eng::M, Eng::TPF, Eng::Region, Eng::ScanTmp, (int)sizeof(typename Eng::T),
-                    (int)sizeof(Ts)};
+                    (int)sizeof(Ts), (int)sizeof(typename Eng::Tc)};
 }
 
 // Dynamic LDS above the 64 KiB default needs a per-kernel limit (gfx950 has
@@ -515,10 +524,14 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
 // PQ_PREC_FP32: FP32 fields, scratch and transforms (odd-extension engine).
 // PQ_PREC_MIXED: FP64 fields, FP32 scratch and transforms.
 // PQ_PREC_FP32_STORAGE: FP32 fields and scratch, FP64 transforms.
+// PQ_PREC_FP32_WIDE: FP32 fields, scratch and LDS, FP64 arithmetic
+// (odd-extension engine).
+using ExtWide = DstExt<float, PQ_N, double>;
 EngineInfo PQ_CAT(inv_info_, PQ_N)(int prec) {
   switch (prec) {
     case PQ_PREC_FP64: return info_of<DstSint<double, PQ_N>, double>();
     case PQ_PREC_FP32_STORAGE: return info_of<DstSint<double, PQ_N>, float>();
+    case PQ_PREC_FP32_WIDE: return info_of<ExtWide, float>();
     default: return info_of<DstExt<float, PQ_N>, float>();
   }
 }
@@ -532,6 +545,8 @@ hipError_t PQ_CAT(inv_launch_, PQ_N)(int prec, int pass, const InvArgs &a, const
       return launch_pass<DstExt<float, PQ_N>, float, float>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32_STORAGE:
       return launch_pass<DstSint<double, PQ_N>, float, float>(pass, a, pt, blocks, threads, lds, s);
+    case PQ_PREC_FP32_WIDE:
+      return launch_pass<ExtWide, float, float>(pass, a, pt, blocks, threads, lds, s);
     default:
       return launch_pass<DstExt<float, PQ_N>, double, float>(pass, a, pt, blocks, threads, lds, s);
   }

@@ -271,7 +271,8 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   PQ_CHECK_ARG(g >= 9 && pq_supported_n(g - 2),
                "pq_inv_create: unsupported grid size g=%d (need g-1 = 2^a or 3*2^a, 8 <= g-1 <= 4096)", g);
   PQ_CHECK_ARG(L >= 1 && L <= 8, "pq_inv_create: layers must be in 1..8 (got %d)", L);
-  PQ_CHECK_ARG(precision >= 0 && precision <= 4, "pq_inv_create: bad precision %d", precision);
+  PQ_CHECK_ARG(precision >= 0 && precision <= PQ_PREC_FP32_WIDE, "pq_inv_create: bad precision %d",
+               precision);
   if (precision == PQ_PREC_FP32_AUTO) precision = pq_prec_fp32_choice(L, m2z, z2m);
   PQ_CHECK_ARG(batch_max >= 1, "pq_inv_create: batch_max must be >= 1");
   PQ_CHECK_ARG(ss != nullptr, "pq_inv_create: ss is null");
@@ -310,7 +311,7 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
     pq_inv_destroy(h);
     return code;
   };
-  int rc = h->ei.elem == 8 ? upload_tables<double>(h, h->ei.M) : upload_tables<float>(h, h->ei.M);
+  int rc = h->ei.celem == 8 ? upload_tables<double>(h, h->ei.M) : upload_tables<float>(h, h->ei.M);
   if (rc) return fail(rc);
   h->coef_size = CoefLayout{L, N}.size();
   std::vector<double> coef_h(h->coef_size);
@@ -569,7 +570,7 @@ static int run_host(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *p
                "pq_inv_run: leading dimensions smaller than g=%d", h->g);
   const int g = h->g, L = h->L;
   // FP64 storage for FP64 and mixed handles, FP32 for FP32 handles
-  const size_t el = (h->prec == PQ_PREC_FP32 || h->prec == PQ_PREC_FP32_STORAGE) ? 4 : 8;
+  const size_t el = (h->prec == PQ_PREC_FP64 || h->prec == PQ_PREC_MIXED) ? 8 : 4;
   void *mz = pq_mirror_device_ptr(zeta);
   void *mp = pq_mirror_device_ptr(psi);
   if (mz && mp && el == 8) {
