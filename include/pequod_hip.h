@@ -61,7 +61,7 @@ extern "C" {
 
 /* PQ_PREC_FP32 or PQ_PREC_FP32_STORAGE for an L-layer stratification: FP32
  * transforms when kappa_inf(modes_to_z) = |modes_to_z|_inf |z_to_modes|_inf
- * <= 8 (the FP32 transform error reaches psi amplified by it).            */
+ * <= 32 (the FP32 transform error can reach psi amplified by it).         */
 int pq_prec_fp32_choice(int layers, const double *modes_to_z,
                         const double *z_to_modes);
 

@@ -187,8 +187,10 @@ template <typename T, int M, int EF = 0> struct TwBase {
   static constexpr int NW = offset(P::S.nst) > 0 ? offset(P::S.nst) : 1;
   cx<double> w[NW];
 
+  // tw64 holds exp(-2 pi i n / (M * ts)): the table of a transform ts times
+  // longer (entry n ts of it is entry n of the length-M table).
   template <int S>
-  __device__ __forceinline__ void load_stage(int t, const cx<double> *__restrict__ tw64) {
+  __device__ __forceinline__ void load_stage(int t, const cx<double> *__restrict__ tw64, int ts) {
     if constexpr (S < P::S.nst) {
       constexpr int R = P::S.rad[S], LNS = P::S.lns[S];
       if constexpr (LNS > 0) {
@@ -196,14 +198,14 @@ template <typename T, int M, int EF = 0> struct TwBase {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           const int k = (t + b * P::TPF) & (NS - 1);
-          w[offset(S) + b] = tw64[k * STEP];
+          w[offset(S) + b] = tw64[k * STEP * ts];
         }
       }
-      load_stage<S + 1>(t, tw64);
+      load_stage<S + 1>(t, tw64, ts);
     }
   }
-  __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64) {
-    load_stage<0>(t, tw64);
+  __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64, int ts = 1) {
+    load_stage<0>(t, tw64, ts);
   }
 };
 
@@ -390,12 +392,15 @@ template <typename T> struct RowSink {
 // Row stride (elements of T) of a padded row of N slots.
 template <int N> constexpr int row_stride() { return N + N / 16 + 1; }
 
-template <typename T_, int N_> struct DstSint {
+// EF_ > 0 sets the elements per thread (to pair it with another engine of
+// the same threads per group).
+template <typename T_, int N_, int EF_ = 0> struct DstSint {
   using T = T_;
   using Tc = T_;
   static constexpr int N = N_;
   static constexpr int M = N;
-  using P = FftPlan<T, M>;
+  static constexpr int EF = EF_;
+  using P = FftPlan<T, M, EF>;
   static constexpr int TPF = P::TPF;
   static constexpr int E = P::E;
   static constexpr int RS = row_stride<N>();
@@ -409,11 +414,12 @@ template <typename T_, int N_> struct DstSint {
   // Per-thread constants, loaded once per kernel: base twiddles and the
   // pre-twiddle weights was[k] = 2 sin(k pi / N) of the thread's k.
   struct Ctx {
-    TwBase<T, M> tw;
+    TwBase<T, M, EF> tw;
     T was[H];
+    // tw64: exp(-2 pi i n / (M ts)); was_g: 2 sin(k pi / N), k < N / 2
     __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64,
-                                         const T *__restrict__ was_g) {
-      tw.load(t, tw64);
+                                         const T *__restrict__ was_g, int ts = 1) {
+      tw.load(t, tw64, ts);
 #pragma unroll
       for (int e = 0; e < H; ++e) was[e] = was_g[t + e * TPF];
     }
@@ -483,7 +489,7 @@ template <typename T_, int N_> struct DstSint {
       buf[pad16(k == 0 ? N / 2 : N - k)] = yhi[e];
     }
     group_sync<TPF>();
-    fft_stages<T, M, 0>(buf, t, cx_.tw);
+    fft_stages<T, M, 0, -1, EF>(buf, t, cx_.tw);
     T ra[H], ia[H], rb[H], ib[H];
 #pragma unroll
     for (int e = 0; e < H; ++e) {
@@ -522,11 +528,23 @@ template <typename T_, int N_> struct DstSint {
 // T = float and Tc = double every butterfly, twiddle and output is computed
 // in FP64 and only the stage results are rounded to FP32 in LDS (three
 // roundings per transform instead of one per operation).
-template <typename T_, int N_, typename Tc_ = T_> struct DstExt {
+//
+// LOW_ > 0 splits off the inputs x_1 .. x_LOW: the FFT sees them as zero and
+// their part 2 sum_{j <= LOW} x_j sin(pi j k / N) is added to every output
+// in FP64 (sin by the Chebyshev recurrence from the table's sin and cos of
+// pi k / N); the outputs are then FP64.  For smooth fields (spectra falling
+// off with wavenumber, as in an inverse transform of the solution) the
+// lowest modes carry most of the amplitude, and the FFT's FP32 rounding
+// error, which scales with the norm of its input, shrinks with them.
+template <typename T_, int N_, typename Tc_ = T_, int LOW_ = 0> struct DstExt {
   using T = T_;
   using Tc = Tc_;
   static constexpr int N = N_;
   static constexpr int M = 2 * N;
+  static constexpr int LOW = LOW_;
+  static_assert(LOW >= 0 && LOW < N / 2, "DstExt: LOW must be below N / 2");
+  // type of the outputs handed to the sink
+  using To = typename std::conditional<(LOW > 0), double, Tc>::type;
   // FP64 arithmetic at M >= 4096: 32 elements per thread (not 16), so that
   // one group has at most 128 threads and a column workgroup of 8 layers
   // fits in 512
@@ -544,9 +562,11 @@ template <typename T_, int N_, typename Tc_ = T_> struct DstExt {
 
   struct Ctx {
     TwBase<Tc, M, EF> tw;
+    const cx<double> *tab;  // exp(-2 pi i n / M): (cos, -sin) of pi n / N
     __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64,
-                                         const Tc *__restrict__) {
-      tw.load(t, tw64);
+                                         const Tc *__restrict__, int ts = 1) {
+      tw.load(t, tw64, ts);
+      tab = tw64;  // ts = 1 when LOW > 0 (checked by the caller's table length)
     }
   };
 
@@ -560,9 +580,11 @@ template <typename T_, int N_, typename Tc_ = T_> struct DstExt {
   using S0 = Stage<Tc, M, 0, EF>;
   using SL = Stage<Tc, M, LAST, EF>;
 
-  // Raw inputs of a thread: the first stage's elements of the extension.
+  // Raw inputs of a thread: the first stage's elements of the extension,
+  // and 2 x_j of the split-off inputs j = 1..LOW of both rows.
   struct In {
     cx<Tc> v[S0::NB][S0::R];
+    double la[LOW > 0 ? LOW : 1], lb[LOW > 0 ? LOW : 1];
   };
 
   template <typename Loader>
@@ -577,9 +599,16 @@ template <typename T_, int N_, typename Tc_ = T_> struct DstExt {
         const int k = lo ? e : M - e;
         const int kp = lo ? S0::in_pad(t, pt, b, r) : pad16(k);
         T xa = T(0), xb = T(0);
-        if (k != 0 && k != N) ld(k, kp, xa, xb);
+        if (k > LOW && k != N) ld(k, kp, xa, xb);
         in.v[b][r] = lo ? cx<Tc>{Tc(xa), Tc(xb)} : cx<Tc>{-Tc(xa), -Tc(xb)};
       }
+#pragma unroll
+    for (int j = 0; j < LOW; ++j) {
+      T xa, xb;
+      ld(j + 1, pad16(j + 1), xa, xb);
+      in.la[j] = 2.0 * double(xa);
+      in.lb[j] = 2.0 * double(xb);
+    }
   }
 
   template <typename Loader, typename Sink>
@@ -616,8 +645,24 @@ template <typename T_, int N_, typename Tc_ = T_> struct DstExt {
 #pragma unroll
       for (int r = 0; r < SL::R; ++r) {
         const int k = SL::out_index(t, b, r);
-        if (k < N)
-          st(k, SL::out_pad(t, ob, b, r), k == 0 ? Tc(0) : -v[b][r].y, k == 0 ? Tc(0) : v[b][r].x);
+        if (k < N) {
+          To ya = k == 0 ? To(0) : To(-v[b][r].y), yb = k == 0 ? To(0) : To(v[b][r].x);
+          if constexpr (LOW > 0) {
+            // s_j = sin(pi j k / N): s_{j+1} = 2 cos(pi k / N) s_j - s_{j-1}
+            const cx<double> w = cx_.tab[k];
+            const double c2 = 2.0 * w.x;
+            double s0 = 0.0, s1 = -w.y;
+#pragma unroll
+            for (int j = 0; j < LOW; ++j) {
+              ya = fma(in.la[j], s1, ya);
+              yb = fma(in.lb[j], s1, yb);
+              const double s2 = fma(c2, s1, -s0);
+              s0 = s1;
+              s1 = s2;
+            }
+          }
+          st(k, SL::out_pad(t, ob, b, r), ya, yb);
+        }
       }
   }
 };

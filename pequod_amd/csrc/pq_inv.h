@@ -67,6 +67,8 @@ struct EngineInfo {
   int elem;          // sizeof(T), the LDS (row and FFT buffer) type
   int selem;         // sizeof of the scratch type
   int celem;         // sizeof of the arithmetic type (tables lam4, was)
+  int fregion;       // LDS elements of T per group of the row forward pass
+  int fscan;         //   (forward engine only) and its scan scratch
 };
 
 // Supported transform lengths N = 2^a (8..4096) and 3 * 2^a (12..3072); one
@@ -87,7 +89,7 @@ PQ_INV_SIZES(PQ_INV_DECL)
 
 // LDS bytes of a pass for `groups` FFT groups.
 inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups) {
-  size_t b = (size_t)e.elem * groups * (e.region + e.scan);
+  size_t b = (size_t)e.elem * groups * (pass == 0 ? e.fregion + e.fscan : e.region + e.scan);
   b = (b + 15) & ~size_t(15);
   if (pass == 1) b += 64 * 8 + (size_t)e.celem * (64 + 64 + 8);
   if (pass == 2) b += 16;

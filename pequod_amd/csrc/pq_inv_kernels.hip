@@ -43,6 +43,25 @@
 
 namespace pq {
 
+// Forward and inverse transform engines of one precision.  Both run on the
+// same FFT groups (equal threads per group, row layout and arithmetic
+// type); a group's LDS region holds the larger of the two, and the twiddle
+// table is the longer engine's.
+template <typename F, typename I> struct EngPair {
+  using Fwd = F;
+  using Inv = I;
+  using T = typename I::T;
+  using Tc = typename I::Tc;
+  static_assert(std::is_same<typename F::T, T>::value && std::is_same<typename F::Tc, Tc>::value,
+                "engine pair: forward and inverse engines must share their types");
+  static constexpr int N = I::N, TPF = I::TPF, RS = I::RS;
+  static_assert(F::N == N && F::TPF == TPF && F::RS == RS,
+                "engine pair: forward and inverse engines must share their groups");
+  static constexpr int Region = F::Region > I::Region ? F::Region : I::Region;
+  static constexpr int ScanTmp = F::ScanTmp > I::ScanTmp ? F::ScanTmp : I::ScanTmp;
+  static constexpr int M = F::M > I::M ? F::M : I::M;
+};
+
 template <typename T> struct V16;
 template <> struct V16<float> {
   typedef float type __attribute__((ext_vector_type(4)));
@@ -100,7 +119,8 @@ __device__ __forceinline__ int xcd_block(int i, int n) {
 }
 
 // ------------------------------------------------------------------ K1
-// Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
+// Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads; LDS regions of
+// the forward engine only.
 template <typename Eng, typename Tin, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
@@ -116,16 +136,17 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   const int l = rem / NB, jb = rem - l * NB, j0 = jb << lgR;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int ngrp = R >> 1, nt = ngrp * TPF;
-  typename Eng::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was));
-  T *region = lds + f * Eng::Region;
-  T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
+  typename Eng::Fwd::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
+           Eng::M / Eng::Fwd::M);
+  T *region = lds + f * Eng::Fwd::Region;
+  T *scan_tmp = lds + ngrp * Eng::Fwd::Region + f * Eng::Fwd::ScanTmp;
   const Tin *z = static_cast<const Tin *>(ptrs.zeta[b]) + (size_t)l * ptrs.ldz1 * ptrs.ldz2;
   const int ja = j0 + 2 * f;
   const Tin *za = z + (size_t)ja * ptrs.ldz1;
   const Tin *zb = za + ptrs.ldz1;
   const bool va = ja > 0;  // node row 0 is the boundary: not transformed
-  Eng::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
+  Eng::Fwd::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
     xa = va ? T(za[k]) : T(0);
     xb = T(zb[k]);
   });
@@ -137,7 +158,7 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
     const int e0 = (threadIdx.x + s * nt) * V, r0 = e0 & (R - 1), p = e0 >> lgR;
     VT v;
 #pragma unroll
-    for (int q = 0; q < V; ++q) v[q] = Ts(row_ptr<Eng>(lds, r0 + q)[pad16(p)]);
+    for (int q = 0; q < V; ++q) v[q] = Ts(row_ptr<typename Eng::Fwd>(lds, r0 + q)[pad16(p)]);
     *reinterpret_cast<VT *>(out + e0) = v;
   }
 }
@@ -239,8 +260,9 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   const int ngrp = blockDim.x / TPF, nt = blockDim.x;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int nrows = L << lgC;  // row rho = l * C + c
-  typename Eng::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was));
+  typename Eng::Fwd::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
+           Eng::M / Eng::Fwd::M);
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
   const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
@@ -268,7 +290,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   const Ts *sa = scr + (size_t)la * N * N + (size_t)pa * R;
   const Ts *sb = scr + (size_t)lb * N * N + (size_t)pb * R;
   const int blk = N << lgR;
-  Eng::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
+  Eng::Fwd::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
     const int off = (k >> lgR) * blk + (k & (R - 1));
     xa = va ? T(sa[off]) : T(0);
     xb = vb ? T(sb[off]) : T(0);
@@ -312,8 +334,11 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
 
   // Inverse DST along j, in place.
   {
+    typename Eng::Inv::Ctx ctxi;
+    ctxi.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
+              Eng::M / Eng::Inv::M);
     const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::run(region, t, scan_tmp, ctx, [&](int, int kp, T &xa, T &xb) {
+    Eng::Inv::run(region, t, scan_tmp, ctxi, [&](int, int kp, T &xa, T &xb) {
       xa = rowa[kp];
       xb = rowb[kp];
     });
@@ -373,8 +398,9 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
     for (int m = 1; m < 8; ++m)
       if (m < L) beta[m] = kb[L + l * L + m];
   }
-  typename Eng::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was));
+  typename Eng::Inv::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
+           Eng::M / Eng::Inv::M);
 
   // Tile block (l, jb): row r at p <- S[p * R + r].
 #pragma unroll
@@ -450,9 +476,9 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   const bool za = ja == 0;
   const bool m0 = a.mode == 0;
   const double sub = m0 ? kappa : 0.0;
-  auto sink = [&](int k, int, Tc xa, Tc xb) {
+  auto sink = [&](int k, int, auto xa, auto xb) {
     if (m0 || k != 0) {
-      pa[k] = Tout(double(za ? Tc(0) : xa) - sub);
+      pa[k] = Tout((za ? 0.0 : double(xa)) - sub);
       pb[k] = Tout(double(xb) - sub);
     }
   };
@@ -462,11 +488,11 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
       xa = rowa[kp];
       xb = rowb[kp];
     };
-    if constexpr (Eng::kLaneOut) {
+    if constexpr (Eng::Inv::kLaneOut) {
       // outputs leave the last FFT stage with consecutive k across lanes
-      Eng::run(region, t, scan_tmp, ctx, ld, sink);
+      Eng::Inv::run(region, t, scan_tmp, ctx, ld, sink);
     } else {
-      Eng::run(region, t, scan_tmp, ctx, ld);
+      Eng::Inv::run(region, t, scan_tmp, ctx, ld);
       group_sync<TPF>();
       for (int k = t; k < N; k += TPF) sink(k, 0, Tc(rowa[pad16(k)]), Tc(rowb[pad16(k)]));
     }
@@ -485,8 +511,24 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
 // ------------------------------------------------------------ dispatch
 template <typename Eng, typename Ts> static EngineInfo info_of() {
   return EngineInfo{Eng::M, Eng::TPF, Eng::Region, Eng::ScanTmp, (int)sizeof(typename Eng::T),
-                    (int)sizeof(Ts), (int)sizeof(typename Eng::Tc)};
+                    (int)sizeof(Ts), (int)sizeof(typename Eng::Tc), Eng::Fwd::Region,
+                    Eng::Fwd::ScanTmp};
 }
+
+// Engines per precision.  FP32 transforms are odd extensions (no prefix
+// sum: FFTPACK sint's recurrence accumulates ~sqrt(N) eps, which reaches psi
+// wherever ss_m flattens the solve's damping of high wavenumbers); the
+// inverse ones sum the lowest kLow input modes in FP64 (the solution's
+// spectrum is dominated by them, so the FFT's rounding error, which scales
+// with its input norm, shrinks with them).
+constexpr int kLow = PQ_N >= 16 ? 4 : PQ_N / 2 - 1;
+using ExtF = DstExt<float, PQ_N, float>;
+using ExtFLow = DstExt<float, PQ_N, float, kLow>;
+using SintD = DstSint<double, PQ_N>;
+using ExtWide = DstExt<float, PQ_N, double>;
+using P32 = EngPair<ExtF, ExtFLow>;
+using P64 = EngPair<SintD, SintD>;
+using PWide = EngPair<ExtWide, ExtWide>;
 
 // Dynamic LDS above the 64 KiB default needs a per-kernel limit (gfx950 has
 // 160 KiB per CU); raised once per kernel to the largest size requested.
@@ -521,18 +563,17 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
 #define PQ_CAT(a, b) PQ_CAT2(a, b)
 
 // PQ_PREC_FP64: FP64 everywhere (FFTPACK-style sint engine).
-// PQ_PREC_FP32: FP32 fields, scratch and transforms (odd-extension engine).
+// PQ_PREC_FP32: FP32 fields, scratch and transforms (P32 engines).
 // PQ_PREC_MIXED: FP64 fields, FP32 scratch and transforms.
 // PQ_PREC_FP32_STORAGE: FP32 fields and scratch, FP64 transforms.
 // PQ_PREC_FP32_WIDE: FP32 fields, scratch and LDS, FP64 arithmetic
 // (odd-extension engine).
-using ExtWide = DstExt<float, PQ_N, double>;
 EngineInfo PQ_CAT(inv_info_, PQ_N)(int prec) {
   switch (prec) {
-    case PQ_PREC_FP64: return info_of<DstSint<double, PQ_N>, double>();
-    case PQ_PREC_FP32_STORAGE: return info_of<DstSint<double, PQ_N>, float>();
-    case PQ_PREC_FP32_WIDE: return info_of<ExtWide, float>();
-    default: return info_of<DstExt<float, PQ_N>, float>();
+    case PQ_PREC_FP64: return info_of<P64, double>();
+    case PQ_PREC_FP32_STORAGE: return info_of<P64, float>();
+    case PQ_PREC_FP32_WIDE: return info_of<PWide, float>();
+    default: return info_of<P32, float>();
   }
 }
 
@@ -540,15 +581,15 @@ hipError_t PQ_CAT(inv_launch_, PQ_N)(int prec, int pass, const InvArgs &a, const
                                      int blocks, int threads, size_t lds, hipStream_t s) {
   switch (prec) {
     case PQ_PREC_FP64:
-      return launch_pass<DstSint<double, PQ_N>, double, double>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P64, double, double>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32:
-      return launch_pass<DstExt<float, PQ_N>, float, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P32, float, float>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32_STORAGE:
-      return launch_pass<DstSint<double, PQ_N>, float, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P64, float, float>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32_WIDE:
-      return launch_pass<ExtWide, float, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<PWide, float, float>(pass, a, pt, blocks, threads, lds, s);
     default:
-      return launch_pass<DstExt<float, PQ_N>, double, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P32, double, float>(pass, a, pt, blocks, threads, lds, s);
   }
 }
 

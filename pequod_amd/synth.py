@@ -134,6 +134,17 @@ def config(name: str) -> QGConfig:
     raise KeyError(name)
 
 
+def steep_member(layers: int, ratio: float, n: int = 127, member: int = 99):
+    """Member whose layer depths grow by `ratio` per layer (summing to
+    4000 m; g' linear 0.03 -> 0.005): an ill-conditioned layer <-> mode
+    projection for large ratios (8 layers, ratio 2.5: kappa_inf 85.8)."""
+    H = np.array([ratio ** k for k in range(layers)])
+    H = list(4000.0 * H / H.sum())
+    gp = list(np.linspace(0.03, 0.005, layers - 1))
+    return QGConfig(f"steep {layers}-layer member", n, 5e3, H, gp, rms=1e-5, config_id=3,
+                    extra={"member": member})
+
+
 def ensemble_member(member: int, seed_base: int = 3):
     """Member of the 64-member mixed ensemble: interior size from
     {255, 511, 767, 1023}, 2..8 layers, H geometric summing to 4000 m,

@@ -62,8 +62,9 @@ def test_supported_sizes_rejected_without_device():
 
 
 def test_fp32_precision_choice_follows_projection_condition():
-    """FP32 transforms for the 3-layer stratification (kappa_inf 6.8), FP64
-    transforms for the 6-layer one (kappa_inf 16) and for 8 layers."""
+    """FP32 transforms up to kappa_inf = 32: the 3-layer (4.1), 6-layer
+    (14.4) and 8-layer ensemble (26.2) stratifications; FP64 transforms for
+    8 layers with depths growing 2.5x per layer (85.8)."""
     import ctypes
     import numpy as np
     from pequod_amd import _lib, synth
@@ -76,6 +77,7 @@ def test_fp32_precision_choice_follows_projection_condition():
         return L.pq_prec_fp32_choice(st.layers, m2z.ctypes.data_as(dp), z2m.ctypes.data_as(dp))
 
     assert choice(synth.config("qg3_1023").stratification()) == _lib.PREC["fp32"]
-    assert choice(synth.config("qg6_2047").stratification()) == _lib.PREC["fp32s"]
+    assert choice(synth.config("qg6_2047").stratification()) == _lib.PREC["fp32"]
     eight = [m for m in (synth.ensemble_member(k) for k in range(16)) if m.layers == 8][0]
-    assert choice(eight.stratification()) == _lib.PREC["fp32s"]
+    assert choice(eight.stratification()) == _lib.PREC["fp32"]
+    assert choice(synth.steep_member(8, 2.5).stratification()) == _lib.PREC["fp32s"]

@@ -111,10 +111,11 @@ def test_ensemble_model_step_matches_standalone(dev, graph):
 
 def test_ensemble_fp32auto_within_1e6(dev):
     """Mixed ensemble members with 2..8 layers (the ensemble workload's
-    stratifications, on small grids): the FP32-storage ensemble with the
-    per-class transform choice stays within 1e-6 max relative psi error of
-    the FP64 oracle for every member; well-conditioned classes use FP32
-    transforms, the others FP64 transforms."""
+    stratifications, on small grids) plus an ill-conditioned 8-layer member
+    (kappa_inf 85.8): the FP32-storage ensemble with the per-class transform
+    choice stays within 1e-6 max relative psi error of the FP64 oracle for
+    every member; the ensemble's classes use FP32 transforms, the steep
+    member's class FP64 transforms."""
     import torch
     from oracle import oracle as O
     from helpers import from_dev, rel_err
@@ -125,6 +126,7 @@ def test_ensemble_fp32auto_within_1e6(dev):
             c.n = 127 if k % 2 else 191
             cfgs.append(c)
     assert sorted(c.layers for c in cfgs) == list(range(2, 9))
+    cfgs.append(synth.steep_member(8, 2.5, n=191))
     ens = Ensemble("fp32auto")
     zs, ps, refs = [], [], []
     for c in cfgs:

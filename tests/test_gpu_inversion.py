@@ -203,6 +203,10 @@ def test_inversion_reduced_precision_k3(dev, prec, cfg):
     if prec == "fp32w":
         # FP64 arithmetic, FP32 rounding between FFT stages
         assert err < 5e-7
+    if prec in ("fp32", "mixed"):
+        # FP32 transforms with the lowest input modes of the inverse
+        # transforms summed in FP64 (measured 1.3e-7 .. 1.7e-7)
+        assert err < 4e-7
     for l in range(st.layers):
         assert abs(O.mean(out[:, :, l])) < 1e-6 * np.max(np.abs(ref[:, :, l]))
 
@@ -283,11 +287,9 @@ def test_inversion_rejects_unsupported_size(dev):
 @pytest.mark.parametrize("prec", ["fp64", "fp32s", "fp32", "fp32w"])
 def test_inversion_six_layer_2047(dev, prec):
     """The 6-layer 2047^2 workload (dx 2.5 km, FP32 storage for the reduced
-    paths): FP32 storage with FP64 transforms stays within 1e-6 max relative
-    psi error vs the FP64 oracle (FP64 within its condition-scaled bound).
-    FP32 transforms reach ~1.2e-6 here: their ~2.5 eps per transform is
-    amplified by the 6-layer projection (cond(modes_to_z) ~ 8), so they are
-    bounded at 2e-6 and fp32s is the mode for this workload."""
+    paths): every FP32-storage mode stays within 1e-6 max relative psi error
+    vs the FP64 oracle (FP64 within its condition-scaled bound); FP32
+    transforms measure 3.4e-7 here."""
     import torch
     c = synth.config("qg6_2047")
     st = c.stratification()
@@ -304,5 +306,5 @@ def test_inversion_six_layer_2047(dev, prec):
     out = from_dev(pt)
     err = rel_err(out, ref)
     print(f"{prec} rel err {err:.3e}", flush=True)
-    bound = {"fp64": fp64_tol(g, st.modes_to_z), "fp32s": 1e-6, "fp32": 2e-6, "fp32w": 1e-6}[prec]
+    bound = {"fp64": fp64_tol(g, st.modes_to_z), "fp32s": 1e-6, "fp32": 1e-6, "fp32w": 1e-6}[prec]
     assert err < bound
