@@ -201,6 +201,19 @@ def main():
                 e = autotune.record(g, L, prec, bt, res)
                 log(f"[sweep] {prec} B={bt} best {e}")
                 hc.deallocate()
+        # size classes of the 64-member mixed ensemble (FP32 storage)
+        from pequod_amd.ensemble import Ensemble
+        cfgs = [synth.ensemble_member(k) for k in range(64)]
+        ens = Ensemble("fp32auto")
+        keep = []
+        for cm in cfgs:
+            zt = torch.from_numpy(np.ascontiguousarray(cm.q(member=cm.extra["member"]).transpose(2, 1, 0)))
+            zt = zt.to(dev, torch.float32)
+            keep += [zt, torch.zeros_like(zt)]
+            ens.add(cm.stratification(), keep[-2], keep[-1])
+        ens.finalize(tiles=None)
+        autotune.sweep_ensemble(ens, log=log)
+        ens.close()
         return
 
     log(f"[bench] 3-layer 1023^2 {args.precision} batch {B}: {args.steps} steps")

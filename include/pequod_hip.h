@@ -136,6 +136,12 @@ int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1,
  * share a grid size and layer count but not a stratification.  Synchronous. */
 int pq_inv_set_member(pq_inv *h, int slot, const double *ss,
                       const double *modes_to_z, const double *z_to_modes);
+/* As pq_inv_set_member for a member of `layers` <= the handle's layer count
+ * (no counterpart in the reference): one handle then serves members of
+ * several layer counts on the same grid in one batch (ragged batches); its
+ * psi / zeta views hold `layers` layers.  Synchronous.                     */
+int pq_inv_set_member_layers(pq_inv *h, int slot, int layers, const double *ss,
+                             const double *modes_to_z, const double *z_to_modes);
 /* Waits for the handle's last run and returns its status (NaN / large
  * psiC); psic (may be NULL) receives batch*L values of the last run.       */
 int pq_inv_status(pq_inv *h, double *psic);

@@ -49,6 +49,7 @@ SIGNATURES = {
     "pq_inv_stream": (_vp, [_vp]),
     "pq_inv_set_tiles": (_i, [_vp, _i, _i]),
     "pq_inv_set_member": (_i, [_vp, _i, _dp, _dp, _dp]),
+    "pq_inv_set_member_layers": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "pq_inv_get_tiles": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "pq_inv_pass_times": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _i,
                                ctypes.POINTER(ctypes.c_float)]),

@@ -79,3 +79,49 @@ def record(g: int, layers: int, precision: str, batch: int, results) -> dict:
     with open(TABLE, "w") as f:
         json.dump(tab, f, indent=1, sort_keys=True)
     return tab[key(g, layers, precision, batch)]
+
+
+def sweep_ensemble(ens, reps: int = 10, log=None):
+    """Tile shapes of a finalized Ensemble's size classes, tuned in place:
+    class by class (the others fixed), every accepted (R, C) is timed as
+    the whole ensemble's inversion (wall time of `reps` eager runs, so the
+    classes' overlap on the device counts), and the fastest is kept and
+    stored in tiles.json under the class's key (grid, largest layer count,
+    precision, member count bucket).  Returns {class: (R, C, ms)}."""
+    import ctypes
+    import time
+
+    from ._lib import load
+
+    lib = load()
+
+    def run_ms():
+        ens.invert()
+        ens.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ens.invert()
+        ens.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    out = {}
+    tab = _table()
+    for k in range(ens.class_count):
+        h = lib.pq_ens_class_handle(ens._h, k)
+        res = []
+        for R in ROWS:
+            for C in COLS:
+                if lib.pq_inv_set_tiles(h, R, C) == 0:
+                    res.append((run_ms(), R, C))
+        ms, R, C = min(res)
+        lib.pq_inv_set_tiles(h, R, C)
+        g, L, n = ens.class_shape(k)
+        prec = ens.class_precision(k)
+        tab[key(g, L, prec, n)] = {"R": R, "C": C, "ensemble_ms": round(ms, 4)}
+        out[k] = (R, C, ms)
+        if log:
+            log(f"[sweep] ensemble class {k} (g={g}, L<={L}, {n} members, {prec}): R={R} C={C} "
+                f"{ms:.3f} ms")
+    with open(TABLE, "w") as f:
+        json.dump(tab, f, indent=1, sort_keys=True)
+    return out

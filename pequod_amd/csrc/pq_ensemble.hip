@@ -1,10 +1,12 @@
 // pq_ensemble.hip -- ensembles of independent members of mixed grid sizes,
 // layer counts and stratifications on one GPU.
 //
-// Members are grouped into size classes (same grid, layer count and array
-// leading dimensions).  Each class owns one PV-inversion handle whose member
-// slots carry the members' own stratifications (pq_inv_set_member), so one
-// inversion of the whole ensemble is one batched launch sequence per class.
+// Members are grouped into size classes (same grid, array leading
+// dimensions and transform precision; any layer counts).  Each class owns
+// one PV-inversion handle for its largest layer count whose member slots
+// carry the members' own layer counts and stratifications
+// (pq_inv_set_member_layers), so one inversion of the whole ensemble is one
+// ragged batched launch sequence per class.
 // Classes run on their own streams, forked from and joined back to the
 // ensemble's stream, so small classes overlap on the device.  The sequence
 // (and, for members that are CABARET models, the whole time step: the first
@@ -177,10 +179,14 @@ int pq_ens_add_model(pq_ens *e, pq_model *model, const double *ss, const double 
 int pq_ens_finalize(pq_ens *e) {
   PQ_CHECK_ARG(e && !e->finalized, "pq_ens_finalize: null or finalized ensemble");
   PQ_CHECK_ARG(!e->mem.empty(), "pq_ens_finalize: no members");
-  std::map<std::tuple<int, int, long, long, long, long>, int> key2cls;
+  std::map<std::tuple<int, long, long, long, long, int>, int> key2cls;
   for (size_t i = 0; i < e->mem.size(); ++i) {
     auto &m = e->mem[i];
-    auto key = std::make_tuple(m.g, m.L, m.ldz1, m.ldz2, m.ldp1, m.ldp2);
+    // FP32_AUTO resolves per member (pq_prec_fp32_choice)
+    const int prec = e->prec == PQ_PREC_FP32_AUTO
+                         ? pq_prec_fp32_choice(m.L, m.m2z.data(), m.z2m.data())
+                         : e->prec;
+    auto key = std::make_tuple(m.g, m.ldz1, m.ldz2, m.ldp1, m.ldp2, prec);
     auto it = key2cls.find(key);
     int k;
     if (it == key2cls.end()) {
@@ -188,40 +194,40 @@ int pq_ens_finalize(pq_ens *e) {
       key2cls[key] = k;
       pq_ens::Class c;
       c.g = m.g;
-      c.L = m.L;
+      c.L = 0;
       c.ldz1 = m.ldz1;
       c.ldz2 = m.ldz2;
       c.ldp1 = m.ldp1;
       c.ldp2 = m.ldp2;
+      c.prec = prec;
       e->cls.push_back(c);
     } else {
       k = it->second;
     }
+    auto &c = e->cls[k];
     m.cls = k;
-    m.slot = (int)e->cls[k].members.size();
-    e->cls[k].members.push_back((int)i);
-    e->cls[k].z.push_back(m.zeta);
-    e->cls[k].p.push_back(m.psi);
+    m.slot = (int)c.members.size();
+    c.L = std::max(c.L, m.L);
+    c.members.push_back((int)i);
+    c.z.push_back(m.zeta);
+    c.p.push_back(m.psi);
   }
   for (auto &c : e->cls) {
-    const auto &m0 = e->mem[c.members[0]];
-    // FP32_AUTO: the more accurate FP32-storage mode for the class when any
-    // member needs it
-    int prec = e->prec;
-    if (prec == PQ_PREC_FP32_AUTO) {
-      prec = PQ_PREC_FP32;
-      for (int q : c.members) {
-        const int p = pq_prec_fp32_choice(c.L, e->mem[q].m2z.data(), e->mem[q].z2m.data());
-        if (p != PQ_PREC_FP32) prec = p;
+    // created with a member of the largest layer count; every slot then
+    // gets its member's own layers and stratification
+    int top = c.members[0];
+    for (int q : c.members)
+      if (e->mem[q].L == c.L) {
+        top = q;
+        break;
       }
-    }
-    c.prec = prec;
-    int rc = pq_inv_create(&c.inv, c.g, c.L, m0.ss.data(), m0.m2z.data(), m0.z2m.data(), prec,
+    const auto &m0 = e->mem[top];
+    int rc = pq_inv_create(&c.inv, c.g, c.L, m0.ss.data(), m0.m2z.data(), m0.z2m.data(), c.prec,
                            (int)c.members.size());
     if (rc) return rc;
-    for (size_t q = 1; q < c.members.size(); ++q) {
+    for (size_t q = 0; q < c.members.size(); ++q) {
       const auto &m = e->mem[c.members[q]];
-      rc = pq_inv_set_member(c.inv, (int)q, m.ss.data(), m.m2z.data(), m.z2m.data());
+      rc = pq_inv_set_member_layers(c.inv, (int)q, m.L, m.ss.data(), m.m2z.data(), m.z2m.data());
       if (rc) return rc;
     }
     c.stream = (hipStream_t)pq_inv_stream(c.inv);

@@ -26,13 +26,33 @@ struct CoefLayout {
   }
 };
 
-// Member arrays of one launch (passed by value).
+// Member arrays of one launch (passed by value).  Members may have fewer
+// layers than the handle (ragged batches): member b has lay[b] layers, its
+// column-pass tiles are 2^lgc[b] columns wide, and its workgroups are the
+// blocks cum_r[b] .. cum_r[b+1]-1 of the row passes and cum_c[b] ..
+// cum_c[b+1]-1 of the column pass.
 constexpr int kMaxBatchArg = 64;
 struct PtrTab {
   const void *zeta[kMaxBatchArg];
   void *psi[kMaxBatchArg];
   long long ldz1, ldz2, ldp1, ldp2;
+  int nb;
+  int cum_r[kMaxBatchArg + 1];
+  int cum_c[kMaxBatchArg + 1];
+  unsigned char lay[kMaxBatchArg];
+  unsigned char lgc[kMaxBatchArg];
 };
+
+// Member of block `blk` given the block prefix sums cum[0..nb] (binary search).
+__device__ __forceinline__ int member_of(const int *cum, int nb, int blk) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cum[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 // Launch-wide arguments of the three passes.  Member b of a launch (b < nb)
 // uses the scratch at scr + b * scr_stride bytes (L x N x N elements of the
@@ -43,7 +63,8 @@ struct InvArgs {
   long long scr_stride;
   double *part;
   int part_stride;
-  double *psic;      // L per member: psiC of the last run
+  double *psic;      // psic_stride per member: psiC of the last run
+  int psic_stride;
   double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m
   int kb_stride;
   int *status;       // sticky per member: 1 NaN psiC, 2 |psiC| > 1e100
@@ -53,9 +74,8 @@ struct InvArgs {
   const void *was;   // Tc[N/2]: 2 sin(k pi / N) (DstSint), Tc the arithmetic type
   const void *lam4;  // Tc[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
   const double *shat;  // double[N]: sum_i 2 sin(pi p i / N) (0 for even p)
-  int L, mode;       // mode 0 pv_inversion, 1 helm
+  int L, mode;       // L: the handle's (largest) layer count; mode 0 pv_inversion, 1 helm
   int R, lgR;        // rows per tile of the row passes (scratch block height)
-  int C, lgC;        // columns per workgroup of the column pass
 };
 
 // Static properties of a transform engine (per precision and length).

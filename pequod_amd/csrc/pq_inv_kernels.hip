@@ -119,8 +119,8 @@ __device__ __forceinline__ int xcd_block(int i, int n) {
 }
 
 // ------------------------------------------------------------------ K1
-// Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads; LDS regions of
-// the forward engine only.
+// Grid: sum over members of L_b * (N / R) workgroups of (R / 2) * TPF
+// threads; LDS regions of the forward engine only.
 template <typename Eng, typename Tin, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
@@ -131,8 +131,8 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
   const int R = a.R, lgR = a.lgR, NB = N >> lgR;
-  const int per_member = a.L * NB;
-  const int b = blockIdx.x / per_member, rem = blockIdx.x - b * per_member;
+  const int b = member_of(ptrs.cum_r, ptrs.nb, blockIdx.x);
+  const int rem = blockIdx.x - ptrs.cum_r[b];
   const int l = rem / NB, jb = rem - l * NB, j0 = jb << lgR;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int ngrp = R >> 1, nt = ngrp * TPF;
@@ -165,30 +165,37 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
 
 // Spectral step of the column pass on the points (p0 + c, q), c < C,
 // 1 <= q < N, rows (l, c) at LDS row l C + c, computed in the engine's
-// arithmetic type Tc (the coefficients cD, cM, cS are in Tc).  pv_inversion (mode 0):
+// arithmetic type Tc.  pv_inversion (LT = L, 1..8):
 // phi_m = sum_l D(m,l) x_l / ((2N)^2 (lambda_p + lambda_q - ss_m)), FP64
 // psiC partial acc_m += phi_m shat_p shat_q, x_l = sum_m M(l,m) phi_m.
-// helm (mode 1): x_l /= (2N)^2 (lambda_p + lambda_q - ss_l).  LT is the
-// layer count when small enough to hold the projections in registers (1..4),
-// else 0 (coefficients read from LDS per point).
+// helm (LT = 0): x_l /= (2N)^2 (lambda_p + lambda_q - ss_l).  The
+// coefficients sit in LDS as rows of 8: cD[m * 8 + l] = D(m,l),
+// cM[l * 8 + m] = M(l,m), cS[m] = (2N)^2 ss_m; up to 4 layers they are held
+// in registers, above that each row is read per point as 16-byte vectors.
 template <typename Eng, int LT>
 __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T *lds,
                                               const typename Eng::Tc *cD, const typename Eng::Tc *cM,
-                                              const typename Eng::Tc *cS, int p0, int nt,
-                                              double (&acc)[8]) {
+                                              const typename Eng::Tc *cS, int p0, int nt, int Lm,
+                                              int lgC, double (&acc)[8]) {
   using T = typename Eng::T;
   using Tc = typename Eng::Tc;
+  using VC = typename V16<Tc>::type;
+  constexpr int VN = V16<Tc>::n;
   constexpr int N = Eng::N;
-  const int L = LT > 0 ? LT : a.L, C = a.C, lgC = a.lgC;
+  constexpr bool REG = LT >= 1 && LT <= 4;
+  constexpr int LR = REG ? LT : 1;
+  constexpr int NV = (LT + VN - 1) / VN;  // 16-byte vectors per coefficient row
+  const int C = 1 << lgC;
   const Tc *lam4 = static_cast<const Tc *>(a.lam4);
-  constexpr int LR = LT > 0 ? LT : 1;
   Tc d[LR * LR], mm[LR * LR], sv[LR];
-  if constexpr (LT > 0) {
+  if constexpr (REG) {
 #pragma unroll
-    for (int i = 0; i < LT * LT; ++i) {
-      d[i] = cD[i];
-      mm[i] = cM[i];
-    }
+    for (int m = 0; m < LT; ++m)
+#pragma unroll
+      for (int l = 0; l < LT; ++l) {
+        d[m * LT + l] = cD[m * 8 + l];
+        mm[l * LT + m] = cM[l * 8 + m];
+      }
 #pragma unroll
     for (int i = 0; i < LT; ++i) sv[i] = cS[i];
   }
@@ -198,64 +205,80 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
     const Tc lp = lam4[p];
     const double sp = a.shat[p];
     for (int q = 1 + threadIdx.x; q < N; q += nt) {
-      if constexpr (LT == 0) {
-        // keep the L x L coefficient reads inside the loop (hoisted into
-        // registers they would spill for large L)
-        asm volatile("" ::: "memory");
-      }
       const Tc lpq = lp + lam4[q];
-      if (a.mode == 0) {
-        const double wpq = sp * a.shat[q];
-        Tc col[8], phi[8];
-#pragma unroll
-        for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) col[ll] = Tc(row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)]);
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-          if (m < L) {
-            Tc s = Tc(0);
-#pragma unroll
-            for (int ll = 0; ll < 8; ++ll)
-              if (ll < L) s += (LT > 0 ? d[(m + L * ll) % (LR * LR)] : cD[m + L * ll]) * col[ll];
-            phi[m] = s / (lpq - (LT > 0 ? sv[m % LR] : cS[m]));
-            acc[m] += double(phi[m]) * wpq;  // wpq = 0 unless p, q odd
-          }
-#pragma unroll
-        for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) {
-            Tc s = Tc(0);
-#pragma unroll
-            for (int m = 0; m < 8; ++m)
-              if (m < L) s += (LT > 0 ? mm[(ll + L * m) % (LR * LR)] : cM[ll + L * m]) * phi[m];
-            row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = T(s);
-          }
+      if constexpr (LT == 0) {
+        for (int ll = 0; ll < Lm; ++ll) {
+          T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(q);
+          *x = T(Tc(*x) / (lpq - cS[ll]));
+        }
       } else {
+        if constexpr (!REG) {
+          // keep the coefficient rows' reads inside the loop (hoisted into
+          // registers they would spill)
+          asm volatile("" ::: "memory");
+        }
+        const double wpq = sp * a.shat[q];
+        Tc col[LT], phi[LT];
 #pragma unroll
-        for (int ll = 0; ll < 8; ++ll)
-          if (ll < L) {
-            T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(q);
-            *x = T(Tc(*x) / (lpq - cS[ll]));
+        for (int ll = 0; ll < LT; ++ll) col[ll] = Tc(row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)]);
+#pragma unroll
+        for (int m = 0; m < LT; ++m) {
+          Tc s = Tc(0);
+          if constexpr (REG) {
+#pragma unroll
+            for (int ll = 0; ll < LT; ++ll) s += d[m * LT + ll] * col[ll];
+            phi[m] = s / (lpq - sv[m]);
+          } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+              const VC dv = *reinterpret_cast<const VC *>(cD + m * 8 + v * VN);
+#pragma unroll
+              for (int e = 0; e < VN; ++e)
+                if (v * VN + e < LT) s += dv[e] * col[v * VN + e];
+            }
+            phi[m] = s / (lpq - cS[m]);
           }
+          acc[m] += double(phi[m]) * wpq;  // wpq = 0 unless p, q odd
+        }
+#pragma unroll
+        for (int ll = 0; ll < LT; ++ll) {
+          Tc s = Tc(0);
+          if constexpr (REG) {
+#pragma unroll
+            for (int m = 0; m < LT; ++m) s += mm[ll * LT + m] * phi[m];
+          } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+              const VC mv = *reinterpret_cast<const VC *>(cM + ll * 8 + v * VN);
+#pragma unroll
+              for (int e = 0; e < VN; ++e)
+                if (v * VN + e < LT) s += mv[e] * phi[v * VN + e];
+            }
+          }
+          row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = T(s);
+        }
       }
     }
   }
 }
 
 // ------------------------------------------------------------------ K2
-// Grid: nb * (N / C) workgroups of ceil(C L / 2) * TPF threads.
+// Grid: sum over members of N / C_b workgroups of G * TPF threads, G >=
+// ceil(C_b L_b / 2) for every member.
 template <typename Eng, typename Ts>
-__global__ void __launch_bounds__(512) k_cols(InvArgs a) {
+__global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
   using Tc = typename Eng::Tc;
   using VT = typename V16<Ts>::type;
   constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int L = a.L, C = a.C, lgC = a.lgC, R = a.R, lgR = a.lgR;
-  const int per_member = N >> lgC;
+  const int R = a.R, lgR = a.lgR;
   // neighbouring column blocks share the cache lines of the C R runs
   const int bid = xcd_block(blockIdx.x, gridDim.x);
-  const int b = bid / per_member, pblk = bid - b * per_member;
+  const int b = member_of(ptrs.cum_c, ptrs.nb, bid);
+  const int L = ptrs.lay[b], lgC = ptrs.lgc[b], C = 1 << lgC;
+  const int pblk = bid - ptrs.cum_c[b];
   const int p0 = pblk << lgC;
   const int ngrp = blockDim.x / TPF, nt = blockDim.x;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
@@ -267,16 +290,22 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
   const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
   double *red = reinterpret_cast<double *>(smem + red_off);
-  Tc *cD = reinterpret_cast<Tc *>(red + 64);  // D, M, ssN in Tc (L <= 8)
+  Tc *cD = reinterpret_cast<Tc *>(red + 64);  // D, M (rows of 8), ssN in Tc (L <= 8)
   Tc *cM = cD + 64;
   Tc *cS = cM + 64;
   const CoefLayout CL{L, N};
   const double *coef = a.coef + b * a.coef_stride;
-  // coefficients: loads issued now, stored to LDS after the forward transform
+  // coefficients: loads issued now, stored to LDS after the forward
+  // transform.  Thread i < 64 takes entry (r, c) = (i / 8, i % 8) of both
+  // matrices (zero outside L x L): cD[i] = D(r,c), cM[i] = M(r,c); in the
+  // coefficient block X(r,c) sits at r + L c.
   double cd0 = 0.0, cm0 = 0.0, cs0 = 0.0;
-  if ((int)threadIdx.x < L * L) {
-    cd0 = coef[CL.D() + threadIdx.x];
-    cm0 = coef[CL.M() + threadIdx.x];
+  {
+    const int i = threadIdx.x, r = i >> 3, c = i & 7;
+    if (i < 64 && r < L && c < L) {
+      cd0 = coef[CL.D() + r + L * c];
+      cm0 = coef[CL.M() + r + L * c];
+    }
   }
   if ((int)threadIdx.x < L) cs0 = coef[CL.ssN() + threadIdx.x];
 
@@ -295,15 +324,17 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
     xa = va ? T(sa[off]) : T(0);
     xb = vb ? T(sb[off]) : T(0);
   });
-  if ((int)threadIdx.x < L * L) {
+  if ((int)threadIdx.x < 64) {
     cD[threadIdx.x] = Tc(cd0);
     cM[threadIdx.x] = Tc(cm0);
   }
   if ((int)threadIdx.x < L) cS[threadIdx.x] = Tc(cs0);
   // small workgroups (fewer threads than coefficients)
-  for (int i = threadIdx.x + nt; i < L * L; i += nt) {
-    cD[i] = Tc(coef[CL.D() + i]);
-    cM[i] = Tc(coef[CL.M() + i]);
+  for (int i = threadIdx.x + nt; i < 64; i += nt) {
+    const int r = i >> 3, c = i & 7;
+    const bool in = r < L && c < L;
+    cD[i] = in ? Tc(coef[CL.D() + r + L * c]) : Tc(0);
+    cM[i] = in ? Tc(coef[CL.M() + r + L * c]) : Tc(0);
   }
   for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = Tc(coef[CL.ssN() + i]);
   __syncthreads();
@@ -313,11 +344,15 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = 0.0;
   switch (a.mode == 0 ? L : 0) {
-    case 1: spectral_step<Eng, 1>(a, lds, cD, cM, cS, p0, nt, acc); break;
-    case 2: spectral_step<Eng, 2>(a, lds, cD, cM, cS, p0, nt, acc); break;
-    case 3: spectral_step<Eng, 3>(a, lds, cD, cM, cS, p0, nt, acc); break;
-    case 4: spectral_step<Eng, 4>(a, lds, cD, cM, cS, p0, nt, acc); break;
-    default: spectral_step<Eng, 0>(a, lds, cD, cM, cS, p0, nt, acc); break;
+    case 1: spectral_step<Eng, 1>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 2: spectral_step<Eng, 2>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 3: spectral_step<Eng, 3>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 4: spectral_step<Eng, 4>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 5: spectral_step<Eng, 5>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 6: spectral_step<Eng, 6>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 7: spectral_step<Eng, 7>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 8: spectral_step<Eng, 8>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    default: spectral_step<Eng, 0>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
   }
   // psiC partial of this workgroup (fixed-order FP64 reduction, which also
   // publishes the spectral step's LDS writes); the row inverse pass reduces
@@ -361,7 +396,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a) {
 }
 
 // ------------------------------------------------------------------ K3
-// Grid: nb * L * (N / R) workgroups of (R / 2) * TPF threads.
+// Grid: sum over members of L_b * (N / R) workgroups of (R / 2) * TPF threads.
 template <typename Eng, typename Tout, typename Ts>
 __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
@@ -371,9 +406,10 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int L = a.L, R = a.R, lgR = a.lgR, NB = N >> lgR;
-  const int per_member = L * NB;
-  const int b = blockIdx.x / per_member, rem = blockIdx.x - b * per_member;
+  const int R = a.R, lgR = a.lgR, NB = N >> lgR;
+  const int b = member_of(ptrs.cum_r, ptrs.nb, blockIdx.x);
+  const int L = ptrs.lay[b];
+  const int rem = blockIdx.x - ptrs.cum_r[b];
   const int l = rem / NB, jb = rem - l * NB, j0 = jb << lgR;
   const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int ngrp = R >> 1, nt = ngrp * TPF;
@@ -551,7 +587,7 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
     hipLaunchKernelGGL((k_rows_fwd<Eng, Tio, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   } else if (pass == 1) {
     lds_limit(k_cols<Eng, Ts>, lds);
-    hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a);
+    hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   } else {
     lds_limit(k_rows_inv<Eng, Tio, Ts>, lds);
     hipLaunchKernelGGL((k_rows_inv<Eng, Tio, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);

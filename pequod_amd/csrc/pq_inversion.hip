@@ -57,10 +57,11 @@ static int ilog2i(int x) {
 // beta_lm = M(l,m) psiC_m b_m (qg3p_diagnostics.F90:615-635 rewritten for
 // the closed-form correction).  Status bits: 1 NaN psiC, 2 |psiC| > 1e100
 // (qg3p_aux.F90:504-517).  Grid: one 256-thread workgroup per member.
-__global__ void __launch_bounds__(256) k_psic(InvArgs a, int nblk, int N) {
+__global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
   __shared__ double red[4 * 8];
   __shared__ double s_psic[8];
-  const int b = blockIdx.x, L = a.L, tid = threadIdx.x;
+  const int b = blockIdx.x, L = ptrs.lay[b], tid = threadIdx.x;
+  const int nblk = N >> ptrs.lgc[b];
   const double *part = a.part + (size_t)b * a.part_stride;
   double acc[8];
 #pragma unroll
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, int nblk, int N) {
     for (int m = 0; m < L; ++m) {
       const double v = (red[m] + red[8 + m] + red[16 + m] + red[24 + m]) * inv_n2;
       s_psic[m] = v;
-      a.psic[(size_t)b * L + m] = v;
+      a.psic[(size_t)b * a.psic_stride + m] = v;
       if (v != v) st |= 1;
       else if (fabs(v) > 1.0e100) st |= 2;
     }
@@ -118,6 +119,7 @@ using namespace pq;
 struct pq_inv {
   int g = 0, N = 0, L = 0, prec = 0, batch_max = 0, mode = 0;
   int R = 0, C = 0;
+  std::vector<int> lay;  // layers of each member slot (<= L)
   KernelSet ks;
   EngineInfo ei{};
   // tables
@@ -173,10 +175,11 @@ template <typename T> static int upload_tables(pq_inv *h, int M) {
   return PQ_OK;
 }
 
-// Coefficient block of one member (CoefLayout) into c[0 .. CL.size()).
-static void compute_coefs(const pq_inv *h, const double *ss, const double *m2z, const double *z2m,
-                          double *c) {
-  const int L = h->L, N = h->N;
+// Coefficient block of one member with L layers (CoefLayout) into
+// c[0 .. CL.size()).
+static void compute_coefs(const pq_inv *h, int L, const double *ss, const double *m2z,
+                          const double *z2m, double *c) {
+  const int N = h->N;
   CoefLayout CL{L, N};
   std::fill(c, c + CL.size(), 0.0);
   const double fourN2 = 4.0 * N * (double)N;
@@ -242,8 +245,10 @@ static void default_tiles(const pq_inv *h, int &R, int &C) {
   R = 16;
   while (R > 2 && !tiles_ok(h, R, 1, false)) R /= 2;
   if (h->ei.selem == 8 && R > 8) R = 8;  // FP64 scratch: 64-byte runs at R = 8
+  // column workgroups of at most 4 FFT groups (C L <= 8 rows), so that two
+  // or more fit a CU's LDS; 2 columns where that allows
   C = 2;
-  while (C > 1 && !tiles_ok(h, R, C, false)) C /= 2;
+  while (C > 1 && ((C * h->L + 1) / 2 > 4 || !tiles_ok(h, R, C, false))) C /= 2;
 }
 
 // FP32-storage precision for a stratification: FP32 transforms when the
@@ -316,7 +321,8 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   if (rc) return fail(rc);
   h->coef_size = CoefLayout{L, N}.size();
   std::vector<double> coef_h(h->coef_size);
-  compute_coefs(h, ss, m2z, z2m, coef_h.data());
+  compute_coefs(h, L, ss, m2z, z2m, coef_h.data());
+  h->lay.assign(batch_max, L);
   h->scr_member = (size_t)L * N * N * h->ei.selem;
   // every member slot starts with the creation stratification
   if (hipMalloc(&h->coef, h->coef_size * sizeof(double) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
@@ -351,6 +357,7 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.part = h->part + (size_t)h->part_cap * L * mem0;
   a.part_stride = h->part_cap * L;
   a.psic = h->psic + (size_t)L * mem0;
+  a.psic_stride = L;
   a.status = h->status + mem0;
   a.kb = h->kb + (size_t)(L + L * L) * mem0;
   a.kb_stride = L + L * L;
@@ -364,8 +371,6 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.mode = h->mode;
   a.R = h->R;
   a.lgR = ilog2i(h->R);
-  a.C = h->C;
-  a.lgC = ilog2i(h->C);
   const int rows_threads = (h->R / 2) * e.tpf;
   const int col_groups = (h->C * L + 1) / 2;
   for (int pass = 0; pass < 3; ++pass) {
@@ -373,16 +378,16 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
     int blocks, threads;
     size_t lds;
     if (pass == 1) {
-      blocks = nb * (N / h->C);
+      blocks = pt.cum_c[nb];
       threads = col_groups * e.tpf;
       lds = inv_lds_bytes(e, 1, col_groups);
     } else {
-      blocks = nb * L * (N / h->R);
+      blocks = pt.cum_r[nb];
       threads = rows_threads;
       lds = inv_lds_bytes(e, pass, h->R / 2);
     }
     if (pass == 2 && h->mode == 0) {
-      hipLaunchKernelGGL(k_psic, dim3(nb), dim3(256), 0, s, a, N / h->C, N);
+      hipLaunchKernelGGL(k_psic, dim3(nb), dim3(256), 0, s, a, pt, N);
       hipError_t e2 = hipGetLastError();
       if (e2 != hipSuccess) {
         pq_set_error("pq_inv: launch of the psiC reduction failed: %s", hipGetErrorString(e2));
@@ -411,9 +416,24 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
     const int nb = std::min(kMaxBatchArg, batch - b0);
     PtrTab pt;
     std::memset(&pt, 0, sizeof(pt));
+    // per-member layers, column tiles (the widest power of two up to 8
+    // whose C L_b rows fit the column workgroup's groups) and block ranges
+    const int groups = (h->C * h->L + 1) / 2;
+    pt.nb = nb;
+    pt.cum_r[0] = pt.cum_c[0] = 0;
     for (int b = 0; b < nb; ++b) {
       pt.zeta[b] = zeta[b0 + b];
       pt.psi[b] = psi[b0 + b];
+      const int Lb = h->lay[b0 + b];
+      int Cb = h->C;
+      if (Lb != h->L) {
+        Cb = 8;
+        while (Cb > 1 && ((Cb * Lb + 1) / 2 > groups || h->N % Cb)) Cb /= 2;
+      }
+      pt.lay[b] = (unsigned char)Lb;
+      pt.lgc[b] = (unsigned char)ilog2i(Cb);
+      pt.cum_r[b + 1] = pt.cum_r[b] + Lb * (h->N / h->R);
+      pt.cum_c[b + 1] = pt.cum_c[b] + h->N / Cb;
     }
     pt.ldz1 = ldz1;
     pt.ldz2 = ldz2;
@@ -472,26 +492,35 @@ int pq_inv_get_tiles(pq_inv *h, int *rows, int *cols) {
   return PQ_OK;
 }
 
-int pq_inv_set_member(pq_inv *h, int slot, const double *ss, const double *modes_to_z,
-                      const double *z_to_modes) {
+int pq_inv_set_member_layers(pq_inv *h, int slot, int layers, const double *ss,
+                             const double *modes_to_z, const double *z_to_modes) {
   PQ_CHECK_ARG(h != nullptr, "pq_inv_set_member: null handle");
   PQ_CHECK_ARG(h->mode == 0, "pq_inv_set_member: only for PV inversion handles");
   PQ_CHECK_ARG(slot >= 0 && slot < h->batch_max, "pq_inv_set_member: slot %d outside 0..%d", slot,
                h->batch_max - 1);
+  PQ_CHECK_ARG(layers >= 1 && layers <= h->L, "pq_inv_set_member: layers %d outside 1..%d", layers,
+               h->L);
   PQ_CHECK_ARG(ss && modes_to_z && z_to_modes, "pq_inv_set_member: null stratification");
-  const int L = h->L;
+  const int L = layers;
   PQ_CHECK_ARG(ss[0] == 0.0, "pq_inv_set_member: ss(1) must be 0 (barotropic mode)");
   for (int l = 0; l < L; ++l) {
     PQ_CHECK_ARG(modes_to_z[l] == 1.0, "pq_inv_set_member: modes_to_z(:,1) must be all ones");
     PQ_CHECK_ARG(ss[l] >= 0.0 && std::isfinite(ss[l]), "pq_inv_set_member: ss must be finite and >= 0");
   }
   std::vector<double> c(h->coef_size);
-  compute_coefs(h, ss, modes_to_z, z_to_modes, c.data());
+  compute_coefs(h, L, ss, modes_to_z, z_to_modes, c.data());
   // ordered after the handle's earlier work on its stream
   PQ_HIP(hipMemcpyAsync(h->coef + h->coef_size * slot, c.data(), h->coef_size * sizeof(double),
                         hipMemcpyHostToDevice, h->stream));
   PQ_HIP(hipStreamSynchronize(h->stream));
+  h->lay[slot] = L;
   return PQ_OK;
+}
+
+int pq_inv_set_member(pq_inv *h, int slot, const double *ss, const double *modes_to_z,
+                      const double *z_to_modes) {
+  PQ_CHECK_ARG(h != nullptr, "pq_inv_set_member: null handle");
+  return pq_inv_set_member_layers(h, slot, h->L, ss, modes_to_z, z_to_modes);
 }
 
 int pq_inv_run_dev(pq_inv *h, int batch, const void *const *zeta, long ldz1, long ldz2,

@@ -1,10 +1,11 @@
 """Ensembles of independent members of mixed grid sizes, layer counts and
 stratifications on one GPU (pq_ens_* of the C ABI).
 
-Members of the same size class (grid, layers, leading dimensions) share one
-batched inversion handle with per-member stratifications; the classes run
-concurrently, and the whole inversion -- or one time step of all CABARET
-model members -- can be captured into a HIP graph and replayed.
+Members of the same size class (grid, leading dimensions and transform
+precision) share one ragged batched inversion handle with per-member layer
+counts and stratifications; the classes run concurrently, and the whole
+inversion -- or one time step of all CABARET model members -- can be
+captured into a HIP graph and replayed.
 """
 from __future__ import annotations
 
@@ -97,10 +98,10 @@ class Ensemble:
         return c.value, s.value
 
     def class_shape(self, k: int):
-        """(g, layers, members) of size class k."""
+        """(g, largest layer count, members) of size class k."""
         ms = [i for i in range(len(self.members)) if self.member_class(i)[0] == k]
-        _, g, L = self.members[ms[0]]
-        return g, L, len(ms)
+        g = self.members[ms[0]][1]
+        return g, max(self.members[i][2] for i in ms), len(ms)
 
     def invert(self) -> None:
         check(load().pq_ens_invert(self._h))

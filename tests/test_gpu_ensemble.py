@@ -1,7 +1,9 @@
 """Ensembles of mixed sizes, layer counts and stratifications: the batched,
-class-grouped inversion (eager and replayed from a HIP graph) gives exactly
-what each member's own handle gives, and a graph-replayed ensemble time step
-of CABARET models gives exactly the standalone models' states."""
+class-grouped inversion (eager and replayed from a HIP graph) gives what
+each member's own handle gives (exactly for members with their class's
+largest layer count; to the psiC summation order, which follows the column
+tile, for the others), and a graph-replayed ensemble time step of CABARET
+models gives exactly the standalone models' states."""
 import numpy as np
 import pytest
 
@@ -26,6 +28,16 @@ def _member(g, L, seed, dtype):
     return st, zt, torch.zeros_like(zt)
 
 
+def _same(out, ref, prec, exact):
+    """Ensemble vs alone: bitwise for full-layer members, else within a few
+    roundings of the output type."""
+    import torch
+    if exact:
+        return torch.equal(out, ref)
+    d = float((out.double() - ref.double()).abs().max() / ref.double().abs().max())
+    return d < (1e-13 if prec == "fp64" else 3e-7)
+
+
 def _alone(g, st, prec, zt):
     import torch
     hc = qg.HelmholtzCache(g, st, prec, tiles=None)
@@ -45,11 +57,12 @@ def test_ensemble_inversion_matches_members(dev, prec):
     for st, zt, pt in mem:
         ens.add(st, zt, pt)
     ens.finalize(tiles=None)
-    assert ens.class_count == 4
+    assert ens.class_count == 3  # one ragged class per grid size
+    lmax = {g: max(L for gg, L, _ in MEMBERS if gg == g) for g, _, _ in MEMBERS}
     ens.invert()
     assert ens.status() == 0
     for (g, L, s), (st, zt, pt) in zip(MEMBERS, mem):
-        assert torch.equal(pt, _alone(g, st, prec, zt)), (g, L, s)
+        assert _same(pt, _alone(g, st, prec, zt), prec, L == lmax[g]), (g, L, s)
 
 
 def test_ensemble_graph_replays(dev):
@@ -73,8 +86,9 @@ def test_ensemble_graph_replays(dev):
         ens.invert()
         ens.synchronize()
         assert ens.status() == 0
+        lmax = {g: max(L for gg, L, _ in MEMBERS if gg == g) for g, _, _ in MEMBERS}
         for (g, L, s), (st, zt, pt) in zip(MEMBERS, mem):
-            assert torch.equal(pt, _alone(g, st, "fp32", zt)), (rep, g, L, s)
+            assert _same(pt, _alone(g, st, "fp32", zt), "fp32", L == lmax[g]), (rep, g, L, s)
 
 
 def _model(g, n_layers_seed):
