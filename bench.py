@@ -92,6 +92,69 @@ def ensemble_rates(torch, dev, precision, log, members=64, reps=20):
             "max_rel_psi_err_vs_fp64": err}
 
 
+def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11):
+    """Serving leg: Poisson arrivals (mean `rate` per s over `window` s), 70 %
+    inversions of field members and 30 % `steps`-step advances of CABARET
+    model members, members resident on the device.  Field members: grids
+    {127, 255, 511, 1023, 2047}^2 x 2..6 layers, FP32 storage; model members
+    (FP64): grids {127, 255, 511, 1023}^2 x {2, 4, 6} layers.  Returns
+    latency percentiles per kind and the completed rate."""
+    from pequod_amd import qg, synth
+    from pequod_amd.serve import Server
+
+    srv = Server("fp32", cap_bytes=200e9, max_batch=16)
+    fields, models = [], []
+    keep = []
+    for n in (127, 255, 511, 1023, 2047):
+        for L in range(2, 7):
+            c = synth.steep_member(L, 1.6, n=n, member=100 + 10 * L)
+            zt = torch.from_numpy(np.ascontiguousarray(c.q(member=100 + L).transpose(2, 1, 0)))
+            zt = zt.to(dev, torch.float32)
+            pt = torch.zeros_like(zt)
+            keep += [zt, pt]
+            fields.append(srv.open_field(c.stratification(), zt, pt))
+    for n in (127, 255, 511, 1023):
+        for L in (2, 4, 6):
+            c = synth.steep_member(L, 1.6, n=n, member=200 + L)
+            nd = c.nondim()
+            models.append(srv.open_model(c.g, nd["strat"], nd["beta"], c.wind(), c.q(member=200 + L),
+                                         viscosity=nd["viscosity"], visc_bot=nd["visc_bot"],
+                                         dt=nd["dt"], bc_type=1, alpha=nd["alpha"]))
+    # warm-up: one request per member; statistics start after it
+    for m in fields:
+        srv.submit(m, Server.FIELD)
+    for m in models:
+        first = srv.submit(m, Server.MODEL, 1)
+    srv.drain(600)
+    first += 1
+    used = srv.used_bytes
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * window * 1.5)))
+    arrivals = arrivals[arrivals < window]
+    kinds = rng.random(arrivals.size) < 0.7
+    targets = [int(rng.integers(len(fields))) if k else int(rng.integers(len(models))) for k in kinds]
+    log(f"[bench] serving: {arrivals.size} requests over {window} s ({len(fields)} field members, "
+        f"{len(models)} model members)")
+    t0 = time.perf_counter()
+    for a, inv, tgt in zip(arrivals, kinds, targets):
+        while time.perf_counter() - t0 < a:
+            pass
+        if inv:
+            srv.submit(fields[tgt], Server.FIELD)
+        else:
+            srv.submit(models[tgt], Server.MODEL, steps)
+    srv.drain(1200)
+    makespan = time.perf_counter() - t0
+    out = {"offered_per_s": rate, "window_s": window, "requests": int(arrivals.size),
+           "served_per_s": round(arrivals.size / makespan, 1), "hbm_used_gb": round(used / 1e9, 2)}
+    for name, kind in (("inversion", Server.FIELD), ("advance_50", Server.MODEL), ("all", -1)):
+        st = srv.stats(kind, first)
+        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()
+                     if k in ("count", "p50_ms", "p95_ms", "p99_ms", "max_ms", "failed")}
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +173,10 @@ def main():
     ap.add_argument("--no-cabaret", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-member mixed-size ensemble leg")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving leg (Poisson request stream, continuous batching)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg: requests per second")
+    ap.add_argument("--serve-window", type=float, default=1.0, help="serving leg: arrival window [s]")
     ap.add_argument("--no-latency", action="store_true",
                     help="skip the B=1 latency leg (every inversion launch then has the bench's batch)")
     ap.add_argument("--sweep-tiles", action="store_true",
@@ -288,6 +355,10 @@ def main():
     if not args.no_ensemble:
         ens = ensemble_rates(torch, dev, args.ensemble_precision, log)
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_rates(torch, dev, log, rate=args.serve_rate, window=args.serve_window)
+
     # CPU baseline (oracle, FP64, one core) on a bounded sample, and the
     # error of the measured path against it on the same input.
     cpu = None
@@ -338,6 +409,7 @@ def main():
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
         "ensemble_64_mixed": ens,
+        "serving": serving,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic},

@@ -255,9 +255,10 @@ void *pq_model_stream(pq_model *m);
 /* ======================================================================
  * Ensembles (no counterpart in the reference, which runs one member per
  * MPI job): independent members of mixed grid sizes, layer counts and
- * stratifications on one GPU.  Members with the same grid, layer count and
- * leading dimensions share a batched inversion handle (a size class) with
- * per-member stratifications; classes run concurrently.  The inversion of
+ * stratifications on one GPU.  Members with the same grid, leading
+ * dimensions and transform precision share a ragged batched inversion
+ * handle (a size class) with per-member layer counts and stratifications;
+ * classes run concurrently.  The inversion of
  * all members, or one time step of all CABARET model members, can be
  * captured into a HIP graph and replayed.
  * ==================================================================== */
@@ -295,6 +296,48 @@ int pq_ens_synchronize(pq_ens *e);
 /* Waits and returns the worst status of the last inversions.              */
 int pq_ens_status(pq_ens *e);
 int pq_ens_destroy(pq_ens *e);
+
+/* ======================================================================
+ * Serving (no counterpart in the reference): members resident in HBM and a
+ * stream of requests -- the PV inversion of a field member, or n CABARET
+ * steps of a model member -- served with continuous batching.  A scheduler
+ * thread launches every waiting inversion of a grid size as one ragged batch
+ * whenever one of the size's handles is free, and a model member's advance
+ * as replays of its captured one-step graph on its own stream; requests
+ * that arrive meanwhile join the next launches.  Members are admitted by
+ * device footprint: an open that would take the reserved bytes above the
+ * cap fails with PQ_ERR_NOMEM.  Requests of one member run in order.
+ * ==================================================================== */
+typedef struct pq_srv pq_srv;
+/* precision: PQ_PREC_FP32, _FP32_STORAGE, _MIXED or _FP64 for field
+ * members (models are FP64); max_batch: members per inversion batch.      */
+int pq_srv_create(pq_srv **s, int precision, size_t hbm_cap_bytes, int max_batch);
+/* Field member: device g x g x layers views zeta / psi (types as for
+ * pq_inv_run_dev; the caller keeps them allocated until pq_srv_close) with
+ * its own stratification.                                                 */
+int pq_srv_open_field(pq_srv *s, int g, int layers, const double *ss,
+                      const double *modes_to_z, const double *z_to_modes,
+                      const void *zeta, long ldz1, long ldz2, void *psi,
+                      long ldp1, long ldp2, int *member);
+/* Model member (fixed dt): created, started from zeta (as pq_model_init)
+ * and owned by the engine.                                                */
+int pq_srv_open_model(pq_srv *s, const pq_model_params *p,
+                      const double *zeta_sdata_host, int *member);
+pq_model *pq_srv_model(pq_srv *s, int member);  /* e.g. for pq_model_get */
+/* kind 0: inversion of a field member; 1: `steps` steps of a model member. */
+int pq_srv_submit(pq_srv *s, int member, int kind, int steps, long long *request);
+/* 1 done (latency_s = completion - submission), 0 pending, < 0 failed.    */
+int pq_srv_poll(pq_srv *s, long long request, double *latency_s);
+/* Waits for every submitted request (at most timeout_s seconds).          */
+int pq_srv_drain(pq_srv *s, double timeout_s);
+/* Completed requests of kind (-1 all) numbered `first` or later: count,
+ * mean, p50, p95, p99, max latency [s], completions per second (first
+ * submission to last completion), failures.                               */
+int pq_srv_stats(pq_srv *s, int kind, long long first, double *out8);
+size_t pq_srv_used_bytes(pq_srv *s);
+/* Closes an idle member and releases its footprint.                       */
+int pq_srv_close(pq_srv *s, int member);
+int pq_srv_destroy(pq_srv *s);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,18 @@ SIGNATURES = {
     "pq_inv_set_tiles": (_i, [_vp, _i, _i]),
     "pq_inv_set_member": (_i, [_vp, _i, _dp, _dp, _dp]),
     "pq_inv_set_member_layers": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
+    "pq_srv_create": (_i, [ctypes.POINTER(_vp), _i, _sz, _i]),
+    "pq_srv_open_field": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp, _l, _l, _vp, _l, _l,
+                               ctypes.POINTER(_i)]),
+    "pq_srv_open_model": (_i, [_vp, _vp, _dp, ctypes.POINTER(_i)]),
+    "pq_srv_model": (_vp, [_vp, _i]),
+    "pq_srv_submit": (_i, [_vp, _i, _i, _i, ctypes.POINTER(ctypes.c_longlong)]),
+    "pq_srv_poll": (_i, [_vp, ctypes.c_longlong, _dp]),
+    "pq_srv_drain": (_i, [_vp, _d]),
+    "pq_srv_stats": (_i, [_vp, _i, ctypes.c_longlong, _dp]),
+    "pq_srv_used_bytes": (_sz, [_vp]),
+    "pq_srv_close": (_i, [_vp, _i]),
+    "pq_srv_destroy": (_i, [_vp]),
     "pq_inv_get_tiles": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "pq_inv_pass_times": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _i,
                                ctypes.POINTER(ctypes.c_float)]),

@@ -24,6 +24,7 @@ SOURCES = [
     "pq_inversion.hip",
     "pq_cabaret.hip",
     "pq_ensemble.hip",
+    "pq_serve.cpp",
 ]
 
 # Transform lengths N = g - 1 with a kernel unit each (pq_inv_kernels.hip

@@ -14,6 +14,7 @@
 #pragma clang fp contract(off)
 
 #include "pq_common.h"
+#include "pq_internal.h"
 
 #include <cmath>
 #include <cstring>
@@ -732,6 +733,7 @@ int pq_model_step(pq_model *m, int nsteps, int final_partial) {
 namespace pq {
 int model_step_pre(pq_model *m, hipStream_t s) { return model_pre(m, s); }
 int model_step_post(pq_model *m, hipStream_t s, bool last) { return model_post(m, s, last); }
+int model_step_invert(pq_model *m) { return model_invert(m); }
 hipStream_t model_stream(pq_model *m) { return m->stream; }
 bool model_adaptive(const pq_model *m) { return m->adapt != 0; }
 void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld) {
@@ -742,6 +744,12 @@ void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_
   *psi_sdata = m->psi + 1 + (m->g + 2);
 }
 double model_time(const pq_model *m) { return m->model_time; }
+size_t model_device_bytes(const pq_model *m) {
+  const Dims &d = m->d;
+  return sizeof(double) * (4 * d.n_nd() + 5 * d.n_fi() + 5 * d.n_fj() + 2 * d.n_rs() +
+                           (size_t)m->g * m->g + 3 * (size_t)m->L) +
+         inv_device_bytes(m->inv);
+}
 void model_set_time(pq_model *m, double t) { m->model_time = t; }
 // the same two half-step additions as model_post, so times round alike
 void model_account_step(pq_model *m) {

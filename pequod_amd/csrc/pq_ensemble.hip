@@ -14,6 +14,7 @@
 // captured once into a HIP graph and replayed, which removes the per-kernel
 // launch cost of the O(100) small launches of a mixed ensemble.
 #include "pq_common.h"
+#include "pq_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -22,16 +23,6 @@
 #include <tuple>
 #include <vector>
 
-namespace pq {
-int model_step_pre(pq_model *m, hipStream_t s);
-int model_step_post(pq_model *m, hipStream_t s, bool last);
-hipStream_t model_stream(pq_model *m);
-bool model_adaptive(const pq_model *m);
-void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld);
-double model_time(const pq_model *m);
-void model_set_time(pq_model *m, double t);
-void model_account_step(pq_model *m);
-}  // namespace pq
 
 using namespace pq;
 

@@ -30,11 +30,13 @@ struct CoefLayout {
 // layers than the handle (ragged batches): member b has lay[b] layers, its
 // column-pass tiles are 2^lgc[b] columns wide, and its workgroups are the
 // blocks cum_r[b] .. cum_r[b+1]-1 of the row passes and cum_c[b] ..
-// cum_c[b+1]-1 of the column pass.
+// cum_c[b+1]-1 of the column pass.  coef[b] is the member's coefficient
+// block, for its layer count.
 constexpr int kMaxBatchArg = 64;
 struct PtrTab {
   const void *zeta[kMaxBatchArg];
   void *psi[kMaxBatchArg];
+  const double *coef[kMaxBatchArg];  // coefficient block (CoefLayout) of member b
   long long ldz1, ldz2, ldp1, ldp2;
   int nb;
   int cum_r[kMaxBatchArg + 1];
@@ -68,8 +70,6 @@ struct InvArgs {
   double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m
   int kb_stride;
   int *status;       // sticky per member: 1 NaN psiC, 2 |psiC| > 1e100
-  const double *coef;  // coefficient blocks (CoefLayout): member b's at coef + b * coef_stride
-  long long coef_stride;
   const void *tw64;  // cx<double>[M]: exp(-2 pi i n / M), M the engine's FFT length
   const void *was;   // Tc[N/2]: 2 sin(k pi / N) (DstSint), Tc the arithmetic type
   const void *lam4;  // Tc[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))

@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
   Tc *cM = cD + 64;
   Tc *cS = cM + 64;
   const CoefLayout CL{L, N};
-  const double *coef = a.coef + b * a.coef_stride;
+  const double *coef = ptrs.coef[b];
   // coefficients: loads issued now, stored to LDS after the forward
   // transform.  Thread i < 64 takes entry (r, c) = (i / 8, i % 8) of both
   // matrices (zero outside L x L): cD[i] = D(r,c), cM[i] = M(r,c); in the
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
 
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
   if (a.mode == 0 && L > 1) {
-    const double *tab = a.coef + b * a.coef_stride + CL.tab();
+    const double *tab = ptrs.coef[b] + CL.tab();
     for (int h = threadIdx.x; h < N / 2; h += nt) {
       const int p = 2 * h + 1;
       double acc[16];

@@ -12,6 +12,7 @@
 // j = jb R .. jb R + R - 1 transposed (element (p, j) at
 // l N^2 + jb N R + p R + j % R).  The row passes read and write whole blocks
 // contiguously; the column pass moves C R contiguous elements per block.
+#include "pq_internal.h"
 #include "pq_inv.h"
 
 #include <hip/hip_runtime.h>
@@ -94,7 +95,7 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
   }
   __syncthreads();
   const CoefLayout CL{L, N};
-  const double *coef = a.coef + b * a.coef_stride;
+  const double *coef = ptrs.coef[b];
   const double *M = coef + CL.M(), *ca = coef + CL.a(), *cb = coef + CL.b();
   double *kb = a.kb + (size_t)b * a.kb_stride;
   if (tid < L) {
@@ -361,8 +362,6 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.status = h->status + mem0;
   a.kb = h->kb + (size_t)(L + L * L) * mem0;
   a.kb_stride = L + L * L;
-  a.coef = h->coef + h->coef_size * mem0;
-  a.coef_stride = (long long)h->coef_size;
   a.tw64 = h->tw64;
   a.was = h->was;
   a.lam4 = h->lam4;
@@ -404,14 +403,21 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   return PQ_OK;
 }
 
+// coef / layers: per-member coefficient blocks and layer counts (NULL: the
+// handle's member slots).
 static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, long ldz2,
-                      void *const *psi, long ldp1, long ldp2, hipStream_t s, int pass_mask = 7) {
+                      void *const *psi, long ldp1, long ldp2, hipStream_t s, int pass_mask = 7,
+                      const double *const *coef = nullptr, const int *layers = nullptr) {
   PQ_CHECK_ARG(batch >= 1 && batch <= h->batch_max, "pq_inv_run_dev: batch %d outside 1..%d", batch,
                h->batch_max);
   PQ_CHECK_ARG(ldz1 >= h->g && ldz2 >= h->g && ldp1 >= h->g && ldp2 >= h->g,
                "pq_inv_run_dev: leading dimensions smaller than g=%d", h->g);
-  for (int b = 0; b < batch; ++b)
+  for (int b = 0; b < batch; ++b) {
     PQ_CHECK_ARG(zeta[b] != nullptr && psi[b] != nullptr, "pq_inv_run_dev: null member pointer %d", b);
+    PQ_CHECK_ARG(!layers || (layers[b] >= 1 && layers[b] <= h->L),
+                 "pq_inv_run_dev: member %d has %d layers, outside 1..%d", b, layers ? layers[b] : 0, h->L);
+    PQ_CHECK_ARG(!coef || coef[b], "pq_inv_run_dev: null coefficient block of member %d", b);
+  }
   for (int b0 = 0; b0 < batch; b0 += kMaxBatchArg) {
     const int nb = std::min(kMaxBatchArg, batch - b0);
     PtrTab pt;
@@ -424,7 +430,8 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
     for (int b = 0; b < nb; ++b) {
       pt.zeta[b] = zeta[b0 + b];
       pt.psi[b] = psi[b0 + b];
-      const int Lb = h->lay[b0 + b];
+      pt.coef[b] = coef ? coef[b0 + b] : h->coef + h->coef_size * (size_t)(b0 + b);
+      const int Lb = layers ? layers[b0 + b] : h->lay[b0 + b];
       int Cb = h->C;
       if (Lb != h->L) {
         Cb = 8;
@@ -444,6 +451,34 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
   }
   return PQ_OK;
 }
+
+namespace pq {
+size_t inv_coef_size(const pq_inv *h) { return h->coef_size; }
+
+int inv_make_coefs(const pq_inv *h, int layers, const double *ss, const double *m2z,
+                   const double *z2m, double *dev_block) {
+  PQ_CHECK_ARG(h && dev_block && ss, "inv_make_coefs: null argument");
+  PQ_CHECK_ARG(layers >= 1 && layers <= h->L, "inv_make_coefs: layers %d outside 1..%d", layers, h->L);
+  std::vector<double> c(h->coef_size);
+  compute_coefs(h, layers, ss, m2z, z2m, c.data());
+  PQ_HIP(hipMemcpy(dev_block, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+int inv_run_members(pq_inv *h, int batch, const void *const *zeta, long ldz1, long ldz2,
+                    void *const *psi, long ldp1, long ldp2, const double *const *coef,
+                    const int *layers, hipStream_t s) {
+  PQ_CHECK_ARG(h && coef && layers, "inv_run_members: null argument");
+  return inv_launch(h, batch, zeta, ldz1, ldz2, psi, ldp1, ldp2, s, 7, coef, layers);
+}
+
+size_t inv_device_bytes(const pq_inv *h) {
+  const size_t N = (size_t)h->N, L = (size_t)h->L, B = (size_t)h->batch_max;
+  return h->scr_member * B + sizeof(double) * (h->coef_size * B + (size_t)h->part_cap * L * B +
+                                               L * B + (L + L * L) * B + 2 * (size_t)h->ei.M + N) +
+         (size_t)h->ei.celem * (N / 2 + N) + sizeof(int) * B;
+}
+}  // namespace pq
 
 extern "C" {
 

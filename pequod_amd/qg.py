@@ -274,25 +274,42 @@ FIELDS = {"psi": 0, "zeta": 1, "rel": 2, "zeta_old": 3, "ui": 4, "uj": 5, "zeta_
           "flux_j": 13}
 
 
+def field_shape(name: str, g: int, L: int):
+    """Shape of a model field (the reference's array shapes)."""
+    if name in ("psi", "zeta", "rel", "zeta_old"):
+        return (g + 2, g + 2, L)
+    if name in ("ui", "zeta_fi", "uimo", "flux_i"):
+        return (g + 3, g + 2, L)
+    if name in ("uj", "zeta_fj", "ujmo", "flux_j"):
+        return (g + 2, g + 3, L)
+    return (g, g, L)
+
+
+def model_params(g: int, strat: Stratification, beta, force_wind: np.ndarray, *,
+                 viscosity: float, visc_bot: float, dt: float, bc_type: int = BC_PARTIAL_SLIP,
+                 alpha: float = 0.0, adapt_dt: bool = False, target_cfl: float = 0.1,
+                 max_dt: float = 0.0, max_dt_increase: float = 1.5, precision: str = "fp64"):
+    """pq_model_params for a model; returns (params, arrays it points into),
+    the arrays to be kept alive while the params are used."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (strat.s1, strat.s2, strat.ss)]
+    keep += [np.asfortranarray(strat.modes_to_z), np.asfortranarray(strat.z_to_modes),
+             np.ascontiguousarray(beta, dtype=np.float64),
+             np.asfortranarray(force_wind, dtype=np.float64)]
+    s1, s2, ss, m2z, z2m, b, fw = keep
+    p = _lib.PqModelParams(g, strat.layers, _np_ptr(s1), _np_ptr(s2), _np_ptr(ss), _np_ptr(m2z),
+                           _np_ptr(z2m), _np_ptr(b), _np_ptr(fw), viscosity, visc_bot, bc_type,
+                           alpha, int(adapt_dt), target_cfl, max_dt, max_dt_increase, dt,
+                           PREC[precision])
+    return p, keep
+
+
 class Model:
     """Device-resident CABARET model: one time step in the order of the
     reference's time loop (qg3.F90:590-967)."""
 
-    def __init__(self, g: int, strat: Stratification, beta, force_wind: np.ndarray, *,
-                 viscosity: float, visc_bot: float, dt: float, bc_type: int = BC_PARTIAL_SLIP,
-                 alpha: float = 0.0, adapt_dt: bool = False, target_cfl: float = 0.1,
-                 max_dt: float = 0.0, max_dt_increase: float = 1.5, precision: str = "fp64"):
+    def __init__(self, g: int, strat: Stratification, beta, force_wind: np.ndarray, **kw):
         self.g, self.layers = g, strat.layers
-        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (
-            strat.s1, strat.s2, strat.ss)]
-        self._keep += [np.asfortranarray(strat.modes_to_z), np.asfortranarray(strat.z_to_modes),
-                       np.ascontiguousarray(beta, dtype=np.float64),
-                       np.asfortranarray(force_wind, dtype=np.float64)]
-        s1, s2, ss, m2z, z2m, b, fw = self._keep
-        p = _lib.PqModelParams(g, strat.layers, _np_ptr(s1), _np_ptr(s2), _np_ptr(ss), _np_ptr(m2z),
-                               _np_ptr(z2m), _np_ptr(b), _np_ptr(fw), viscosity, visc_bot, bc_type,
-                               alpha, int(adapt_dt), target_cfl, max_dt, max_dt_increase, dt,
-                               PREC[precision])
+        p, self._keep = model_params(g, strat, beta, force_wind, **kw)
         h = _vp()
         check(load().pq_model_create(ctypes.byref(h), ctypes.byref(p)))
         self._h = h
@@ -305,14 +322,7 @@ class Model:
         check(load().pq_model_step(self._h, nsteps, int(final_partial)))
 
     def shape(self, name: str):
-        g, L = self.g, self.layers
-        if name in ("psi", "zeta", "rel", "zeta_old"):
-            return (g + 2, g + 2, L)
-        if name in ("ui", "zeta_fi", "uimo", "flux_i"):
-            return (g + 3, g + 2, L)
-        if name in ("uj", "zeta_fj", "ujmo", "flux_j"):
-            return (g + 2, g + 3, L)
-        return (g, g, L)
+        return field_shape(name, self.g, self.layers)
 
     def get(self, name: str) -> np.ndarray:
         a = np.zeros(self.shape(name), order="F")
