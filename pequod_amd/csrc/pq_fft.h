@@ -458,6 +458,19 @@ template <typename T_, int N_, int EF_ = 0> struct DstSint {
                                              Loader ld) {
     run(region, t, scan_tmp, cx_, ld, RowSink<T>{region, region + RS});
   }
+  // Inputs already in the region's rows.
+  template <typename Sink>
+  __device__ __forceinline__ static void run_rows(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                                  Sink st) {
+    const T *ra = region, *rb = region + RS;
+    run(region, t, scan_tmp, cx_, [&](int, int kp, T &xa, T &xb) {
+      xa = ra[kp];
+      xb = rb[kp];
+    }, st);
+  }
+  __device__ __forceinline__ static void run_rows(T *region, int t, T *scan_tmp, const Ctx &cx_) {
+    run_rows(region, t, scan_tmp, cx_, RowSink<T>{region, region + RS});
+  }
 
   // The transform of loaded inputs; the group's region may still be read by
   // the loader until this call (it synchronises the group first).
@@ -622,6 +635,19 @@ template <typename T_, int N_, typename Tc_ = T_, int LOW_ = 0> struct DstExt {
   __device__ __forceinline__ static void run(T *region, int t, T *scan_tmp, const Ctx &cx_,
                                              Loader ld) {
     run(region, t, scan_tmp, cx_, ld, RowSink<T>{region, region + RS});
+  }
+  // Inputs already in the region's rows.
+  template <typename Sink>
+  __device__ __forceinline__ static void run_rows(T *region, int t, T *scan_tmp, const Ctx &cx_,
+                                                  Sink st) {
+    const T *ra = region, *rb = region + RS;
+    run(region, t, scan_tmp, cx_, [&](int, int kp, T &xa, T &xb) {
+      xa = ra[kp];
+      xb = rb[kp];
+    }, st);
+  }
+  __device__ __forceinline__ static void run_rows(T *region, int t, T *scan_tmp, const Ctx &cx_) {
+    run_rows(region, t, scan_tmp, cx_, RowSink<T>{region, region + RS});
   }
 
   // The transform of loaded inputs; the group's region may still be read by

@@ -372,11 +372,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
     typename Eng::Inv::Ctx ctxi;
     ctxi.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
               Eng::M / Eng::Inv::M);
-    const T *rowa = region, *rowb = region + Eng::RS;
-    Eng::Inv::run(region, t, scan_tmp, ctxi, [&](int, int kp, T &xa, T &xb) {
-      xa = rowa[kp];
-      xb = rowb[kp];
-    });
+    Eng::Inv::run_rows(region, t, scan_tmp, ctxi);
   }
   __syncthreads();
   // Write back: per layer and block, C*R contiguous elements.
@@ -397,8 +393,14 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
 
 // ------------------------------------------------------------------ K3
 // Grid: sum over members of L_b * (N / R) workgroups of (R / 2) * TPF threads.
+// At least 3 waves per SIMD (at most 168 VGPRs) up to N = 2048: without
+// the hint the correction loop's FP64 temporaries take the FP32 kernel to
+// 182 VGPRs and 2 waves per SIMD (-6 % time at 1023^2 x 3 x 8).  Longer
+// transforms would spill under it.
+constexpr int kRowsInvWaves = PQ_N <= 2048 ? 3 : 1;
 template <typename Eng, typename Tout, typename Ts>
-__global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(kRowsInvWaves)))
+k_rows_inv(InvArgs a, PtrTab ptrs) {
   using T = typename Eng::T;
   using Tc = typename Eng::Tc;
   using VT = typename V16<Ts>::type;
@@ -520,15 +522,11 @@ __global__ void __launch_bounds__(512) k_rows_inv(InvArgs a, PtrTab ptrs) {
   };
   {
     const T *rowa = region, *rowb = region + Eng::RS;
-    auto ld = [&](int, int kp, T &xa, T &xb) {
-      xa = rowa[kp];
-      xb = rowb[kp];
-    };
     if constexpr (Eng::Inv::kLaneOut) {
       // outputs leave the last FFT stage with consecutive k across lanes
-      Eng::Inv::run(region, t, scan_tmp, ctx, ld, sink);
+      Eng::Inv::run_rows(region, t, scan_tmp, ctx, sink);
     } else {
-      Eng::Inv::run(region, t, scan_tmp, ctx, ld);
+      Eng::Inv::run_rows(region, t, scan_tmp, ctx);
       group_sync<TPF>();
       for (int k = t; k < N; k += TPF) sink(k, 0, Tc(rowa[pad16(k)]), Tc(rowb[pad16(k)]));
     }
