@@ -251,6 +251,49 @@ int pq_model_set(pq_model *m, int which, const double *host, size_t count);
 /* dt, tweight_a, tweight_u, model_time */
 int pq_model_scalars(pq_model *m, double *out4);
 void *pq_model_stream(pq_model *m);
+/* Time mean of psi (qg3.F90:503, 672-681): with enable = 1 the sum and the
+ * averaging time restart from zero, and every later step whose model time
+ * (after the inversion) exceeds max_spin adds dt * psi over all G x G
+ * nodes on the device; enable = 0 stops adding.  Synchronous.             */
+int pq_model_average(pq_model *m, int enable, double max_spin);
+/* The time mean psi_av / diagnostic_time as written to the _s_av output
+ * (qg3.F90:886-898): (G, G, L) into host; diag_time (may be NULL) receives
+ * the averaging time.  PQ_ERR_ARG before the first sample.                */
+int pq_model_psi_av(pq_model *m, double *host, size_t count, double *diag_time);
+/* energy(strat, psi, ekin, epot) of the model's current psi
+ * (qg3p_diagnostics.F90:778-863, qg3.F90:744) with layer depths h[L] (the
+ * model's s2 is the stratification's); FP64 reduction on the device, ekin[L]
+ * and epot on the host.  PQ_ERR_NAN when a result is NaN.                 */
+int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot);
+
+/* ======================================================================
+ * Diagnostics of a stream function field: replaces energy(strat, psi, ekin,
+ * epot) (qg3p_diagnostics.F90:778-863) for the closed basin on one domain,
+ * and the time-mean accumulation of the driver (qg3.F90:672-681).
+ * Kinetic energy per layer (cell-centred squared gradient, weighted by
+ * h/H / 2) and potential energy of the interfaces (trapezoid-weighted
+ * s2(k) h(k) / (2H) (psi_k - psi_k+1)^2), both divided by (G-1)^2.
+ * FP64, reduced on the device in a fixed order (deterministic).
+ * ==================================================================== */
+typedef struct pq_diag pq_diag;
+/* h[L] layer depths (any unit: only h / sum(h) enters), s2[L] the
+ * stratification's s2 (qg3p_diagnostics.F90:25-47); layers 1..8.          */
+int pq_diag_create(pq_diag **d, int g, int layers, const double *h, const double *s2);
+int pq_diag_destroy(pq_diag *d);
+void *pq_diag_stream(pq_diag *d);
+/* Host drop-in: psi a g x g x L view (FP64; a registered mirror is used in
+ * place).  ekin[L], *epot as the reference's; synchronous; PQ_ERR_NAN when
+ * a result is NaN (the reference aborts).                                 */
+int pq_diag_energy(pq_diag *d, const double *psi, int ld1, int ld2, double *ekin, double *epot);
+/* Device field psi (g x g x L FP64 view); out_dev[L + 1] receives
+ * ekin(1..L), epot on the device.  Asynchronous on stream; allocates
+ * nothing, so it may be captured into a HIP graph.                        */
+int pq_diag_energy_dev(pq_diag *d, const void *psi, long ld1, long ld2, double *out_dev,
+                       void *stream);
+/* psi_av += dt * psi over every node of g x g x L device views (FP64);
+ * asynchronous on stream (NULL: the null stream).                         */
+int pq_psi_av_add_dev(int g, int layers, double dt, const void *psi, long ld1, long ld2,
+                      double *psi_av, long la1, long la2, void *stream);
 
 /* ======================================================================
  * Ensembles (no counterpart in the reference, which runs one member per

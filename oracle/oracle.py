@@ -46,6 +46,8 @@ def lib():
             "pqo_apply_dirichlet_bc": (None, [i, i, d, _dp, i, i]),
             "pqo_add_laplace_bc_rhs": (None, [i, i, d, _dp, i, i]),
             "pqo_mean": (d, [i, _dp, i]),
+            "pqo_energy": (None, [i, i, i, _dp, _dp, _dp, _dp, _dp]),
+            "pqo_psi_av_add": (None, [i, i, i, d, _dp, _dp]),
             "pqo_strat_modes": (i, [i, _dp, _dp, _dp, _dp, _dp]),
             "pqo_strat_l2": (None, [d, d, d, _dp, _dp]),
             "pqo_strat_l3": (None, [d, d, d, d, d, _dp, _dp, _dp, _dp]),
@@ -134,6 +136,26 @@ def add_laplace_bc_rhs(rhs: np.ndarray, bc: float):
 def mean(psi: np.ndarray) -> float:
     psi = f64(psi)
     return lib().pqo_mean(psi.shape[0], _p(psi), psi.shape[0])
+
+
+def energy(psi_data: np.ndarray, h, s2):
+    """energy (qg3p_diagnostics.F90:778-863) of a data array (gi+2, gj+2, L)
+    with halo; returns (ekin[L], epot)."""
+    psi = f64(psi_data)
+    gi, gj, L = psi.shape[0] - 2, psi.shape[1] - 2, psi.shape[2]
+    h, s2 = f64(h), f64(np.resize(np.asarray(s2, dtype=np.float64), L))
+    ekin = np.zeros(L)
+    epot = ctypes.c_double()
+    lib().pqo_energy(gi, gj, L, _p(h), _p(s2), _p(psi), _p(ekin), ctypes.byref(epot))
+    return ekin, epot.value
+
+
+def psi_av_add(psi_data: np.ndarray, psi_av: np.ndarray, dt: float) -> None:
+    """psi_av (gi, gj, L) += dt * psi over all nodes (qg3.F90:672-681)."""
+    psi = f64(psi_data)
+    assert psi_av.flags.f_contiguous and psi_av.dtype == np.float64
+    gi, gj, L = psi_av.shape
+    lib().pqo_psi_av_add(gi, gj, L, dt, _p(psi), _p(psi_av))
 
 
 def strat_modes(s1, s2):
@@ -282,7 +304,8 @@ class State(ctypes.Structure):
     _fields_ = [("g", ctypes.c_int), ("layers", ctypes.c_int)] + [
         (n, _dp) for n in ("psi", "rel", "zeta", "zeta_old", "zeta_fi", "ui", "uimo", "flux_i",
                            "zeta_fj", "uj", "ujmo", "flux_j", "res", "res_beta", "force_wind")] + [
-        (n, ctypes.c_double) for n in ("dt", "tweight_a", "tweight_u", "model_time")]
+        (n, ctypes.c_double) for n in ("dt", "tweight_a", "tweight_u", "model_time")] + [
+        ("psi_av", _dp), ("max_spin", ctypes.c_double), ("diag_time", ctypes.c_double)]
 
 
 class Params(ctypes.Structure):
@@ -354,3 +377,18 @@ class Model:
     def scalars(self):
         return dict(dt=self.st.dt, tweight_a=self.st.tweight_a, tweight_u=self.st.tweight_u,
                     model_time=self.st.model_time)
+
+    def average(self, max_spin=0.0):
+        """Start the time mean of psi (qg3.F90:503, 672-681)."""
+        g, L = self.g, self.layers
+        self.a["psi_av"] = np.zeros((g, g, L), order="F")
+        self.st.psi_av = _p(self.a["psi_av"])
+        self.st.max_spin = max_spin
+        self.st.diag_time = 0.0
+
+    def psi_av(self):
+        """psi_av / diagnostic_time (qg3.F90:886-898) and diagnostic_time."""
+        return self.a["psi_av"] * (1.0 / self.st.diag_time), self.st.diag_time
+
+    def energy(self, h):
+        return energy(self.a["psi"], h, self.s2)

@@ -1029,6 +1029,62 @@ void pqo_wind_field(int g, double tau, double wind_shift, double wind_tilt,
     }
 }
 
+/* energy (qg3p_diagnostics.F90:778-863) on one domain with every neighbour
+ * absent (p_a = p_b = p_c = p_d = -1): l_i = 2..ii-1 in sdata, the cell
+ * loop over data indices f_i_min..fb_i_max = 3..ii+1 (qg3p_base.F90:855-941);
+ * allsum is the identity.  Statement and loop order as the reference.      */
+void pqo_energy(int gi, int gj, int L, const double *h, const double *s2,
+                const double *psi, double *ekin, double *epot) {
+#define PD(i, j, k) psi[((i)-1) + (size_t)(gi + 2) * (((j)-1) + (size_t)(gj + 2) * (k))]
+#define PS(i, j, k) PD((i) + 1, (j) + 1, k)
+#define SQ(x) ((x) * (x))
+  double h_total = 0.0;
+  for (int k = 0; k < L; ++k) h_total += h[k];
+  double fac[64];
+  for (int k = 0; k < L - 1; ++k) fac[k] = 0.5 * s2[k] * h[k] / h_total;
+  double ep = 0.0;
+  for (int k = 0; k < L - 1; ++k) {
+    for (int j = 2; j <= gj - 1; ++j)
+      for (int i = 2; i <= gi - 1; ++i) ep = ep + fac[k] * SQ(PS(i, j, k) - PS(i, j, k + 1));
+    /* left boundary, its corners */
+    for (int i = 2; i <= gi - 1; ++i) ep = ep + 0.5 * fac[k] * SQ(PS(i, 1, k) - PS(i, 1, k + 1));
+    ep = ep + 0.25 * fac[k] * SQ(PS(1, 1, k) - PS(1, 1, k + 1));
+    ep = ep + 0.25 * fac[k] * SQ(PS(gi, 1, k) - PS(gi, 1, k + 1));
+    /* right boundary, its corners */
+    for (int i = 2; i <= gi - 1; ++i) ep = ep + 0.5 * fac[k] * SQ(PS(i, gj, k) - PS(i, gj, k + 1));
+    ep = ep + 0.25 * fac[k] * SQ(PS(1, gj, k) - PS(1, gj, k + 1));
+    ep = ep + 0.25 * fac[k] * SQ(PS(gi, gj, k) - PS(gi, gj, k + 1));
+    /* top and bottom boundaries */
+    for (int j = 2; j <= gj - 1; ++j) ep = ep + 0.5 * fac[k] * SQ(PS(1, j, k) - PS(1, j, k + 1));
+    for (int j = 2; j <= gj - 1; ++j) ep = ep + 0.5 * fac[k] * SQ(PS(gi, j, k) - PS(gi, j, k + 1));
+  }
+  for (int k = 0; k < L; ++k) {
+    double e = 0.0;
+    for (int j = 3; j <= gj + 1; ++j)
+      for (int i = 3; i <= gi + 1; ++i)
+        e = e + 0.25 * (SQ(PD(i - 1, j, k) + PD(i, j, k) - PD(i - 1, j - 1, k) - PD(i, j - 1, k)) +
+                        SQ(PD(i, j - 1, k) + PD(i, j, k) - PD(i - 1, j - 1, k) - PD(i - 1, j, k)));
+    ekin[k] = e;
+  }
+  const double cff = 1.0 / (double)((gi - 1) * (gj - 1));
+  *epot = cff * ep;
+  for (int k = 0; k < L; ++k) ekin[k] = 0.5 * cff * (h[k] / h_total) * ekin[k];
+#undef SQ
+#undef PS
+#undef PD
+}
+
+/* the time-mean accumulation of the driver (qg3.F90:672-681) */
+void pqo_psi_av_add(int gi, int gj, int L, double dt, const double *psi, double *psi_av) {
+  for (int l = 0; l < L; ++l)
+    for (int j = 1; j <= gj; ++j)
+      for (int i = 1; i <= gi; ++i) {
+        double *o = &psi_av[(i - 1) + (size_t)gi * ((j - 1) + (size_t)gj * l)];
+        const double p = psi[i + (size_t)(gi + 2) * (j + (size_t)(gj + 2) * l)];
+        *o = *o + dt * p;
+      }
+}
+
 /* One time step in the order of the CABARET time loop (qg3.F90:590-967). */
 int pqo_step(pqo_state *s, const pqo_params *p, pqo_inv *inv, int do_corrector) {
   const int g = s->g, L = s->layers;
@@ -1044,6 +1100,10 @@ int pqo_step(pqo_state *s, const pqo_params *p, pqo_inv *inv, int do_corrector) 
                             s->rel);
   pqo_update_zeta_bc(g, L, p->s1, p->s2, s->rel, s->psi, s->zeta);
   s->model_time += 0.5 * s->dt;
+  if (s->psi_av && s->model_time > s->max_spin) {
+    pqo_psi_av_add(g, g, L, s->dt, s->psi, s->psi_av);
+    s->diag_time = s->diag_time + s->dt;
+  }
   pqo_cabaret_forcing(g, L, s->dt, s->force_wind, p->viscosity, p->visc_bot, s->rel,
                       s->zeta);
   pqo_cabaret_cell_boundaries(g, L, s->tweight_u, s->psi, s->ui, s->uj, s->uimo, s->ujmo);

@@ -46,6 +46,16 @@ void pqo_add_laplace_bc_rhs(int g, int layers, double bc, double *rhs,
 /* Trapezoid mean of one g x g layer (qg3p_diagnostics.F90:669-720).       */
 double pqo_mean(int g, const double *psi, int ld1);
 
+/* ---- energy diagnostics (qg3p_diagnostics.F90:778-863) ---- */
+/* One closed domain of gi x gj nodes; psi is the data array
+ * (gi+2, gj+2, L) with halo (sdata(i, j) = data(i+1, j+1)); h, s2 [L].   */
+void pqo_energy(int gi, int gj, int layers, const double *h, const double *s2,
+                const double *psi, double *ekin, double *epot);
+/* psi_av(i,j,l) += dt * psi(i,j,l) over all nodes (qg3.F90:672-681); psi is
+ * a data array as above, psi_av (gi, gj, L).                              */
+void pqo_psi_av_add(int gi, int gj, int layers, double dt, const double *psi,
+                    double *psi_av);
+
 /* ---- stratification (qg3p_diagnostics.F90:210-422) ---- */
 /* s1[l], s2[l] as in the reference's stratification type (s1(1) and s2(L)
  * unused).  Outputs: ss[L], modes_to_z[L*L], z_to_modes[L*L] (column-major).
@@ -124,6 +134,9 @@ typedef struct {
   double *res, *res_beta;               /* g g L */
   double *force_wind;                   /* g g   */
   double dt, tweight_a, tweight_u, model_time;
+  /* time mean (qg3.F90:672-681): psi_av (g g L) or NULL when not averaging */
+  double *psi_av;
+  double max_spin, diag_time;
 } pqo_state;
 
 typedef struct {

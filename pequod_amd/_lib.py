@@ -84,6 +84,15 @@ SIGNATURES = {
     "pq_model_set": (_i, [_vp, _i, _dp, _sz]),
     "pq_model_scalars": (_i, [_vp, _dp]),
     "pq_model_stream": (_vp, [_vp]),
+    "pq_model_average": (_i, [_vp, _i, _d]),
+    "pq_model_psi_av": (_i, [_vp, _dp, _sz, _dp]),
+    "pq_model_energy": (_i, [_vp, _dp, _dp, _dp]),
+    "pq_diag_create": (_i, [ctypes.POINTER(_vp), _i, _i, _dp, _dp]),
+    "pq_diag_destroy": (_i, [_vp]),
+    "pq_diag_stream": (_vp, [_vp]),
+    "pq_diag_energy": (_i, [_vp, _dp, _i, _i, _dp, _dp]),
+    "pq_diag_energy_dev": (_i, [_vp, _vp, _l, _l, _vp, _vp]),
+    "pq_psi_av_add_dev": (_i, [_i, _i, _d, _vp, _l, _l, _vp, _l, _l, _vp]),
     "pq_ens_create": (_i, [ctypes.POINTER(_vp), _i]),
     "pq_ens_add": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp, _l, _l, _vp, _l, _l, ctypes.POINTER(_i)]),
     "pq_ens_add_model": (_i, [_vp, _vp, _dp, _dp, _dp, ctypes.POINTER(_i)]),

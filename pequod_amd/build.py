@@ -24,6 +24,7 @@ SOURCES = [
     "pq_inversion.hip",
     "pq_cabaret.hip",
     "pq_ensemble.hip",
+    "pq_diag.hip",
     "pq_serve.cpp",
 ]
 

@@ -16,6 +16,7 @@
 #include "pq_common.h"
 #include "pq_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -509,6 +510,12 @@ struct pq_model {
   double visc = 0, visc_bot = 0, alpha = 0;
   int bc_type = 1, adapt = 0;
   double target_cfl = 0, max_dt = 0, max_dt_inc = 0;
+  // time mean of psi (qg3.F90:672-681) and energy diagnostics
+  int average = 0;
+  double max_spin = 0, diag_time = 0;
+  double *psi_av = nullptr;          // (G, G, L)
+  std::vector<double> s2_host, diag_h;
+  pq_diag *diag = nullptr;
 };
 
 static double **model_field(pq_model *m, int which, size_t *count) {
@@ -539,6 +546,8 @@ int pq_model_destroy(pq_model *m) {
                     m->zfj, m->uj, m->ujmo, m->flux_j, m->res, m->res_beta, m->fw, m->fin, m->fjn,
                     m->s1, m->s2, m->beta})
     (void)hipFree(p);
+  (void)hipFree(m->psi_av);
+  pq_diag_destroy(m->diag);
   (void)hipFree(m->maxu);
   (void)hipHostFree(m->maxu_host);
   pq_inv_destroy(m->inv);  // also destroys m->stream, which is the handle's stream
@@ -587,6 +596,7 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
   (void)hipMemcpy(m->s2, p->s2, sizeof(double) * p->layers, hipMemcpyHostToDevice);
   (void)hipMemcpy(m->beta, p->beta, sizeof(double) * p->layers, hipMemcpyHostToDevice);
   m->stream = (hipStream_t)pq_inv_stream(m->inv);
+  m->s2_host.assign(p->s2, p->s2 + p->layers);
   m->dt = p->dt;
   m->visc = p->viscosity;
   m->visc_bot = p->visc_bot;
@@ -693,6 +703,12 @@ static int model_post(pq_model *m, hipStream_t s, bool last) {
   hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,
                      m->psi, m->zeta);
   m->model_time += 0.5 * dt;
+  if (m->average && m->model_time > m->max_spin) {
+    const int ld = g + 2;
+    int rc = pq_psi_av_add_dev(g, L, dt, m->psi + 1 + ld, ld, ld, m->psi_av, g, g, s);
+    if (rc) return rc;
+    m->diag_time += dt;
+  }
   hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw, m->visc,
                      m->visc_bot, m->rel, m->zeta);
   hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->tw_u,
@@ -743,17 +759,25 @@ void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_
   *zeta_sdata = m->zeta + 1 + (m->g + 2);
   *psi_sdata = m->psi + 1 + (m->g + 2);
 }
-double model_time(const pq_model *m) { return m->model_time; }
+ModelClock model_clock(const pq_model *m) { return {m->model_time, m->diag_time}; }
+void model_set_clock(pq_model *m, ModelClock c) {
+  m->model_time = c.t;
+  m->diag_time = c.diag_t;
+}
+int model_step_kind(const pq_model *m) {
+  if (!m->average) return 0;
+  return m->model_time + 0.5 * m->dt > m->max_spin ? 1 : -1;
+}
 size_t model_device_bytes(const pq_model *m) {
   const Dims &d = m->d;
   return sizeof(double) * (4 * d.n_nd() + 5 * d.n_fi() + 5 * d.n_fj() + 2 * d.n_rs() +
-                           (size_t)m->g * m->g + 3 * (size_t)m->L) +
+                           (size_t)m->g * m->g + 3 * (size_t)m->L + (m->psi_av ? d.n_rs() : 0)) +
          inv_device_bytes(m->inv);
 }
-void model_set_time(pq_model *m, double t) { m->model_time = t; }
 // the same two half-step additions as model_post, so times round alike
 void model_account_step(pq_model *m) {
   m->model_time += 0.5 * m->dt;
+  if (m->average && m->model_time > m->max_spin) m->diag_time += m->dt;
   m->model_time += 0.5 * m->dt;
 }
 }  // namespace pq
@@ -789,6 +813,53 @@ int pq_model_scalars(pq_model *m, double *out4) {
   out4[2] = m->tw_u;
   out4[3] = m->model_time;
   return PQ_OK;
+}
+
+int pq_model_average(pq_model *m, int enable, double max_spin) {
+  PQ_CHECK_ARG(m != nullptr, "pq_model_average: null model");
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  if (enable) {
+    if (!m->psi_av && hipMalloc((void **)&m->psi_av, sizeof(double) * m->d.n_rs()) != hipSuccess) {
+      m->psi_av = nullptr;
+      pq_set_error("pq_model_average: out of device memory");
+      return PQ_ERR_NOMEM;
+    }
+    PQ_HIP(hipMemset(m->psi_av, 0, sizeof(double) * m->d.n_rs()));
+    m->diag_time = 0.0;
+    m->max_spin = max_spin;
+  }
+  m->average = enable ? 1 : 0;
+  return PQ_OK;
+}
+
+int pq_model_psi_av(pq_model *m, double *host, size_t count, double *diag_time) {
+  PQ_CHECK_ARG(m && host, "pq_model_psi_av: null argument");
+  const size_t n = m->d.n_rs();
+  PQ_CHECK_ARG(count >= n, "pq_model_psi_av: needs %zu values, buffer holds %zu", n, count);
+  PQ_CHECK_ARG(m->psi_av && m->diag_time > 0.0, "pq_model_psi_av: no time-mean samples yet");
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  PQ_HIP(hipMemcpy(host, m->psi_av, n * sizeof(double), hipMemcpyDeviceToHost));
+  const double cff = 1.0 / m->diag_time;
+  for (size_t k = 0; k < n; ++k) host[k] *= cff;
+  if (diag_time) *diag_time = m->diag_time;
+  return PQ_OK;
+}
+
+int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot) {
+  PQ_CHECK_ARG(m && h && ekin && epot, "pq_model_energy: null argument");
+  if (!m->diag || !std::equal(m->diag_h.begin(), m->diag_h.end(), h)) {
+    pq_diag_destroy(m->diag);
+    m->diag = nullptr;
+    m->diag_h.clear();
+    int rc = pq_diag_create(&m->diag, m->g, m->L, h, m->s2_host.data());
+    if (rc) return rc;
+    m->diag_h.assign(h, h + m->L);
+  }
+  // the model's psi is read on the model's stream; the energy handle's own
+  // synchronous call then copies the results
+  const int ld = m->g + 2;
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  return pq::diag_energy_sync(m->diag, m->psi + 1 + ld, ld, ld, ekin, epot);
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@ struct pq_ens {
     long ldz1, ldz2, ldp1, ldp2;
     pq_model *model;
     int cls = -1, slot = -1;
+    int step_kind = 0;  // model_step_kind at the capture of the step
   };
   std::vector<Member> mem;
   struct Class {
@@ -268,6 +269,9 @@ int pq_ens_step(pq_ens *e, int nsteps) {
   PQ_CHECK_ARG(nsteps >= 0, "pq_ens_step: nsteps < 0");
   for (int it = 0; it < nsteps; ++it) {
     if (e->exec_step) {
+      for (auto &m : e->mem)
+        PQ_CHECK_ARG(!m.model || model_step_kind(m.model) == m.step_kind,
+                     "pq_ens_step: a model's time-mean setting changed since the capture; capture again");
       PQ_HIP(hipGraphLaunch(e->exec_step, e->stream));
       for (auto &m : e->mem)
         if (m.model) model_account_step(m.model);
@@ -287,6 +291,14 @@ int pq_ens_capture(pq_ens *e, int what) {
   int rc = ens_check(e, "pq_ens_capture");
   if (rc) return rc;
   PQ_CHECK_ARG(what == 0 || what == 1, "pq_ens_capture: what must be 0 or 1");
+  // a captured step holds the time-mean accumulation or not for good
+  if (what == 1)
+    for (auto &m : e->mem) {
+      PQ_CHECK_ARG(!m.model || model_step_kind(m.model) >= 0,
+                   "pq_ens_capture: a model's time mean starts within the coming steps (spin-up not "
+                   "over); step it eagerly until then");
+      if (m.model) m.step_kind = model_step_kind(m.model);
+    }
   // one eager inversion first (it changes only psi, which every step
   // recomputes): kernels get their LDS limits outside the capture
   rc = ens_invert_on(e, e->stream);
@@ -302,16 +314,16 @@ int pq_ens_capture(pq_ens *e, int what) {
     (void)hipGraphDestroy(graph);
     graph = nullptr;
   }
-  // host-side model times advance as the graph replays, not at capture
-  std::vector<double> times;
+  // host-side model clocks advance as the graph replays, not at capture
+  std::vector<ModelClock> times;
   for (auto &m : e->mem)
-    if (m.model) times.push_back(model_time(m.model));
+    if (m.model) times.push_back(model_clock(m.model));
   PQ_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeGlobal));
   rc = what == 0 ? ens_invert_on(e, e->stream) : ens_step_on(e, e->stream);
   {
     size_t k = 0;
     for (auto &m : e->mem)
-      if (m.model) model_set_time(m.model, times[k++]);
+      if (m.model) model_set_clock(m.model, times[k++]);
   }
   hipGraph_t g = nullptr;
   hipError_t ee = hipStreamEndCapture(e->stream, &g);

@@ -34,9 +34,22 @@ int model_step_invert(pq_model *m);  // the step's inversion, on the model's str
 hipStream_t model_stream(pq_model *m);
 bool model_adaptive(const pq_model *m);
 void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld);
-double model_time(const pq_model *m);
-void model_set_time(pq_model *m, double t);
+// Host-side clocks of a model: model time and the time-mean weight.
+struct ModelClock {
+  double t, diag_t;
+};
+ModelClock model_clock(const pq_model *m);
+void model_set_clock(pq_model *m, ModelClock c);
+// What one captured step of the model contains: 0 no time-mean
+// accumulation, 1 an accumulation in every step from the next one on, -1
+// changes over the coming steps (averaging enabled, spin-up not yet over),
+// so that a captured step would go stale.
+int model_step_kind(const pq_model *m);
+// Host-side bookkeeping of one replayed step (clocks as model_step_post).
 void model_account_step(pq_model *m);
 size_t model_device_bytes(const pq_model *m);
+
+// ---- diagnostics (pq_diag.hip)
+int diag_energy_sync(pq_diag *d, const void *psi, long ld1, long ld2, double *ekin, double *epot);
 
 }  // namespace pq

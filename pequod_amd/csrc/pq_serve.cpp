@@ -68,6 +68,7 @@ struct Member {
   pq_model *model = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int step_kind = 0;  // model_step_kind at the capture
   hipEvent_t ev = nullptr;
   long long running = -1;
 };
@@ -196,7 +197,8 @@ bool pq_srv::pass() {
     m.q.pop_front();
     Req &r = req[id];
     hipStream_t s = model_stream(m.model);
-    int rc = PQ_OK;
+    // a captured step goes stale when pq_model_average changed what a step does
+    int rc = model_step_kind(m.model) == m.step_kind ? PQ_OK : PQ_ERR_ARG;
     for (int it = 0; it < r.steps && rc == PQ_OK; ++it) {
       if (hipGraphLaunch(m.exec, s) != hipSuccess) rc = PQ_ERR_HIP;
       else model_account_step(m.model);
@@ -375,7 +377,8 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sd
   if (rc) return unreserve(rc);
   // one full time step, captured once and replayed per step
   hipStream_t st = model_stream(m->model);
-  const double t0 = model_time(m->model);
+  const ModelClock t0 = model_clock(m->model);
+  m->step_kind = model_step_kind(m->model);
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return unreserve(PQ_ERR_HIP);
   // the three parts of pq_model_step (the inversion's kernels were first
   // launched by pq_model_init, so no attribute call happens in the capture)
@@ -384,7 +387,7 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sd
   if (!rc) rc = model_step_post(m->model, st, false);
   hipGraph_t gr = nullptr;
   const hipError_t ee = hipStreamEndCapture(st, &gr);
-  model_set_time(m->model, t0);
+  model_set_clock(m->model, t0);
   if (rc || ee != hipSuccess) {
     if (gr) (void)hipGraphDestroy(gr);
     if (!rc) pq_set_error("pq_srv_open_model: capture failed: %s", hipGetErrorString(ee));

@@ -117,6 +117,32 @@ module pequod_hip_c
       integer(c_int) :: rc
     end function pq_mirror_pull
 
+    ! qg3p_diagnostics.F90:778-863 (energy)
+    function pq_diag_create(d, g, layers, h, s2) bind(C, name = "pq_diag_create") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), intent(out) :: d
+      integer(c_int), value :: g, layers
+      real(c_double), dimension(*), intent(in) :: h, s2
+      integer(c_int) :: rc
+    end function pq_diag_create
+
+    function pq_diag_destroy(d) bind(C, name = "pq_diag_destroy") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: d
+      integer(c_int) :: rc
+    end function pq_diag_destroy
+
+    function pq_diag_energy(d, psi, ld1, ld2, ekin, epot) &
+      & bind(C, name = "pq_diag_energy") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: d
+      real(c_double), dimension(*), intent(in) :: psi
+      integer(c_int), value :: ld1, ld2
+      real(c_double), dimension(*), intent(out) :: ekin
+      real(c_double), intent(out) :: epot
+      integer(c_int) :: rc
+    end function pq_diag_energy
+
     function pq_model_stream(m) bind(C, name = "pq_model_stream") result(s)
       import :: c_ptr
       type(c_ptr), value :: m

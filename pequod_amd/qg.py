@@ -309,6 +309,7 @@ class Model:
 
     def __init__(self, g: int, strat: Stratification, beta, force_wind: np.ndarray, **kw):
         self.g, self.layers = g, strat.layers
+        self.strat = strat
         p, self._keep = model_params(g, strat, beta, force_wind, **kw)
         h = _vp()
         check(load().pq_model_create(ctypes.byref(h), ctypes.byref(p)))
@@ -342,6 +343,26 @@ class Model:
     @property
     def stream(self) -> int:
         return load().pq_model_stream(self._h)
+
+    def average(self, enable: bool = True, max_spin: float = 0.0) -> None:
+        """Time mean of psi from the first step whose model time exceeds
+        max_spin (qg3.F90:672-681); enabling restarts it."""
+        check(load().pq_model_average(self._h, int(enable), max_spin))
+
+    def psi_av(self):
+        """(psi_av / diagnostic_time as a (G, G, L) array, diagnostic_time)."""
+        a = np.zeros((self.g, self.g, self.layers), order="F")
+        t = ctypes.c_double()
+        check(load().pq_model_psi_av(self._h, _np_ptr(a), a.size, ctypes.byref(t)))
+        return a, t.value
+
+    def energy(self, h=None):
+        """energy(strat, psi, ekin, epot) of the current psi: (ekin[L], epot)."""
+        h = np.ascontiguousarray(self.strat.h if h is None else h, dtype=np.float64)
+        ekin = np.zeros(self.layers)
+        epot = ctypes.c_double()
+        check(load().pq_model_energy(self._h, _np_ptr(h), _np_ptr(ekin), ctypes.byref(epot)))
+        return ekin, epot.value
 
     def close(self):
         if getattr(self, "_h", None):

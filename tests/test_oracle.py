@@ -345,3 +345,106 @@ def test_cabaret_uniform_advection_exact_shift():
     shifted = np.concatenate([[prof[0]], prof[:-1]])
     assert np.array_equal(col[3:g], shifted[3:g])
     assert np.array_equal(zfi[3:g, g // 2, 0], prof[2:g - 1])
+
+
+# ------------------------------------------------------------ energy diagnostics
+def _strat2(rd, h1, h2):
+    s21, s12 = O.strat_l2(rd, h1, h2)
+    return np.array([s21, 0.0])
+
+
+def _rows_field(gi, gj, a, b):
+    """Data array (gi+2, gj+2, L) whose layer k is a_k * i + b_k on the owned
+    rows (i = 1..gi the global row) and on every column, halo included."""
+    L = len(a)
+    psi = np.zeros((gi + 2, gj + 2, L), order="F")
+    i = np.arange(1, gi + 1, dtype=np.float64)
+    for k in range(L):
+        psi[1:gi + 1, :, k] = (a[k] * i + b[k])[:, None]
+    return psi
+
+
+@pytest.mark.parametrize("case", KA["energy"]["cases"], ids=lambda c: c["name"])
+def test_energy_known_answers(case):
+    """unittest/test_energy_2.F90 on its own 5 x 9 node grid."""
+    ka = KA["energy"]
+    s2 = _strat2(ka["rd"], *ka["h"])
+    assert _fe(s2[0], 1.0)
+    psi = _rows_field(ka["gi"], ka["gj"], case["a"], case["b"])
+    ekin, epot = O.energy(psi, ka["h"], s2)
+    assert _fe(epot, case["epot"]), (epot, case["epot"])
+    assert all(_fe(x, y) for x, y in zip(ekin, case["ekin"])), (ekin, case["ekin"])
+
+
+def _energy_numpy(psi, h, s2):
+    """The same diagnostic written with array operations: trapezoid weights
+    w_i w_j on the interface differences, and per cell the mean over the cell
+    of the two centred gradient components."""
+    gi, gj, L = psi.shape[0] - 2, psi.shape[1] - 2, psi.shape[2]
+    p = psi[1:gi + 1, 1:gj + 1, :]
+    wi = np.ones(gi)
+    wi[[0, -1]] = 0.5
+    wj = np.ones(gj)
+    wj[[0, -1]] = 0.5
+    H = np.sum(h)
+    cff = 1.0 / ((gi - 1) * (gj - 1))
+    epot = sum(0.5 * s2[k] * h[k] / H * np.sum(np.outer(wi, wj) * (p[:, :, k] - p[:, :, k + 1]) ** 2)
+               for k in range(L - 1)) * cff
+    ekin = np.zeros(L)
+    for k in range(L):
+        q = p[:, :, k]
+        dj = 0.5 * ((q[1:, 1:] - q[1:, :-1]) + (q[:-1, 1:] - q[:-1, :-1]))
+        di = 0.5 * ((q[1:, 1:] - q[:-1, 1:]) + (q[1:, :-1] - q[:-1, :-1]))
+        ekin[k] = 0.5 * h[k] / H * np.sum(di ** 2 + dj ** 2) * cff
+    return ekin, epot
+
+
+@pytest.mark.parametrize("gi,gj,L", [(5, 9, 2), (9, 13, 3), (17, 6, 1), (33, 33, 5)])
+def test_energy_matches_array_form(gi, gj, L):
+    """A non-square grid tells i from j; random layers, depths and s2 tell
+    the layer and interface indices apart; halo values must not enter."""
+    rng = np.random.default_rng(gi * 100 + gj)
+    psi = rng.standard_normal((gi + 2, gj + 2, L))
+    psi = np.asfortranarray(psi)
+    h = rng.uniform(0.5, 2.0, L)
+    s2 = rng.uniform(0.1, 3.0, L)
+    s2[-1] = 0.0
+    ekin, epot = O.energy(psi, h, s2)
+    ek2, ep2 = _energy_numpy(psi, h, s2)
+    assert np.allclose(ekin, ek2, rtol=1e-13, atol=0) and abs(epot - ep2) <= 1e-13 * abs(ep2)
+    # the halo ring is never read on one domain
+    psi2 = psi.copy(order="F")
+    psi2[0, :, :] = psi2[-1, :, :] = np.nan
+    psi2[:, 0, :] = psi2[:, -1, :] = np.nan
+    ek3, ep3 = O.energy(psi2, h, s2)
+    assert np.array_equal(ek3, ekin) and ep3 == epot
+
+
+def test_energy_plane_closed_form():
+    """psi_k = a i + b j + c_k: ekin_k = (h_k / 2H)(a^2 + b^2) and
+    epot = sum_k s2_k h_k / (2H) (c_k - c_k+1)^2, on any grid."""
+    gi, gj, L = 12, 7, 3
+    a, b = 0.3, -1.7
+    c = np.array([2.0, -0.5, 1.0])
+    h = np.array([1.0, 2.0, 4.0])
+    s2 = np.array([0.8, 1.9, 0.0])
+    i = np.arange(gi + 2)[:, None, None]
+    j = np.arange(gj + 2)[None, :, None]
+    psi = np.asfortranarray(a * i + b * j + c[None, None, :] + 0.0 * i * j)
+    ekin, epot = O.energy(psi, h, s2)
+    H = h.sum()
+    assert np.allclose(ekin, h / (2 * H) * (a * a + b * b), rtol=1e-13)
+    ep = sum(s2[k] * h[k] / (2 * H) * (c[k] - c[k + 1]) ** 2 for k in range(L - 1))
+    assert abs(epot - ep) <= 1e-13 * ep
+
+
+def test_psi_av_add_is_dt_weighted_sum():
+    g, L = 6, 2
+    rng = np.random.default_rng(3)
+    av = np.zeros((g, g, L), order="F")
+    ref = np.zeros((g, g, L))
+    for dt in (0.5, 0.25, 1.0):
+        psi = np.asfortranarray(rng.standard_normal((g + 2, g + 2, L)))
+        O.psi_av_add(psi, av, dt)
+        ref = ref + dt * psi[1:g + 1, 1:g + 1, :]
+    assert np.array_equal(av, ref)
