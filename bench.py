@@ -155,6 +155,52 @@ def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11):
     return out
 
 
+def channel_rates(torch, dev, c, st, log, batch=8, reps=50, check=True):
+    """The periodic channel (3-layer, gi = 1025, period gj = 2048, partial
+    slip): batched inversions/s in FP32 and FP64, the FP32 pass times, and
+    the FP32 error against the FP64 oracle on two seeded k^-3 fields."""
+    from pequod_amd import qg, synth
+    g = c.g
+    gj = 2 * (g - 1)
+    L = st.layers
+    q = np.zeros((g, gj, L), order="F")
+    for half in range(2):
+        f = synth.k3_field(g - 2, L, synth.seed_for(c.config_id, 500 + half), c.rms) * c.t_scale
+        q[:, half * (g - 1):(half + 1) * (g - 1), :] = f[:, :g - 1, :]
+    alpha = c.nondim()["alpha"]
+    out = {"workload": f"{L}-layer channel {g} x {gj} (walls along i, periodic along j), partial slip",
+           "batch_per_step": batch}
+    for prec in ("fp32", "fp64"):
+        dt = torch.float64 if prec == "fp64" else torch.float32
+        base = torch.from_numpy(np.ascontiguousarray(q.transpose(2, 1, 0))).to(dev)
+        zs = [base.to(dt) * (1.0 + 1e-3 * k) for k in range(batch)]
+        ps = [torch.empty_like(zs[0]) for _ in range(batch)]
+        hc = qg.ChannelCache(g, st, 1, alpha, precision=prec, batch_max=batch)
+        for _ in range(3):
+            hc.run_dev(zs, ps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hc.run_dev(zs, ps)
+        torch.cuda.synchronize()
+        out[f"{prec}_inversions_per_s"] = round(batch * reps / (time.perf_counter() - t0), 1)
+        assert hc.status()[0] == 0
+        if prec == "fp32":
+            out["fp32_pass_ms"] = [round(x, 5) for x in hc.pass_times(zs, ps, reps=10)]
+            if check:
+                from oracle import oracle as O
+                ref, rc = O.Channel(g, gj, st.ss, st.modes_to_z, st.z_to_modes, 1, alpha).run(q)
+                hc.run_dev([base.to(dt)], [ps[0]])
+                torch.cuda.synchronize()
+                o = ps[0].double().cpu().numpy().transpose(2, 1, 0)
+                out["fp32_max_rel_psi_err_vs_fp64"] = float(max(
+                    np.max(np.abs(o[:, :, l] - ref[:, :, l])) / np.max(np.abs(ref[:, :, l]))
+                    for l in range(L)))
+        hc.deallocate()
+        log(f"[bench] channel {prec}: {out[f'{prec}_inversions_per_s']} inversions/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +223,7 @@ def main():
                     help="skip the serving leg (Poisson request stream, continuous batching)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg: requests per second")
     ap.add_argument("--serve-window", type=float, default=1.0, help="serving leg: arrival window [s]")
+    ap.add_argument("--no-channel", action="store_true", help="skip the periodic channel leg")
     ap.add_argument("--no-latency", action="store_true",
                     help="skip the B=1 latency leg (every inversion launch then has the bench's batch)")
     ap.add_argument("--sweep-tiles", action="store_true",
@@ -355,6 +402,10 @@ def main():
     if not args.no_ensemble:
         ens = ensemble_rates(torch, dev, args.ensemble_precision, log)
 
+    chan = None
+    if not args.no_channel:
+        chan = channel_rates(torch, dev, c, st, log, check=not args.no_cpu)
+
     serving = None
     if not args.no_serving:
         serving = serving_rates(torch, dev, log, rate=args.serve_rate, window=args.serve_window)
@@ -409,6 +460,7 @@ def main():
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
         "ensemble_64_mixed": ens,
+        "channel": chan,
         "serving": serving,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),

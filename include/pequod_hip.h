@@ -99,6 +99,12 @@ int pq_mirror_pull(void *host);       /* device -> host for the whole range */
 typedef struct pq_helm pq_helm;
 int pq_helm_create(pq_helm **h, int ii, int jj, int layers, const double *ss,
                    int precision);
+/* The channel's helm: walls at i = 1 and i = ii, periodic in j with period
+ * jj = 2 (ii - 1) (allocate(helm_c, decomp, ss) on a decomposition with
+ * periodic_j, qg3p_helmholtz.F90:292-386, 423-610).  The same solve and
+ * boundary semantics along i; every j is interior.                        */
+int pq_helm_create_periodic(pq_helm **h, int ii, int jj, int layers, const double *ss,
+                            int precision);
 int pq_helm_destroy(pq_helm *h);
 int pq_helm_solve(pq_helm *h, double *zeta, int ld1, int ld2);
 /* zeta is a device g x g x layers view of FP64 (or FP32 for PQ_PREC_FP32). */
@@ -119,6 +125,21 @@ typedef struct pq_inv pq_inv;
 int pq_inv_create(pq_inv **h, int g, int layers, const double *ss,
                   const double *modes_to_z, const double *z_to_modes,
                   int precision, int batch_max);
+/* The periodic channel (decomposition periodic in j): replaces
+ * allocate_helmholtz_cache and pv_inversion with
+ * correction_function_momentum, zeta_boundary and
+ * modes_to_z_correct_momentum (qg3p_aux.F90:152-193, 233-379, 452-479,
+ * 524-575).  Views are gi x gj with gj = 2 (gi - 1) (walls at i = 1 and
+ * i = gi, period gj along j).  The baroclinic modes are corrected so that
+ * their mean boundary relative PV vanishes on both walls; bc_type as the
+ * reference's velocity boundary condition (1 partial slip with alpha,
+ * 2 no slip, 3 free slip).  The returned handle is used with pq_inv_run /
+ * pq_inv_run_dev / pq_inv_status / pq_inv_set_tiles / pq_inv_destroy;
+ * pq_inv_status's psic receives (psiC_1, psiC_2) per mode (2 L values per
+ * member, those of the barotropic mode zero).                             */
+int pq_chan_create(pq_inv **h, int gi, int gj, int layers, const double *ss,
+                   const double *modes_to_z, const double *z_to_modes, int bc_type,
+                   double alpha, int precision, int batch_max);
 int pq_inv_destroy(pq_inv *h);
 /* Host drop-in: zeta/psi are g x g x L views (FP64).  Synchronous unless
  * both arrays are registered mirrors.  Returns PQ_ERR_NAN / PQ_WARN_LARGE

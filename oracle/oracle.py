@@ -47,6 +47,16 @@ def lib():
             "pqo_add_laplace_bc_rhs": (None, [i, i, d, _dp, i, i]),
             "pqo_mean": (d, [i, _dp, i]),
             "pqo_energy": (None, [i, i, i, _dp, _dp, _dp, _dp, _dp]),
+            "pqo_phelm_create": (vp, [i, i, i, _dp]),
+            "pqo_phelm_destroy": (None, [vp]),
+            "pqo_phelm_solve": (None, [vp, _dp, i, i]),
+            "pqo_chan_create": (vp, [i, i, i, _dp, _dp, _dp, i, d]),
+            "pqo_chan_destroy": (None, [vp]),
+            "pqo_chan_corr": (_dp, [vp]),
+            "pqo_chan_inv_corr_a": (_dp, [vp]),
+            "pqo_chan_psic": (_dp, [vp]),
+            "pqo_zeta_boundary": (None, [i, i, i, _dp, i, d, _dp, _dp, _dp]),
+            "pqo_chan_run": (i, [vp, _dp, i, i, _dp, i, i]),
             "pqo_psi_av_add": (None, [i, i, i, d, _dp, _dp]),
             "pqo_strat_modes": (i, [i, _dp, _dp, _dp, _dp, _dp]),
             "pqo_strat_l2": (None, [d, d, d, _dp, _dp]),
@@ -219,6 +229,79 @@ class Inversion:
     def __del__(self):
         if getattr(self, "h", None):
             lib().pqo_inv_destroy(self.h)
+            self.h = None
+
+
+class PeriodicHelm:
+    """helm on gi x gj nodes, walls at i = 1, gi, periodic in j (period gj)
+    (qg3p_helmholtz.F90:423-610, 614-769, the non-PH1D path)."""
+
+    def __init__(self, gi: int, gj: int, ss):
+        self.ss = f64(np.atleast_1d(ss))
+        self.gi, self.gj = gi, gj
+        self.h = lib().pqo_phelm_create(gi, gj, len(self.ss), _p(self.ss))
+        assert self.h
+
+    def solve(self, zeta: np.ndarray) -> None:
+        """In place on a Fortran-ordered (ld1, ld2, L) array."""
+        assert zeta.flags.f_contiguous and zeta.dtype == np.float64
+        z = zeta if zeta.ndim == 3 else zeta[:, :, None]
+        lib().pqo_phelm_solve(self.h, _p(zeta), z.shape[0], z.shape[1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pqo_phelm_destroy(self.h)
+            self.h = None
+
+
+def zeta_boundary(psi: np.ndarray, ss, bc_type: int, alpha: float):
+    """(z_b, z_d) of a (gi, gj, L) modal field (qg3p_aux.F90:288-379)."""
+    psi = f64(psi)
+    gi, gj, L = psi.shape
+    zb, zd = np.zeros(L), np.zeros(L)
+    lib().pqo_zeta_boundary(gi, gj, L, _p(f64(ss)), bc_type, alpha, _p(psi), _p(zb), _p(zd))
+    return zb, zd
+
+
+class Channel:
+    """pv_inversion for the periodic channel (qg3p_aux.F90:152-193, 233-379,
+    452-479, 524-575)."""
+
+    def __init__(self, gi: int, gj: int, ss, m2z, z2m, bc_type: int = 1, alpha: float = 0.0):
+        self.ss, self.m2z, self.z2m = f64(ss), f64(m2z), f64(z2m)
+        self.gi, self.gj, self.layers = gi, gj, len(self.ss)
+        self.h = lib().pqo_chan_create(gi, gj, self.layers, _p(self.ss), _p(self.m2z),
+                                       _p(self.z2m), bc_type, alpha)
+        assert self.h
+
+    def run(self, zeta: np.ndarray, psi: np.ndarray | None = None):
+        """zeta, psi: Fortran-ordered (ld1, ld2, L) arrays whose leading
+        gi x gj blocks are the views; returns (psi, status)."""
+        zeta = f64(zeta)
+        if psi is None:
+            psi = np.zeros((self.gi, self.gj, self.layers), order="F")
+        rc = lib().pqo_chan_run(self.h, _p(zeta), zeta.shape[0], zeta.shape[1], _p(psi),
+                                psi.shape[0], psi.shape[1])
+        return psi, rc
+
+    def corr(self) -> np.ndarray:
+        n = self.gi * self.gj * self.layers * 2
+        return np.ctypeslib.as_array(lib().pqo_chan_corr(self.h), (n,)).reshape(
+            (self.gi, self.gj, self.layers, 2), order="F").copy()
+
+    def inv_corr_a(self) -> np.ndarray:
+        n = 4 * max(self.layers - 1, 1)
+        return np.ctypeslib.as_array(lib().pqo_chan_inv_corr_a(self.h), (n,)).reshape(
+            (2, 2, -1), order="F").copy()
+
+    def psic(self) -> np.ndarray:
+        n = 2 * max(self.layers - 1, 1)
+        return np.ctypeslib.as_array(lib().pqo_chan_psic(self.h), (n,)).reshape(
+            (2, -1), order="F").copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pqo_chan_destroy(self.h)
             self.h = None
 
 

@@ -308,6 +308,341 @@ void pqo_helm_solve(pqo_helm *h, double *zeta, int ld1, int ld2) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Periodic-j Helmholtz solver (the channel)                                 */
+/* ------------------------------------------------------------------------ */
+struct pqo_phelm {
+  int gi, gj, layers;
+  pqo_sint *sint;
+  double *eig; /* eig(i, j, layer), gi x gj x L */
+  double *pb, *pc;
+  double *row;
+};
+
+#define PIDX(h, i, j, l) (((i)-1) + (size_t)(h)->gi * (((j)-1) + (size_t)(h)->gj * (l)))
+
+/* poinit_j (qg3p_helmholtz.F90:423-467) and periodic_coefficients
+ * (qg3p_helmholtz.F90:471-565) on one domain periodic in j: l_i = 2..ii-1,
+ * l_j = 1..jj, g_ii_int = ii - 1.                                          */
+pqo_phelm *pqo_phelm_create(int gi, int gj, int layers, const double *ss) {
+  if (gi < 3 || gj < 3 || layers < 1) return NULL;
+  pqo_phelm *h = (pqo_phelm *)calloc(1, sizeof(pqo_phelm));
+  h->gi = gi;
+  h->gj = gj;
+  h->layers = layers;
+  h->sint = pqo_sint_create(gi - 2);
+  const size_t sz = (size_t)gi * gj * layers;
+  h->eig = (double *)calloc(sz, sizeof(double));
+  h->pb = (double *)calloc(sz, sizeof(double));
+  h->pc = (double *)calloc(sz, sizeof(double));
+  h->row = (double *)malloc(sizeof(double) * (size_t)gi);
+#define EIG(i, j, l) h->eig[PIDX(h, i, j, l)]
+#define B(i, j, l) h->pb[PIDX(h, i, j, l)]
+#define C(i, j, l) h->pc[PIDX(h, i, j, l)]
+  const double cff = PQO_PI / (2.0 * (double)(gi - 1));
+  for (int l = 0; l < layers; ++l)
+    for (int i = 2; i <= gi - 1; ++i) {
+      const double s = sin((double)(i - 1) * cff);
+      const double d = -2.0 - ss[l] - 4.0 * (s * s);
+      for (int j = 1; j <= gj; ++j) EIG(i, j, l) = d;
+    }
+  for (int l = 0; l < layers; ++l)
+    for (int j = 3; j <= gj - 1; ++j)
+      for (int i = 2; i <= gi - 1; ++i) EIG(i, j, l) = EIG(i, j, l) - 1.0 / EIG(i, j - 1, l);
+  for (int l = 0; l < layers; ++l)
+    for (int j = 2; j <= gj - 1; ++j)
+      for (int i = 2; i <= gi - 1; ++i) EIG(i, j, l) = 1.0 / EIG(i, j, l);
+  /* periodic_coefficients */
+  for (int l = 0; l < layers; ++l) {
+    for (int i = 2; i <= gi - 1; ++i) {
+      const double s = sin((double)(i - 1) * cff);
+      const double a = -2.0 - ss[l] - 4.0 * (s * s);
+      B(i, 1, l) = a;
+      B(i, 2, l) = 1.0;
+      for (int j = 3; j <= gj - 1; ++j) B(i, j, l) = 0.0;
+      B(i, gj, l) = 1.0;
+      C(i, 1, l) = 1.0;
+      for (int j = 2; j <= gj - 2; ++j) C(i, j, l) = 0.0;
+      C(i, gj - 1, l) = 1.0;
+      C(i, gj, l) = a;
+    }
+    for (int j = 3; j <= gj - 1; ++j)
+      for (int i = 2; i <= gi - 1; ++i) {
+        B(i, j, l) = B(i, j, l) - B(i, j - 1, l) * EIG(i, j - 1, l);
+        C(i, j, l) = C(i, j, l) - C(i, j - 1, l) * EIG(i, j - 1, l);
+      }
+    for (int i = 2; i <= gi - 1; ++i) {
+      B(i, gj - 1, l) = B(i, gj - 1, l) * EIG(i, gj - 1, l);
+      C(i, gj - 1, l) = C(i, gj - 1, l) * EIG(i, gj - 1, l);
+    }
+    for (int j = gj - 2; j >= 2; --j)
+      for (int i = 2; i <= gi - 1; ++i) {
+        B(i, j, l) = (B(i, j, l) - B(i, j + 1, l)) * EIG(i, j, l);
+        C(i, j, l) = (C(i, j, l) - C(i, j + 1, l)) * EIG(i, j, l);
+      }
+    for (int i = 2; i <= gi - 1; ++i) {
+      B(i, 1, l) = B(i, 1, l) - B(i, 2, l);
+      C(i, 1, l) = C(i, 1, l) - C(i, 2, l);
+    }
+    for (int i = 2; i <= gi - 1; ++i) {
+      B(i, gj, l) = B(i, gj, l) - B(i, gj - 1, l);
+      C(i, gj, l) = C(i, gj, l) - C(i, gj - 1, l);
+    }
+    for (int i = 2; i <= gi - 1; ++i) C(i, 1, l) = C(i, 1, l) / B(i, 1, l);
+    for (int j = 2; j <= gj; ++j)
+      for (int i = 2; i <= gi - 1; ++i) C(i, j, l) = C(i, j, l) - C(i, 1, l) * B(i, j, l);
+  }
+  return h;
+}
+
+void pqo_phelm_destroy(pqo_phelm *h) {
+  if (!h) return;
+  pqo_sint_destroy(h->sint);
+  free(h->eig);
+  free(h->pb);
+  free(h->pc);
+  free(h->row);
+  free(h);
+}
+
+/* helm (qg3p_helmholtz.F90:614-769, single domain, no PH1D) with
+ * periodic_ge (qg3p_helmholtz.F90:567-610).                                */
+void pqo_phelm_solve(pqo_phelm *h, double *zeta, int ld1, int ld2) {
+  const int gi = h->gi, gj = h->gj, n = gi - 2;
+  const double cff = 0.5 / (double)(gi - 1);
+#define Z(i, j, l) zeta[((i)-1) + (size_t)ld1 * (((j)-1) + (size_t)ld2 * (l))]
+  for (int l = 0; l < h->layers; ++l) {
+    for (int j = 1; j <= gj; ++j) {
+      for (int i = 2; i <= gi - 1; ++i) Z(i, j, l) = cff * Z(i, j, l);
+      for (int i = 0; i < n; ++i) h->row[i] = Z(i + 2, j, l);
+      pqo_sint_apply(h->sint, h->row);
+      for (int i = 0; i < n; ++i) Z(i + 2, j, l) = h->row[i];
+    }
+    for (int j = 3; j <= gj - 1; ++j)
+      for (int i = 2; i <= gi - 1; ++i) Z(i, j, l) = Z(i, j, l) - Z(i, j - 1, l) * EIG(i, j - 1, l);
+    for (int i = 2; i <= gi - 1; ++i) Z(i, gj - 1, l) = Z(i, gj - 1, l) * EIG(i, gj - 1, l);
+    for (int j = gj - 2; j >= 2; --j)
+      for (int i = 2; i <= gi - 1; ++i)
+        Z(i, j, l) = (Z(i, j, l) - Z(i, j + 1, l)) * EIG(i, j, l);
+    /* periodic_ge */
+    for (int i = 2; i <= gi - 1; ++i) Z(i, 1, l) = Z(i, 1, l) - Z(i, 2, l);
+    for (int i = 2; i <= gi - 1; ++i) Z(i, gj, l) = Z(i, gj, l) - Z(i, gj - 1, l);
+    for (int i = 2; i <= gi - 1; ++i) Z(i, 1, l) = Z(i, 1, l) / B(i, 1, l);
+    for (int j = 2; j <= gj; ++j)
+      for (int i = 2; i <= gi - 1; ++i) Z(i, j, l) = Z(i, j, l) - Z(i, 1, l) * B(i, j, l);
+    for (int i = 2; i <= gi - 1; ++i) Z(i, gj, l) = Z(i, gj, l) / C(i, gj, l);
+    for (int j = gj - 1; j >= 1; --j)
+      for (int i = 2; i <= gi - 1; ++i) Z(i, j, l) = Z(i, j, l) - Z(i, gj, l) * C(i, j, l);
+    /* inverse transform, zeta(ii, j) = 0 */
+    for (int j = 1; j <= gj; ++j) {
+      for (int i = 0; i < n; ++i) h->row[i] = Z(i + 2, j, l);
+      pqo_sint_apply(h->sint, h->row);
+      for (int i = 0; i < n; ++i) Z(i + 2, j, l) = h->row[i];
+      Z(gi, j, l) = 0.0;
+    }
+  }
+#undef Z
+}
+#undef EIG
+#undef B
+#undef C
+
+/* ------------------------------------------------------------------------ */
+/* Channel PV inversion                                                      */
+/* ------------------------------------------------------------------------ */
+struct pqo_chan {
+  int gi, gj, layers, bc_type;
+  double alpha;
+  pqo_phelm *helm;
+  double *ss, *m2z, *z2m;
+  double *corr;       /* gi gj L 2 */
+  double *inv_corr_a; /* 2 2 (L-1) */
+  double *phi;        /* gi gj L */
+  double *psic;       /* 2 (L-1) */
+};
+
+/* zeta_boundary (qg3p_aux.F90:288-379): p_b = p_d = -1, l_j = 1..jj,
+ * g_jj_int = jj (periodic); allsum is the identity.                        */
+void pqo_zeta_boundary(int gi, int gj, int L, const double *ss, int bc_type,
+                       double alpha, const double *psi, double *z_b, double *z_d) {
+#define P(i, j, l) psi[((i)-1) + (size_t)gi * (((j)-1) + (size_t)gj * (l))]
+  for (int l = 0; l < L; ++l) {
+    z_b[l] = 0.0;
+    z_d[l] = 0.0;
+  }
+  if (bc_type == 1) {
+    const double cff = alpha / (alpha + 3.0);
+    for (int l = 0; l < L; ++l)
+      for (int j = 1; j <= gj; ++j)
+        z_b[l] = z_b[l] + cff * (-3.5 * P(1, j, l) + 4.0 * P(2, j, l) - 0.5 * P(3, j, l));
+    for (int l = 0; l < L; ++l)
+      for (int j = 1; j <= gj; ++j)
+        z_d[l] = z_d[l] + cff * (-3.5 * P(gi, j, l) + 4.0 * P(gi - 1, j, l) - 0.5 * P(gi - 2, j, l));
+  } else if (bc_type == 2) {
+    for (int l = 0; l < L; ++l)
+      for (int j = 1; j <= gj; ++j) z_b[l] = z_b[l] + 2.0 * (P(2, j, l) - P(1, j, l));
+    for (int l = 0; l < L; ++l)
+      for (int j = 1; j <= gj; ++j) z_d[l] = z_d[l] + 2.0 * (P(gi - 1, j, l) - P(gi, j, l));
+  }
+  for (int l = 0; l < L; ++l)
+    for (int j = 1; j <= gj; ++j) z_b[l] = z_b[l] - ss[l] * P(1, j, l);
+  for (int l = 0; l < L; ++l)
+    for (int j = 1; j <= gj; ++j) z_d[l] = z_d[l] - ss[l] * P(gi, j, l);
+  for (int l = 0; l < L; ++l) {
+    z_b[l] = z_b[l] / (double)gj;
+    z_d[l] = z_d[l] / (double)gj;
+  }
+#undef P
+}
+
+/* add_laplace_bc_rhs_periodic_layer (qg3p_laplace_bc.F90:128-150) and
+ * apply_dirichlet_bc_periodic_layer_data (qg3p_laplace_bc.F90:281-303). */
+static void chan_laplace_rhs(int gi, int gj, int L, double bc_b, double bc_d, double *r) {
+  for (int l = 0; l < L; ++l) {
+    for (int j = 1; j <= gj; ++j) r[1 + (size_t)gi * ((j - 1) + (size_t)gj * l)] -= bc_b;
+    for (int j = 1; j <= gj; ++j) r[(gi - 2) + (size_t)gi * ((j - 1) + (size_t)gj * l)] -= bc_d;
+  }
+}
+static void chan_dirichlet(int gi, int gj, int L, double bc_b, double bc_d, double *z) {
+  for (int l = 0; l < L; ++l) {
+    for (int j = 1; j <= gj; ++j) z[0 + (size_t)gi * ((j - 1) + (size_t)gj * l)] = bc_b;
+    for (int j = 1; j <= gj; ++j) z[(gi - 1) + (size_t)gi * ((j - 1) + (size_t)gj * l)] = bc_d;
+  }
+}
+
+/* inverse of a 2 x 2 matrix (lapack_interfaces.F90:319-359 calls LAPACK
+ * dgetrf/dgetri; the closed form agrees to rounding), column-major.        */
+static void inv2(const double *a, double *x) {
+  double a11 = a[0], a21 = a[1], a12 = a[2], a22 = a[3];
+  const double det = a11 * a22 - a12 * a21;
+  x[0] = a22 / det;
+  x[1] = -a21 / det;
+  x[2] = -a12 / det;
+  x[3] = a11 / det;
+}
+
+/* allocate_helmholtz_cache + correction_function_momentum
+ * (qg3p_aux.F90:152-193, 233-286).                                         */
+pqo_chan *pqo_chan_create(int gi, int gj, int L, const double *ss, const double *m2z,
+                          const double *z2m, int bc_type, double alpha) {
+  pqo_chan *v = (pqo_chan *)calloc(1, sizeof(pqo_chan));
+  v->gi = gi;
+  v->gj = gj;
+  v->layers = L;
+  v->bc_type = bc_type;
+  v->alpha = alpha;
+  v->helm = pqo_phelm_create(gi, gj, L, ss);
+  if (!v->helm) {
+    free(v);
+    return NULL;
+  }
+  v->ss = (double *)malloc(sizeof(double) * (size_t)L);
+  v->m2z = (double *)malloc(sizeof(double) * (size_t)L * L);
+  v->z2m = (double *)malloc(sizeof(double) * (size_t)L * L);
+  memcpy(v->ss, ss, sizeof(double) * (size_t)L);
+  memcpy(v->m2z, m2z, sizeof(double) * (size_t)L * L);
+  memcpy(v->z2m, z2m, sizeof(double) * (size_t)L * L);
+  const size_t sz = (size_t)gi * gj * L;
+  v->corr = (double *)calloc(2 * sz, sizeof(double));
+  v->phi = (double *)calloc(sz, sizeof(double));
+  v->inv_corr_a = (double *)calloc((size_t)4 * (L > 1 ? L - 1 : 1), sizeof(double));
+  v->psic = (double *)calloc((size_t)2 * (L > 1 ? L - 1 : 1), sizeof(double));
+  double *c1 = v->corr, *c2 = v->corr + sz;
+  chan_laplace_rhs(gi, gj, L, 1.0, 0.0, c1);
+  pqo_phelm_solve(v->helm, c1, gi, gj);
+  chan_dirichlet(gi, gj, L, 1.0, 0.0, c1);
+  chan_laplace_rhs(gi, gj, L, 0.0, 1.0, c2);
+  pqo_phelm_solve(v->helm, c2, gi, gj);
+  chan_dirichlet(gi, gj, L, 0.0, 1.0, c2);
+  double *zb1 = (double *)malloc(sizeof(double) * 4 * (size_t)L);
+  double *zd1 = zb1 + L, *zb2 = zb1 + 2 * L, *zd2 = zb1 + 3 * L;
+  pqo_zeta_boundary(gi, gj, L, ss, bc_type, alpha, c1, zb1, zd1);
+  pqo_zeta_boundary(gi, gj, L, ss, bc_type, alpha, c2, zb2, zd2);
+  for (int m = 1; m < L; ++m) {
+    /* z_b(mode, wall, k): rows the walls (b, d), columns the functions */
+    const double a[4] = {zb1[m], zd1[m], zb2[m], zd2[m]};
+    inv2(a, v->inv_corr_a + 4 * (m - 1));
+  }
+  free(zb1);
+  return v;
+}
+
+void pqo_chan_destroy(pqo_chan *v) {
+  if (!v) return;
+  pqo_phelm_destroy(v->helm);
+  free(v->ss);
+  free(v->m2z);
+  free(v->z2m);
+  free(v->corr);
+  free(v->phi);
+  free(v->inv_corr_a);
+  free(v->psic);
+  free(v);
+}
+
+const double *pqo_chan_corr(const pqo_chan *v) { return v->corr; }
+const double *pqo_chan_inv_corr_a(const pqo_chan *v) { return v->inv_corr_a; }
+const double *pqo_chan_psic(const pqo_chan *v) { return v->psic; }
+
+/* pv_inversion (qg3p_aux.F90:452-479) with modes_to_z_correct_momentum
+ * (qg3p_aux.F90:524-575).                                                  */
+int pqo_chan_run(pqo_chan *v, const double *zeta, int ldz1, int ldz2, double *psi, int ldp1,
+                 int ldp2) {
+  const int gi = v->gi, gj = v->gj, L = v->layers;
+  const size_t sz = (size_t)gi * gj * L, gg = (size_t)gi * gj;
+#define ZT(i, j, l) zeta[((i)-1) + (size_t)ldz1 * (((j)-1) + (size_t)ldz2 * (l))]
+#define PS(i, j, l) psi[((i)-1) + (size_t)ldp1 * (((j)-1) + (size_t)ldp2 * (l))]
+#define PH(i, j, l) v->phi[((i)-1) + (size_t)gi * ((j)-1) + gg * (l)]
+#define D(m, l) v->z2m[(m) + L * (l)]
+#define M(l, m) v->m2z[(l) + L * (m)]
+  for (int j = 1; j <= gj; ++j)
+    for (int i = 1; i <= gi; ++i)
+      for (int m = 0; m < L; ++m) {
+        double s = 0.0;
+        for (int l = 0; l < L; ++l) s += D(m, l) * ZT(i, j, l);
+        PH(i, j, m) = s;
+      }
+  pqo_phelm_solve(v->helm, v->phi, gi, gj);
+  chan_dirichlet(gi, gj, L, 0.0, 0.0, v->phi);
+  double *zb = (double *)malloc(sizeof(double) * 2 * (size_t)L), *zd = zb + L;
+  pqo_zeta_boundary(gi, gj, L, v->ss, v->bc_type, v->alpha, v->phi, zb, zd);
+  int status = 0;
+  for (int m = 1; m < L; ++m) {
+    const double *ia = v->inv_corr_a + 4 * (m - 1);
+    double p1 = ia[0] * zb[m] + ia[2] * zd[m];
+    double p2 = ia[1] * zb[m] + ia[3] * zd[m];
+    v->psic[2 * (m - 1)] = p1;
+    v->psic[2 * (m - 1) + 1] = p2;
+    if (p1 != p1 || p2 != p2) {
+      status = 3;
+      break;
+    }
+    if (fabs(p1) > 1.0e100 || fabs(p2) > 1.0e100) status = 4;
+    const double *c1 = v->corr + gg * m, *c2 = v->corr + sz + gg * m;
+    for (int j = 1; j <= gj; ++j)
+      for (int i = 1; i <= gi; ++i) {
+        const size_t k = ((size_t)i - 1) + (size_t)gi * (j - 1);
+        PH(i, j, m) = PH(i, j, m) - p1 * c1[k] - p2 * c2[k];
+      }
+  }
+  free(zb);
+  if (status == 3) return status;
+  /* modes_to_z_rank2 (qg3p_diagnostics.F90:570-587) */
+  for (int j = 1; j <= gj; ++j)
+    for (int i = 1; i <= gi; ++i)
+      for (int l = 0; l < L; ++l) {
+        double s = 0.0;
+        for (int m = 0; m < L; ++m) s += M(l, m) * PH(i, j, m);
+        PS(i, j, l) = s;
+      }
+  return status;
+#undef ZT
+#undef PS
+#undef PH
+#undef D
+#undef M
+}
+
+/* ------------------------------------------------------------------------ */
 /* Boundary helpers and mean                                                 */
 /* ------------------------------------------------------------------------ */
 /* apply_dirichlet_bc_layer (qg3p_laplace_bc.F90:193-224) */

@@ -38,6 +38,38 @@ void pqo_helm_destroy(pqo_helm *h);
 /* In place on zeta(ld1, ld2, layers); uses zeta(1:g, 1:g, :).             */
 void pqo_helm_solve(pqo_helm *h, double *zeta, int ld1, int ld2);
 
+/* ---- periodic-j Helmholtz solver, the channel's (non-PH1D path:
+ * poinit_j, periodic_coefficients, periodic_ge, helm;
+ * qg3p_helmholtz.F90:423-610, 614-769) on gi x gj nodes, walls at i = 1
+ * and i = gi, periodic in j with period gj. ---- */
+typedef struct pqo_phelm pqo_phelm;
+pqo_phelm *pqo_phelm_create(int gi, int gj, int layers, const double *ss);
+void pqo_phelm_destroy(pqo_phelm *h);
+/* In place on zeta(ld1, ld2, layers); uses zeta(1:gi, 1:gj, :).           */
+void pqo_phelm_solve(pqo_phelm *h, double *zeta, int ld1, int ld2);
+
+/* ---- channel PV inversion (qg3p_aux.F90:152-193, 233-379, 452-479,
+ * 524-575): helm with the momentum-conserving correction of the
+ * baroclinic modes.  bc_type 1 partial slip (alpha), 2 no slip, 3 free
+ * slip. ---- */
+typedef struct pqo_chan pqo_chan;
+pqo_chan *pqo_chan_create(int gi, int gj, int layers, const double *ss,
+                          const double *modes_to_z, const double *z_to_modes,
+                          int bc_type, double alpha);
+void pqo_chan_destroy(pqo_chan *v);
+/* corr (gi, gj, L, 2) and inv_corr_a (2, 2, L-1) of the handle; psiC
+ * (2, L-1) of the last run.                                               */
+const double *pqo_chan_corr(const pqo_chan *v);
+const double *pqo_chan_inv_corr_a(const pqo_chan *v);
+const double *pqo_chan_psic(const pqo_chan *v);
+/* zeta_boundary (qg3p_aux.F90:288-379) of psi (gi, gj, L), modal.         */
+void pqo_zeta_boundary(int gi, int gj, int layers, const double *ss, int bc_type,
+                       double alpha, const double *psi, double *z_b, double *z_d);
+/* pv_inversion of zeta into psi (g x g views as pqo_inv_run); returns 0, 3
+ * (NaN psiC) or 4 (|psiC| > 1e100).                                       */
+int pqo_chan_run(pqo_chan *v, const double *zeta, int ldz1, int ldz2, double *psi,
+                 int ldp1, int ldp2);
+
 /* ---- boundary helpers (qg3p_laplace_bc.F90:55-87, 193-224) ---- */
 void pqo_apply_dirichlet_bc(int g, int layers, double bc, double *zeta,
                             int ld1, int ld2);

@@ -38,10 +38,12 @@ SIGNATURES = {
     "pq_mirror_push": (_i, [_vp]),
     "pq_mirror_pull": (_i, [_vp]),
     "pq_helm_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _dp, _i]),
+    "pq_helm_create_periodic": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _dp, _i]),
     "pq_helm_destroy": (_i, [_vp]),
     "pq_helm_solve": (_i, [_vp, _dp, _i, _i]),
     "pq_helm_solve_dev": (_i, [_vp, _vp, _l, _l, _vp]),
     "pq_inv_create": (_i, [ctypes.POINTER(_vp), _i, _i, _dp, _dp, _dp, _i, _i]),
+    "pq_chan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _dp, _dp, _dp, _i, _d, _i, _i]),
     "pq_inv_destroy": (_i, [_vp]),
     "pq_inv_run": (_i, [_vp, _dp, _i, _i, _dp, _i, _i]),
     "pq_inv_run_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), _l, _l, ctypes.POINTER(_vp), _l, _l, _vp]),

@@ -693,4 +693,127 @@ template <typename T_, int N_, typename Tc_ = T_, int LOW_ = 0> struct DstExt {
   }
 };
 
+// ------------------------------------------------------ periodic transform
+// Real periodic DFT of length M = 2N (the channel's j direction, period
+// 2N = 2 (G - 1)) of two real rows a and b, packed as z = a + i b into one
+// complex FFT of length M:
+//   A_q = (Z_q + conj Z_{M-q}) / 2,   B_q = (Z_q - conj Z_{M-q}) / (2 i),
+// q = 0..N (the half spectra; A_0, A_N, B_0, B_N are real).  On the
+// physical side a row holds M reals (slot pad16(j)); on the spectral side
+// it holds the half spectrum as 2 (N + 1) reals, re and im of q at slots
+// pad16(2q) and pad16(2q + 1).  The inverse rebuilds Z'_q = A'_q + i B'_q
+// (q <= N), conj(A'_{M-q}) + i conj(B'_{M-q}) (q > N), and returns the
+// real and imaginary parts of its unnormalised inverse DFT, computed as the
+// conjugate of the forward FFT of conj(Z').  Forward then inverse
+// multiplies by M.
+template <typename T_, int N_, typename Tc_ = T_> struct PerRfft {
+  using T = T_;
+  using Tc = Tc_;
+  static constexpr int N = N_;
+  static constexpr int M = 2 * N;
+  // as DstExt: 32 elements per thread for FP64 arithmetic at M >= 4096
+  static constexpr int EF = (sizeof(Tc) > sizeof(T) && M >= 4096) ? 32 : 0;
+  using P = FftPlan<Tc, M, EF>;
+  static constexpr int TPF = P::TPF;
+  static constexpr int E = P::E;
+  static constexpr int RS = row_stride<2 * N + 2>();
+  static constexpr int FB = 2 * (M + M / 16 + 1);  // the complex FFT buffer, in T
+  static constexpr int Region = 2 * RS > FB ? 2 * RS : FB;
+  static constexpr int ScanTmp = 0;
+  static constexpr int KQ = (N + 1 + TPF - 1) / TPF;  // half-spectrum points per thread
+  static_assert(P::S.nst >= 2, "PerRfft needs at least two FFT stages");
+  static constexpr int LAST = P::S.nst - 1;
+  using S0 = Stage<Tc, M, 0, EF>;
+  using SL = Stage<Tc, M, LAST, EF>;
+
+  struct Ctx {
+    TwBase<Tc, M, EF> tw;
+    __device__ __forceinline__ void load(int t, const cx<double> *__restrict__ tw64, int ts = 1) {
+      tw.load(t, tw64, ts);
+    }
+  };
+
+  // Forward: ld(j, xa, xb) for 0 <= j < M (all loads happen before the
+  // first store, so the loader may read the region); the half spectra of a
+  // and b are left in the region's rows.
+  template <typename Loader>
+  __device__ __forceinline__ static void fwd(T *region, int t, const Ctx &c, Loader ld) {
+    cx<T> *buf = reinterpret_cast<cx<T> *>(region);
+    cx<Tc> v[S0::NB][S0::R];
+#pragma unroll
+    for (int b = 0; b < S0::NB; ++b)
+#pragma unroll
+      for (int r = 0; r < S0::R; ++r) {
+        T xa, xb;
+        ld(S0::in_index(t, b, r), xa, xb);
+        v[b][r] = cx<Tc>{Tc(xa), Tc(xb)};
+      }
+    group_sync<TPF>();
+    S0::compute(c.tw, v);
+    S0::store(buf, t, v);
+    group_sync<TPF>();
+    fft_stages<Tc, M, 1, -1, EF>(buf, t, c.tw);  // Z_k at buf[pad16(k)]
+    Tc h[KQ][4];
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) {
+      const int q = t + s * TPF;
+      if (q <= N) {
+        const cx<Tc> z1 = cx_cast<Tc>(buf[pad16(q)]);
+        const cx<Tc> z2 = cx_cast<Tc>(buf[pad16(q == 0 ? 0 : M - q)]);
+        h[s][0] = Tc(0.5) * (z1.x + z2.x);  // A_q
+        h[s][1] = Tc(0.5) * (z1.y - z2.y);
+        h[s][2] = Tc(0.5) * (z1.y + z2.y);  // B_q
+        h[s][3] = Tc(0.5) * (z2.x - z1.x);
+      }
+    }
+    group_sync<TPF>();
+    T *ra = region, *rb = region + RS;
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) {
+      const int q = t + s * TPF;
+      if (q <= N) {
+        ra[pad16(2 * q)] = T(h[s][0]);
+        ra[pad16(2 * q + 1)] = T(h[s][1]);
+        rb[pad16(2 * q)] = T(h[s][2]);
+        rb[pad16(2 * q + 1)] = T(h[s][3]);
+      }
+    }
+  }
+
+  // Inverse of the half spectra in the region's rows; st(j, xa, xb) is
+  // called once for every 0 <= j < M after the region has been read in full
+  // (a sink may write the rows).
+  template <typename Sink>
+  __device__ __forceinline__ static void inv(T *region, int t, const Ctx &c, Sink st) {
+    cx<T> *buf = reinterpret_cast<cx<T> *>(region);
+    const T *ra = region, *rb = region + RS;
+    cx<Tc> v[S0::NB][S0::R];
+#pragma unroll
+    for (int b = 0; b < S0::NB; ++b)
+#pragma unroll
+      for (int r = 0; r < S0::R; ++r) {
+        const int e = S0::in_index(t, b, r);
+        const bool lo = e <= N;
+        const int q = lo ? e : M - e;
+        const Tc ar = Tc(ra[pad16(2 * q)]), ai = Tc(ra[pad16(2 * q + 1)]);
+        const Tc br = Tc(rb[pad16(2 * q)]), bi = Tc(rb[pad16(2 * q + 1)]);
+        // conj(Z'_e)
+        v[b][r] = lo ? cx<Tc>{ar - bi, -(ai + br)} : cx<Tc>{ar + bi, ai - br};
+      }
+    group_sync<TPF>();
+    S0::compute(c.tw, v);
+    S0::store(buf, t, v);
+    group_sync<TPF>();
+    fft_stages<Tc, M, 1, LAST, EF>(buf, t, c.tw);
+    cx<Tc> w[SL::NB][SL::R];
+    SL::load(buf, t, w);
+    group_sync<TPF>();
+    SL::compute(c.tw, w);
+#pragma unroll
+    for (int b = 0; b < SL::NB; ++b)
+#pragma unroll
+      for (int r = 0; r < SL::R; ++r) st(SL::out_index(t, b, r), w[b][r].x, -w[b][r].y);
+  }
+};
+
 }  // namespace pq

@@ -11,7 +11,9 @@ namespace pq {
 //   M[L*L]   modes_to_z, M(l,m) at l + L*m
 //   ss[L], a[L], b[L], ssN[L] = (2N)^2 ss
 //   tab[(L-1) * N/2 * 3]: (rho, gamma, eta) for mode m >= 1 (0-based) and
-//                         odd p = 2h + 1
+//                         odd p = 2h + 1 (closed basin); the channel keeps
+//                         inv_corr_a of mode m >= 1 at tab + 4 m (2 x 2,
+//                         column-major)
 struct CoefLayout {
   int L, N;
   __host__ __device__ int D() const { return 0; }
@@ -57,9 +59,12 @@ __device__ __forceinline__ int member_of(const int *cum, int nb, int blk) {
 }
 
 // Launch-wide arguments of the three passes.  Member b of a launch (b < nb)
-// uses the scratch at scr + b * scr_stride bytes (L x N x N elements of the
-// compute type, tile-blocked), the column pass's psiC partials at
-// part + b * part_stride ((N / C) x L), psic + b * L and status + b.
+// uses the scratch at scr + b * scr_stride bytes (L x N x J elements of the
+// compute type, tile-blocked; J = N rows of j in the closed basin, 2N in
+// the channel), the column pass's partials at part + b * part_stride
+// ((N / C) x L psiC partials; channel: (N / C) x 2L boundary sums),
+// psic + b * psic_stride (L values; channel: psiC pairs of the modes) and
+// status + b.
 struct InvArgs {
   char *scr;
   long long scr_stride;
@@ -67,14 +72,20 @@ struct InvArgs {
   int part_stride;
   double *psic;      // psic_stride per member: psiC of the last run
   int psic_stride;
-  double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m
+  double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m;
+                     // channel: the correction E_l at l (N + 1) + p (p = 1..N-1
+                     // spectral, p = 0 and N the wall values)
   int kb_stride;
   int *status;       // sticky per member: 1 NaN psiC, 2 |psiC| > 1e100
   const void *tw64;  // cx<double>[M]: exp(-2 pi i n / M), M the engine's FFT length
   const void *was;   // Tc[N/2]: 2 sin(k pi / N) (DstSint), Tc the arithmetic type
-  const void *lam4;  // Tc[N]: (2N)^2 * (-4 sin^2(p pi / (2N)))
+  const void *lam4;  // Tc[N + 1]: (2N)^2 * (-4 sin^2(p pi / (2N)))
   const double *shat;  // double[N]: sum_i 2 sin(pi p i / N) (0 for even p)
+  // channel: double[3N]: boundary weights w_p, sin(pi p / N) / N, lambda_p
+  const double *ctab;
   int L, mode;       // L: the handle's (largest) layer count; mode 0 pv_inversion, 1 helm
+  int per;           // 0 closed basin (N rows of j), 1 periodic channel (2N rows)
+  int twm;           // length of the twiddle table tw64
   int R, lgR;        // rows per tile of the row passes (scratch block height)
 };
 
@@ -89,6 +100,8 @@ struct EngineInfo {
   int celem;         // sizeof of the arithmetic type (tables lam4, was)
   int fregion;       // LDS elements of T per group of the row forward pass
   int fscan;         //   (forward engine only) and its scan scratch
+  int ctpf;          // channel column pass (periodic engine of length 2N):
+  int cregion;       //   threads per group and LDS elements of T per group
 };
 
 // Supported transform lengths N = 2^a (8..4096) and 3 * 2^a (12..3072); one
@@ -107,9 +120,11 @@ struct EngineInfo {
 PQ_INV_SIZES(PQ_INV_DECL)
 #undef PQ_INV_DECL
 
-// LDS bytes of a pass for `groups` FFT groups.
-inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups) {
-  size_t b = (size_t)e.elem * groups * (pass == 0 ? e.fregion + e.fscan : e.region + e.scan);
+// LDS bytes of a pass for `groups` FFT groups (per: the channel's column
+// pass).
+inline size_t inv_lds_bytes(const EngineInfo &e, int pass, int groups, bool per = false) {
+  const int reg = pass == 0 ? e.fregion + e.fscan : (pass == 1 && per) ? e.cregion : e.region + e.scan;
+  size_t b = (size_t)e.elem * groups * reg;
   b = (b + 15) & ~size_t(15);
   if (pass == 1) b += 64 * 8 + (size_t)e.celem * (64 + 64 + 8);
   if (pass == 2) b += 16;

@@ -12,6 +12,8 @@
 //               (modes_to_z), inverse DST-I along j, write back in place.
 //   k_psic      (host file) per member: psiC from the column pass's partials,
 //               kappa_l and beta_lm.
+//   k_cols_chan the periodic channel's column pass (periodic transform of
+//               length 2N along j, boundary sums instead of psiC).
 //   k_rows_inv  per (member, layer, tile): read the tile block, subtract the mass
 //               correction in the i-spectral domain (closed form of
 //               helm(1)), inverse DST-I along i, subtract the constant part,
@@ -130,7 +132,7 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int R = a.R, lgR = a.lgR, NB = N >> lgR;
+  const int R = a.R, lgR = a.lgR, NB = (a.per ? 2 * N : N) >> lgR;
   const int b = member_of(ptrs.cum_r, ptrs.nb, blockIdx.x);
   const int rem = blockIdx.x - ptrs.cum_r[b];
   const int l = rem / NB, jb = rem - l * NB, j0 = jb << lgR;
@@ -138,14 +140,16 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
   const int ngrp = R >> 1, nt = ngrp * TPF;
   typename Eng::Fwd::Ctx ctx;
   ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
-           Eng::M / Eng::Fwd::M);
+           a.twm / Eng::Fwd::M);
   T *region = lds + f * Eng::Fwd::Region;
   T *scan_tmp = lds + ngrp * Eng::Fwd::Region + f * Eng::Fwd::ScanTmp;
   const Tin *z = static_cast<const Tin *>(ptrs.zeta[b]) + (size_t)l * ptrs.ldz1 * ptrs.ldz2;
   const int ja = j0 + 2 * f;
   const Tin *za = z + (size_t)ja * ptrs.ldz1;
   const Tin *zb = za + ptrs.ldz1;
-  const bool va = ja > 0;  // node row 0 is the boundary: not transformed
+  // node row 0 is the closed basin's boundary (not transformed); every row
+  // of the channel is a periodic interior row
+  const bool va = a.per || ja > 0;
   Eng::Fwd::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
     xa = va ? T(za[k]) : T(0);
     xb = T(zb[k]);
@@ -172,11 +176,17 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
 // coefficients sit in LDS as rows of 8: cD[m * 8 + l] = D(m,l),
 // cM[l * 8 + m] = M(l,m), cS[m] = (2N)^2 ss_m; up to 4 layers they are held
 // in registers, above that each row is read per point as 16-byte vectors.
-template <typename Eng, int LT>
+// PER (the channel): the rows hold half spectra along the period 2N, the
+// real and imaginary parts of q = 0..N at slots e = 2q and 2q + 1, with
+// the periodic eigenvalue -4 sin^2(pi q / 2N) = lambda_q; instead of psiC
+// the q = 0 point accumulates the boundary sums of the modal solution,
+// acc_m += phi_m w_p (wall i = 1) and acc_{8+m} += phi_m (-1)^(p+1) w_p
+// (wall i = G).
+template <typename Eng, int LT, bool PER = false>
 __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T *lds,
                                               const typename Eng::Tc *cD, const typename Eng::Tc *cM,
                                               const typename Eng::Tc *cS, int p0, int nt, int Lm,
-                                              int lgC, double (&acc)[8]) {
+                                              int lgC, double (&acc)[16]) {
   using T = typename Eng::T;
   using Tc = typename Eng::Tc;
   using VC = typename V16<Tc>::type;
@@ -203,12 +213,15 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
     const int p = p0 + c;
     if (p < 1) continue;
     const Tc lp = lam4[p];
-    const double sp = a.shat[p];
-    for (int q = 1 + threadIdx.x; q < N; q += nt) {
+    const double sp = PER ? a.ctab[p] : a.shat[p];
+    const double sgn = (p & 1) ? 1.0 : -1.0;
+    constexpr int E0 = PER ? 0 : 1, E1 = PER ? 2 * N + 2 : N;
+    for (int e = E0 + threadIdx.x; e < E1; e += nt) {
+      const int q = PER ? e >> 1 : e;
       const Tc lpq = lp + lam4[q];
       if constexpr (LT == 0) {
         for (int ll = 0; ll < Lm; ++ll) {
-          T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(q);
+          T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(e);
           *x = T(Tc(*x) / (lpq - cS[ll]));
         }
       } else {
@@ -217,10 +230,10 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
           // registers they would spill)
           asm volatile("" ::: "memory");
         }
-        const double wpq = sp * a.shat[q];
+        const double wpq = PER ? (e == 0 ? sp : 0.0) : sp * a.shat[q];
         Tc col[LT], phi[LT];
 #pragma unroll
-        for (int ll = 0; ll < LT; ++ll) col[ll] = Tc(row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)]);
+        for (int ll = 0; ll < LT; ++ll) col[ll] = Tc(row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(e)]);
 #pragma unroll
         for (int m = 0; m < LT; ++m) {
           Tc s = Tc(0);
@@ -238,7 +251,8 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
             }
             phi[m] = s / (lpq - cS[m]);
           }
-          acc[m] += double(phi[m]) * wpq;  // wpq = 0 unless p, q odd
+          acc[m] += double(phi[m]) * wpq;  // closed: wpq = 0 unless p, q odd
+          if constexpr (PER) acc[8 + m] += double(phi[m]) * (wpq * sgn);
         }
 #pragma unroll
         for (int ll = 0; ll < LT; ++ll) {
@@ -255,7 +269,7 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
                 if (v * VN + e < LT) s += mv[e] * phi[v * VN + e];
             }
           }
-          row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(q)] = T(s);
+          row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(e)] = T(s);
         }
       }
     }
@@ -285,7 +299,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
   const int nrows = L << lgC;  // row rho = l * C + c
   typename Eng::Fwd::Ctx ctx;
   ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
-           Eng::M / Eng::Fwd::M);
+           a.twm / Eng::Fwd::M);
   T *region = lds + f * Eng::Region;
   T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
   const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
@@ -340,9 +354,9 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
   __syncthreads();
 
   // Spectral step over the points (p0 + c, q).
-  double acc[8];
+  double acc[16];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) acc[m] = 0.0;
+  for (int m = 0; m < 16; ++m) acc[m] = 0.0;
   switch (a.mode == 0 ? L : 0) {
     case 1: spectral_step<Eng, 1>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
     case 2: spectral_step<Eng, 2>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
   {
     typename Eng::Inv::Ctx ctxi;
     ctxi.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
-              Eng::M / Eng::Inv::M);
+              a.twm / Eng::Inv::M);
     Eng::Inv::run_rows(region, t, scan_tmp, ctxi);
   }
   __syncthreads();
@@ -386,6 +400,135 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
 #pragma unroll
       for (int q = 0; q < V; ++q) v[q] = Ts(src[pad16((jb << lgR) + r0 + q)]);
       Ts *dst = scr + (size_t)ll * N * N + ((size_t)jb * N + p0 + c) * R + r0;
+      *reinterpret_cast<VT *>(dst) = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------ K2 channel
+// The channel's column pass: as k_cols, with the periodic transform of
+// length 2N along j (CE, a PerRfft engine) in place of the sine transform,
+// half spectra in the rows, and the FP64 boundary sums of the modal
+// solution (two per mode, the walls i = 1 and i = G) in place of the psiC
+// partials.  Scratch element (p, j) of layer l at
+// l N 2N + (j / R) N R + p R + j % R.
+template <typename CE, typename Ts>
+__global__ void __launch_bounds__(512) k_cols_chan(InvArgs a, PtrTab ptrs) {
+  using T = typename CE::T;
+  using Tc = typename CE::Tc;
+  using VT = typename V16<Ts>::type;
+  constexpr int N = CE::N, M = CE::M, TPF = CE::TPF, V = V16<Ts>::n;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *lds = reinterpret_cast<T *>(smem);
+  const int R = a.R, lgR = a.lgR;
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int b = member_of(ptrs.cum_c, ptrs.nb, bid);
+  const int L = ptrs.lay[b], lgC = ptrs.lgc[b], C = 1 << lgC;
+  const int pblk = bid - ptrs.cum_c[b];
+  const int p0 = pblk << lgC;
+  const int ngrp = blockDim.x / TPF, nt = blockDim.x;
+  const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
+  const int nrows = L << lgC;
+  typename CE::Ctx ctx;
+  ctx.load(t, static_cast<const cx<double> *>(a.tw64), a.twm / M);
+  T *region = lds + f * CE::Region;
+  const size_t red_off = (sizeof(T) * (size_t)ngrp * CE::Region + 15) & ~size_t(15);
+  double *red = reinterpret_cast<double *>(smem + red_off);
+  Tc *cD = reinterpret_cast<Tc *>(red + 64);
+  Tc *cM = cD + 64;
+  Tc *cS = cM + 64;
+  const CoefLayout CL{L, N};
+  const double *coef = ptrs.coef[b];
+  double cd0 = 0.0, cm0 = 0.0, cs0 = 0.0;
+  {
+    const int i = threadIdx.x, r = i >> 3, c = i & 7;
+    if (i < 64 && r < L && c < L) {
+      cd0 = coef[CL.D() + r + L * c];
+      cm0 = coef[CL.M() + r + L * c];
+    }
+  }
+  if ((int)threadIdx.x < L) cs0 = coef[CL.ssN() + threadIdx.x];
+
+  Ts *scr = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride);
+  const int ra = 2 * f, rb = ra + 1;
+  const int la = ra >> lgC, pa = p0 + (ra & (C - 1));
+  const int lb = rb >> lgC, pb = p0 + (rb & (C - 1));
+  const bool va = ra < nrows && pa >= 1;
+  const bool vb = rb < nrows && pb >= 1;
+  const Ts *sa = scr + (size_t)la * N * M + (size_t)pa * R;
+  const Ts *sb = scr + (size_t)lb * N * M + (size_t)pb * R;
+  const int blk = N << lgR;
+  CE::fwd(region, t, ctx, [&](int j, T &xa, T &xb) {
+    const int off = (j >> lgR) * blk + (j & (R - 1));
+    xa = va ? T(sa[off]) : T(0);
+    xb = vb ? T(sb[off]) : T(0);
+  });
+  if ((int)threadIdx.x < 64) {
+    cD[threadIdx.x] = Tc(cd0);
+    cM[threadIdx.x] = Tc(cm0);
+  }
+  if ((int)threadIdx.x < L) cS[threadIdx.x] = Tc(cs0);
+  for (int i = threadIdx.x + nt; i < 64; i += nt) {
+    const int r = i >> 3, c = i & 7;
+    const bool in = r < L && c < L;
+    cD[i] = in ? Tc(coef[CL.D() + r + L * c]) : Tc(0);
+    cM[i] = in ? Tc(coef[CL.M() + r + L * c]) : Tc(0);
+  }
+  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = Tc(coef[CL.ssN() + i]);
+  __syncthreads();
+
+  double acc[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) acc[m] = 0.0;
+  switch (a.mode == 0 ? L : 0) {
+    case 1: spectral_step<CE, 1, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 2: spectral_step<CE, 2, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 3: spectral_step<CE, 3, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 4: spectral_step<CE, 4, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 5: spectral_step<CE, 5, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 6: spectral_step<CE, 6, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 7: spectral_step<CE, 7, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    case 8: spectral_step<CE, 8, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+    default: spectral_step<CE, 0, true>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
+  }
+  if (a.mode == 0) {
+    double top[8], bot[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      top[m] = acc[m];
+      bot[m] = acc[8 + m];
+    }
+    block_sum(top, L, red);
+    block_sum(bot, L, red);
+    if (threadIdx.x == 0) {
+      double *part = a.part + (size_t)b * a.part_stride + (size_t)pblk * 2 * L;
+      for (int m = 0; m < L; ++m) {
+        part[m] = top[m];
+        part[L + m] = bot[m];
+      }
+    }
+  } else {
+    __syncthreads();
+  }
+
+  {
+    T *rowa = region, *rowb = region + CE::RS;
+    CE::inv(region, t, ctx, [&](int j, Tc xa, Tc xb) {
+      rowa[pad16(j)] = T(xa);
+      rowb[pad16(j)] = T(xb);
+    });
+  }
+  __syncthreads();
+  const int per_layer = (M << lgC) / V;
+  for (int ll = 0; ll < L; ++ll) {
+    for (int iv = threadIdx.x; iv < per_layer; iv += nt) {
+      const int e = iv * V;
+      const int r0 = e & (R - 1), c = (e >> lgR) & (C - 1), jb = e >> (lgR + lgC);
+      const T *src = row_ptr<CE>(lds, (ll << lgC) + c);
+      VT v;
+#pragma unroll
+      for (int q = 0; q < V; ++q) v[q] = Ts(src[pad16((jb << lgR) + r0 + q)]);
+      Ts *dst = scr + (size_t)ll * N * M + ((size_t)jb * N + p0 + c) * R + r0;
       *reinterpret_cast<VT *>(dst) = v;
     }
   }
@@ -408,7 +551,7 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   constexpr int NVT = 2 * N / (V * TPF);  // 16-byte vectors per thread of a tile
   extern __shared__ __align__(16) unsigned char smem[];
   T *lds = reinterpret_cast<T *>(smem);
-  const int R = a.R, lgR = a.lgR, NB = N >> lgR;
+  const int R = a.R, lgR = a.lgR, NB = (a.per ? 2 * N : N) >> lgR;
   const int b = member_of(ptrs.cum_r, ptrs.nb, blockIdx.x);
   const int L = ptrs.lay[b];
   const int rem = blockIdx.x - ptrs.cum_r[b];
@@ -429,7 +572,14 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   double beta[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) beta[m] = 0.0;
-  if (a.mode == 0) {
+  // channel: the correction's wall values E_l(0), E_l(N) (written by
+  // k_chan_corr)
+  double e0 = 0.0, eN = 0.0;
+  if (a.mode == 0 && a.per) {
+    const double *E = a.kb + (size_t)b * a.kb_stride + (size_t)l * (N + 1);
+    e0 = E[0];
+    eN = E[N];
+  } else if (a.mode == 0) {
     const double *kb = a.kb + (size_t)b * a.kb_stride;
     kappa = kb[l];
 #pragma unroll
@@ -438,7 +588,7 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   }
   typename Eng::Inv::Ctx ctx;
   ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
-           Eng::M / Eng::Inv::M);
+           a.twm / Eng::Inv::M);
 
   // Tile block (l, jb): row r at p <- S[p * R + r].
 #pragma unroll
@@ -450,8 +600,21 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
 
   __syncthreads();  // publishes the tile
 
+  // Channel: the correction E_l, uniform along j, subtracted in the
+  // i-spectral domain from every row of the tile.
+  if (a.mode == 0 && a.per && L > 1) {
+    const double *E = a.kb + (size_t)b * a.kb_stride + (size_t)l * (N + 1);
+    for (int p = 1 + threadIdx.x; p < N; p += nt) {
+      const double ep = E[p];
+      for (int r = 0; r < R; ++r) {
+        T *x = row_ptr<Eng>(lds, r) + pad16(p);
+        *x = T(double(*x) - ep);
+      }
+    }
+    __syncthreads();
+  }
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
-  if (a.mode == 0 && L > 1) {
+  if (a.mode == 0 && !a.per && L > 1) {
     const double *tab = ptrs.coef[b] + CL.tab();
     for (int h = threadIdx.x; h < N / 2; h += nt) {
       const int p = 2 * h + 1;
@@ -510,14 +673,18 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   const long long ld1 = ptrs.ldp1;
   constexpr int G = N + 1;
   const int ja = j0 + 2 * f;
+  // The channel (per): every row is interior; pv_inversion writes the
+  // walls i = 0 and i = N as -E_l(0) and -E_l(N) (X_0 = 0), helm keeps
+  // column 0 and zeroes i = N.
   Tout *pa = ps + (size_t)ja * ld1, *pb = pa + ld1;
-  const bool za = ja == 0;
+  const bool za = !a.per && ja == 0;
   const bool m0 = a.mode == 0;
-  const double sub = m0 ? kappa : 0.0;
+  const double sub = m0 ? (a.per ? 0.0 : kappa) : 0.0;
   auto sink = [&](int k, int, auto xa, auto xb) {
     if (m0 || k != 0) {
-      pa[k] = Tout((za ? 0.0 : double(xa)) - sub);
-      pb[k] = Tout(double(xb) - sub);
+      const double s0 = (a.per && k == 0) ? e0 : sub;
+      pa[k] = Tout((za ? 0.0 : double(xa)) - s0);
+      pb[k] = Tout(double(xb) - s0);
     }
   };
   {
@@ -532,21 +699,24 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
     }
   }
   if (t == 0) {
-    const Tout edge = Tout(0.0 - sub);  // +0 for helm
+    const Tout edge = Tout(0.0 - (a.per ? (m0 ? eN : 0.0) : sub));  // +0 for helm
     if (m0 || !za) pa[N] = edge;
     pb[N] = edge;
   }
-  if (m0 && j0 + R == N) {
+  if (m0 && !a.per && j0 + R == N) {
     Tout *pn = ps + (size_t)N * ld1;
     for (int i = threadIdx.x; i < G; i += nt) pn[i] = Tout(-kappa);
   }
 }
 
 // ------------------------------------------------------------ dispatch
-template <typename Eng, typename Ts> static EngineInfo info_of() {
+template <typename Eng, typename CE, typename Ts> static EngineInfo info_of() {
+  static_assert(std::is_same<typename CE::T, typename Eng::T>::value &&
+                    std::is_same<typename CE::Tc, typename Eng::Tc>::value,
+                "channel engine: types of the row engines");
   return EngineInfo{Eng::M, Eng::TPF, Eng::Region, Eng::ScanTmp, (int)sizeof(typename Eng::T),
                     (int)sizeof(Ts), (int)sizeof(typename Eng::Tc), Eng::Fwd::Region,
-                    Eng::Fwd::ScanTmp};
+                    Eng::Fwd::ScanTmp, CE::TPF, CE::Region};
 }
 
 // Engines per precision.  FP32 transforms are odd extensions (no prefix
@@ -563,6 +733,11 @@ using ExtWide = DstExt<float, PQ_N, double>;
 using P32 = EngPair<ExtF, ExtFLow>;
 using P64 = EngPair<SintD, SintD>;
 using PWide = EngPair<ExtWide, ExtWide>;
+// The channel's column engines (periodic transforms of length 2N), with
+// the LDS and arithmetic types of the row engines of each precision.
+using C32 = PerRfft<float, PQ_N, float>;
+using C64 = PerRfft<double, PQ_N, double>;
+using CWide = PerRfft<float, PQ_N, double>;
 
 // Dynamic LDS above the 64 KiB default needs a per-kernel limit (gfx950 has
 // 160 KiB per CU); raised once per kernel to the largest size requested.
@@ -576,13 +751,17 @@ template <typename K> static void lds_limit(K kernel, size_t bytes) {
   if (hipGetLastError() == hipSuccess) granted = bytes;
 }
 
-// Eng: transform engine (compute type), Tio: zeta/psi type, Ts: scratch type.
-template <typename Eng, typename Tio, typename Ts>
+// Eng: transform engine (compute type), CE: the channel's column engine,
+// Tio: zeta/psi type, Ts: scratch type.
+template <typename Eng, typename CE, typename Tio, typename Ts>
 static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int blocks, int threads,
                               size_t lds, hipStream_t s) {
   if (pass == 0) {
     lds_limit(k_rows_fwd<Eng, Tio, Ts>, lds);
     hipLaunchKernelGGL((k_rows_fwd<Eng, Tio, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
+  } else if (pass == 1 && a.per) {
+    lds_limit(k_cols_chan<CE, Ts>, lds);
+    hipLaunchKernelGGL((k_cols_chan<CE, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   } else if (pass == 1) {
     lds_limit(k_cols<Eng, Ts>, lds);
     hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
@@ -604,10 +783,10 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
 // (odd-extension engine).
 EngineInfo PQ_CAT(inv_info_, PQ_N)(int prec) {
   switch (prec) {
-    case PQ_PREC_FP64: return info_of<P64, double>();
-    case PQ_PREC_FP32_STORAGE: return info_of<P64, float>();
-    case PQ_PREC_FP32_WIDE: return info_of<PWide, float>();
-    default: return info_of<P32, float>();
+    case PQ_PREC_FP64: return info_of<P64, C64, double>();
+    case PQ_PREC_FP32_STORAGE: return info_of<P64, C64, float>();
+    case PQ_PREC_FP32_WIDE: return info_of<PWide, CWide, float>();
+    default: return info_of<P32, C32, float>();
   }
 }
 
@@ -615,15 +794,15 @@ hipError_t PQ_CAT(inv_launch_, PQ_N)(int prec, int pass, const InvArgs &a, const
                                      int blocks, int threads, size_t lds, hipStream_t s) {
   switch (prec) {
     case PQ_PREC_FP64:
-      return launch_pass<P64, double, double>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P64, C64, double, double>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32:
-      return launch_pass<P32, float, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P32, C32, float, float>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32_STORAGE:
-      return launch_pass<P64, float, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P64, C64, float, float>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32_WIDE:
-      return launch_pass<PWide, float, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<PWide, CWide, float, float>(pass, a, pt, blocks, threads, lds, s);
     default:
-      return launch_pass<P32, double, float>(pass, a, pt, blocks, threads, lds, s);
+      return launch_pass<P32, C32, double, float>(pass, a, pt, blocks, threads, lds, s);
   }
 }
 

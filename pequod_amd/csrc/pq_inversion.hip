@@ -109,6 +109,81 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
   }
 }
 
+// The channel's momentum-conserving correction (qg3p_aux.F90:524-575) per
+// member, from the column pass's boundary-sum partials (FP64, fixed order):
+// for every baroclinic mode m the mean boundary relative PV of the modal
+// solution on the walls i = 1 and i = G, (z_b, z_d) = sum_p phi_m(p, q=0)
+// (w_p, (-1)^(p+1) w_p), gives psiC_m = inv_corr_a_m (z_b, z_d).  The
+// correction functions are uniform along j, c1_m(i) = 1 on i = 1, 0 on
+// i = G, c2_m mirrored, with the i-spectral coefficients (in the inverse
+// row pass's units) -sin(pi p / N) / (N (lambda_p - ss_m)) and
+// (-1)^(p+1) times it.  So psi_l -= E_l with
+//   E_l(p) = sum_{m>=2} M(l,m) (psiC_m1 + (-1)^(p+1) psiC_m2) c_m(p),
+// E_l(0) = sum M(l,m) psiC_m1, E_l(N) = sum M(l,m) psiC_m2 on the walls.
+// Status bits as k_psic.  Grid: one 256-thread workgroup per member.
+__global__ void __launch_bounds__(256) k_chan_corr(InvArgs a, PtrTab ptrs, int N) {
+  __shared__ double red[4 * 16];
+  __shared__ double s_pc[16];
+  const int b = blockIdx.x, L = ptrs.lay[b], tid = threadIdx.x;
+  const int nblk = N >> ptrs.lgc[b];
+  const double *part = a.part + (size_t)b * a.part_stride;
+  double acc[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) acc[m] = 0.0;
+  for (int q = tid; q < nblk; q += 256)
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      if (m < L) {
+        acc[m] += part[(size_t)q * 2 * L + m];
+        acc[8 + m] += part[(size_t)q * 2 * L + L + m];
+      }
+  const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+  for (int m = 0; m < 16; ++m)
+    if ((m & 7) < L) {
+      double x = acc[m];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
+      if (lane == 0) red[w * 16 + m] = x;
+    }
+  __syncthreads();
+  const CoefLayout CL{L, N};
+  const double *coef = ptrs.coef[b];
+  if (tid == 0) {
+    int st = 0;
+    s_pc[0] = s_pc[8] = 0.0;
+    for (int m = 1; m < L; ++m) {
+      const double zb = red[m] + red[16 + m] + red[32 + m] + red[48 + m];
+      const double zd = red[8 + m] + red[24 + m] + red[40 + m] + red[56 + m];
+      const double *ia = coef + CL.tab() + 4 * m;
+      const double p1 = ia[0] * zb + ia[2] * zd, p2 = ia[1] * zb + ia[3] * zd;
+      s_pc[m] = p1;
+      s_pc[8 + m] = p2;
+      a.psic[(size_t)b * a.psic_stride + 2 * m] = p1;
+      a.psic[(size_t)b * a.psic_stride + 2 * m + 1] = p2;
+      if (p1 != p1 || p2 != p2) st |= 1;
+      else if (fabs(p1) > 1.0e100 || fabs(p2) > 1.0e100) st |= 2;
+    }
+    if (st) atomicOr(a.status + b, st);
+  }
+  __syncthreads();
+  const double *M = coef + CL.M(), *ss = coef + CL.ss();
+  const double *sinN = a.ctab + N, *lam = a.ctab + 2 * N;
+  double *E = a.kb + (size_t)b * a.kb_stride;
+  for (int idx = tid; idx < L * (N + 1); idx += 256) {
+    const int l = idx / (N + 1), p = idx - l * (N + 1);
+    double v = 0.0;
+    if (p == 0 || p == N) {
+      for (int m = 1; m < L; ++m) v += M[l + L * m] * s_pc[(p == 0 ? 0 : 8) + m];
+    } else {
+      const double sgn = (p & 1) ? 1.0 : -1.0;
+      for (int m = 1; m < L; ++m)
+        v += M[l + L * m] * (s_pc[m] + sgn * s_pc[8 + m]) * (-sinN[p] / (lam[p] - ss[m]));
+    }
+    E[idx] = v;
+  }
+}
+
 constexpr size_t kLdsMax = 160 * 1024;
 constexpr int kThreadsMax = 512;
 
@@ -119,6 +194,12 @@ using namespace pq;
 // ------------------------------------------------------------------ handle
 struct pq_inv {
   int g = 0, N = 0, L = 0, prec = 0, batch_max = 0, mode = 0;
+  // periodic channel: gj = 2N nodes along j (else gj = g), walls' boundary
+  // condition of the momentum constraint
+  int per = 0, gj = 0, bc_type = 1;
+  double alpha = 0.0;
+  double *ctab = nullptr;
+  int twm = 0;  // twiddle table length
   int R = 0, C = 0;
   std::vector<int> lay;  // layers of each member slot (<= L)
   KernelSet ks;
@@ -146,8 +227,11 @@ struct pq_helm {
 template <typename T> static int upload_tables(pq_inv *h, int M) {
   const int N = h->N;
   const long double pi = 3.141592653589793238462643383279502884L;
+  // the channel's column engine needs the table of length 2N
+  if (h->per && M < 2 * N) M = 2 * N;
+  h->twm = M;
   std::vector<double> tw(2 * (size_t)M);
-  std::vector<T> was(N / 2), lam4(N);
+  std::vector<T> was(N / 2), lam4(N + 1);
   for (int k = 0; k < M; ++k) {
     const long double a = -2.0L * pi * k / M;
     tw[2 * k] = (double)cosl(a);
@@ -157,23 +241,54 @@ template <typename T> static int upload_tables(pq_inv *h, int M) {
   const long double fourN2 = 4.0L * N * (long double)N;
   h->lam_h.assign(N, 0.0);
   h->shat_h.assign(N, 0.0);
-  for (int p = 0; p < N; ++p) {
+  for (int p = 0; p <= N; ++p) {
     const long double x = (long double)p * pi / (2.0L * N);
     const long double s = sinl(x);
     const long double lam = -4.0L * s * s;
+    lam4[p] = T(fourN2 * lam);  // p = N: the channel's q = N
+    if (p == N) break;
     h->lam_h[p] = (double)lam;
-    lam4[p] = T(fourN2 * lam);
     if (p & 1) h->shat_h[p] = (double)(2.0L * cosl(x) / s);
   }
   PQ_HIP(hipMalloc(&h->tw64, sizeof(double) * 2 * M));
   PQ_HIP(hipMalloc(&h->was, sizeof(T) * (N / 2)));
-  PQ_HIP(hipMalloc(&h->lam4, sizeof(T) * N));
+  PQ_HIP(hipMalloc(&h->lam4, sizeof(T) * (N + 1)));
   PQ_HIP(hipMalloc(&h->shat, sizeof(double) * N));
   PQ_HIP(hipMemcpy(h->tw64, tw.data(), sizeof(double) * 2 * M, hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(h->was, was.data(), sizeof(T) * (N / 2), hipMemcpyHostToDevice));
-  PQ_HIP(hipMemcpy(h->lam4, lam4.data(), sizeof(T) * N, hipMemcpyHostToDevice));
+  PQ_HIP(hipMemcpy(h->lam4, lam4.data(), sizeof(T) * (N + 1), hipMemcpyHostToDevice));
   PQ_HIP(hipMemcpy(h->shat, h->shat_h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  if (h->per) {
+    // boundary weights w_p of the mean boundary relative PV of a modal
+    // field from its (p, q = 0) coefficients (zeta_boundary,
+    // qg3p_aux.F90:288-379: the wall values vanish, the rows i = 2, 3 enter
+    // through the inverse sine transform X_k = 2 sum_p x_p sin(pi p k / N)),
+    // then sin(pi p / N) / N and lambda_p
+    std::vector<double> ct(3 * (size_t)N, 0.0);
+    const long double cff = h->bc_type == 1 ? (long double)h->alpha / (h->alpha + 3.0L) : 0.0L;
+    for (int p = 1; p < N; ++p) {
+      const long double s1 = sinl(pi * p / N), s2 = sinl(2.0L * pi * p / N);
+      long double w = 0.0L;
+      if (h->bc_type == 1) w = cff * 2.0L * (4.0L * s1 - 0.5L * s2);
+      else if (h->bc_type == 2) w = 2.0L * 2.0L * s1;
+      ct[p] = (double)w;
+      ct[N + p] = (double)(s1 / N);
+      ct[2 * N + p] = h->lam_h[p];
+    }
+    PQ_HIP(hipMalloc(&h->ctab, sizeof(double) * 3 * N));
+    PQ_HIP(hipMemcpy(h->ctab, ct.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+  }
   return PQ_OK;
+}
+
+// Values at the nodes k = 0..N of the channel's correction function c1 of
+// a mode with Helmholtz constant ss > 0: c1(k-1) + c1(k+1) - (2 + ss) c1(k)
+// = 0, c1(0) = 1, c1(N) = 0, i.e. sinh(kappa (N - k)) / sinh(kappa N) with
+// cosh kappa = 1 + ss / 2, in a form without overflow for large kappa N
+// (c2(k) = c1(N - k)).
+static double chan_c1(int N, double ss, int k) {
+  const double kap = std::log1p(ss / 2.0 + std::sqrt(ss * (1.0 + ss / 4.0)));
+  return std::exp(-kap * k) * std::expm1(-2.0 * kap * (N - k)) / std::expm1(-2.0 * kap * N);
 }
 
 // Coefficient block of one member with L layers (CoefLayout) into
@@ -192,6 +307,7 @@ static void compute_coefs(const pq_inv *h, int L, const double *ss, const double
   for (int m = 0; m < L; ++m) {
     c[CL.ss() + m] = ss[m];
     c[CL.ssN() + m] = fourN2 * ss[m];
+    if (h->per) continue;  // the mass correction's constants: closed basin only
     // mean of c_m = helm_m(1): (1/N^2) sum_{p,q odd} shat_p^2 shat_q^2 /
     // ((2N)^2 (lambda_p + lambda_q - ss_m))
     long double s = 0.0L;
@@ -204,7 +320,36 @@ static void compute_coefs(const pq_inv *h, int L, const double *ss, const double
     c[CL.a() + m] = a;
     c[CL.b() + m] = ss[m] * a;
   }
-  for (int m = 1; m < L; ++m)
+  if (h->per) {
+    // inv_corr_a per baroclinic mode (correction_function_momentum,
+    // qg3p_aux.F90:233-286): the inverse of zeta_boundary of the two
+    // correction functions on the two walls (rows: walls i = 1, i = G;
+    // columns: c1, c2 = c1 mirrored), column-major at tab + 4 m
+    const double cff = h->bc_type == 1 ? h->alpha / (h->alpha + 3.0) : 0.0;
+    for (int m = 1; m < L; ++m) {
+      const double sm = ss[m];
+      double v1[3], v2[3];  // c1 at the wall k = 0 and k = 1, 2; c1 at k = N, N-1, N-2
+      for (int k = 0; k < 3; ++k) {
+        v1[k] = chan_c1(N, sm, k);
+        v2[k] = chan_c1(N, sm, N - k);
+      }
+      auto zw = [&](const double *v) {  // wall value, then the two rows inside
+        double z = 0.0;
+        if (h->bc_type == 1) z = cff * (-3.5 * v[0] + 4.0 * v[1] - 0.5 * v[2]);
+        else if (h->bc_type == 2) z = 2.0 * (v[1] - v[0]);
+        return z - sm * v[0];
+      };
+      // c1 seen from wall i = 1 is v1, from wall i = G is v2; c2 the reverse
+      const double a11 = zw(v1), a21 = zw(v2), a12 = zw(v2), a22 = zw(v1);
+      const double det = a11 * a22 - a12 * a21;
+      double *ia = c + CL.tab() + 4 * m;
+      ia[0] = a22 / det;
+      ia[1] = -a21 / det;
+      ia[2] = -a12 / det;
+      ia[3] = a11 / det;
+    }
+  }
+  for (int m = 1; m < L && !h->per; ++m)
     for (int hh = 0; hh < N / 2; ++hh) {
       const int p = 2 * hh + 1;
       const double mu = lam[p] - ss[m];
@@ -234,8 +379,9 @@ static bool tiles_ok(const pq_inv *h, int R, int C, bool report) {
     return fail("row workgroup above 160 KiB of LDS");
   if (C < 1 || C > 8 || (C & (C - 1)) || h->N % C) return fail("C must be a power of two in 1..8 dividing N");
   const int groups = (C * h->L + 1) / 2;
-  if (groups * e.tpf > kThreadsMax) return fail("column workgroup above 512 threads");
-  if (inv_lds_bytes(e, 1, groups) > kLdsMax) return fail("column workgroup above 160 KiB of LDS");
+  const int ctpf = h->per ? e.ctpf : e.tpf;
+  if (groups * ctpf > kThreadsMax) return fail("column workgroup above 512 threads");
+  if (inv_lds_bytes(e, 1, groups, h->per) > kLdsMax) return fail("column workgroup above 160 KiB of LDS");
   if (h->N / C > h->part_cap) return fail("C below the handle's minimum");
   return true;
 }
@@ -272,8 +418,17 @@ extern "C" int pq_prec_fp32_choice(int L, const double *m2z, const double *z2m) 
   return norm_inf(m2z) * norm_inf(z2m) <= 32.0 ? PQ_PREC_FP32 : PQ_PREC_FP32_STORAGE;
 }
 
+// Per-member sizes (doubles) of the column-pass partials, psiC and the
+// row-inverse constants.
+static size_t part_per(const pq_inv *h) { return (size_t)h->part_cap * h->L * (h->per ? 2 : 1); }
+static size_t psic_per(const pq_inv *h) { return (size_t)h->L * (h->per ? 2 : 1); }
+static size_t kb_per(const pq_inv *h) {
+  return h->per ? (size_t)h->L * (h->N + 1) : (size_t)(h->L + h->L * h->L);
+}
+
 static int inv_create(pq_inv **out, int g, int L, const double *ss, const double *m2z,
-                      const double *z2m, int precision, int batch_max, int mode) {
+                      const double *z2m, int precision, int batch_max, int mode, int per = 0,
+                      int bc_type = 1, double alpha = 0.0) {
   PQ_CHECK_ARG(out != nullptr, "pq_inv_create: null handle pointer");
   PQ_CHECK_ARG(g >= 9 && pq_supported_n(g - 2),
                "pq_inv_create: unsupported grid size g=%d (need g-1 = 2^a or 3*2^a, 8 <= g-1 <= 4096)", g);
@@ -298,6 +453,10 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   h->prec = precision;
   h->batch_max = batch_max;
   h->mode = mode;
+  h->per = per;
+  h->gj = per ? 2 * (g - 1) : g;
+  h->bc_type = bc_type;
+  h->alpha = alpha;
   const int N = h->N;
   if (!kernels_for(N, h->ks)) {
     delete h;
@@ -324,7 +483,7 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   std::vector<double> coef_h(h->coef_size);
   compute_coefs(h, L, ss, m2z, z2m, coef_h.data());
   h->lay.assign(batch_max, L);
-  h->scr_member = (size_t)L * N * N * h->ei.selem;
+  h->scr_member = (size_t)L * N * (per ? 2 * N : N) * h->ei.selem;
   // every member slot starts with the creation stratification
   if (hipMalloc(&h->coef, h->coef_size * sizeof(double) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
   for (int b = 0; b < batch_max; ++b)
@@ -335,14 +494,14 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
     pq_set_error("pq_inv_create: cannot allocate %zu bytes of scratch", h->scr_member * batch_max);
     return fail(PQ_ERR_NOMEM);
   }
-  if (hipMalloc(&h->part, sizeof(double) * (size_t)h->part_cap * L * batch_max) != hipSuccess)
+  if (hipMalloc(&h->part, sizeof(double) * part_per(h) * batch_max) != hipSuccess)
     return fail(PQ_ERR_NOMEM);
-  if (hipMalloc(&h->psic, sizeof(double) * (size_t)L * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
-  if (hipMalloc(&h->kb, sizeof(double) * (size_t)(L + L * L) * batch_max) != hipSuccess)
+  if (hipMalloc(&h->psic, sizeof(double) * psic_per(h) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
+  if (hipMalloc(&h->kb, sizeof(double) * kb_per(h) * batch_max) != hipSuccess)
     return fail(PQ_ERR_NOMEM);
   if (hipMalloc(&h->status, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipMemset(h->status, 0, sizeof(int) * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
-  if (hipMemset(h->psic, 0, sizeof(double) * (size_t)L * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
+  if (hipMemset(h->psic, 0, sizeof(double) * psic_per(h) * batch_max) != hipSuccess) return fail(PQ_ERR_HIP);
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PQ_ERR_HIP);
   *out = h;
   return PQ_OK;
@@ -355,19 +514,22 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   InvArgs a{};
   a.scr = static_cast<char *>(h->scr) + h->scr_member * mem0;
   a.scr_stride = (long long)h->scr_member;
-  a.part = h->part + (size_t)h->part_cap * L * mem0;
-  a.part_stride = h->part_cap * L;
-  a.psic = h->psic + (size_t)L * mem0;
-  a.psic_stride = L;
+  a.part = h->part + part_per(h) * mem0;
+  a.part_stride = (int)part_per(h);
+  a.psic = h->psic + psic_per(h) * mem0;
+  a.psic_stride = (int)psic_per(h);
   a.status = h->status + mem0;
-  a.kb = h->kb + (size_t)(L + L * L) * mem0;
-  a.kb_stride = L + L * L;
+  a.kb = h->kb + kb_per(h) * mem0;
+  a.kb_stride = (int)kb_per(h);
   a.tw64 = h->tw64;
   a.was = h->was;
   a.lam4 = h->lam4;
   a.shat = h->shat;
+  a.ctab = h->ctab;
   a.L = L;
   a.mode = h->mode;
+  a.per = h->per;
+  a.twm = h->twm;
   a.R = h->R;
   a.lgR = ilog2i(h->R);
   const int rows_threads = (h->R / 2) * e.tpf;
@@ -378,15 +540,16 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
     size_t lds;
     if (pass == 1) {
       blocks = pt.cum_c[nb];
-      threads = col_groups * e.tpf;
-      lds = inv_lds_bytes(e, 1, col_groups);
+      threads = col_groups * (h->per ? e.ctpf : e.tpf);
+      lds = inv_lds_bytes(e, 1, col_groups, h->per);
     } else {
       blocks = pt.cum_r[nb];
       threads = rows_threads;
       lds = inv_lds_bytes(e, pass, h->R / 2);
     }
     if (pass == 2 && h->mode == 0) {
-      hipLaunchKernelGGL(k_psic, dim3(nb), dim3(256), 0, s, a, pt, N);
+      if (h->per) hipLaunchKernelGGL(k_chan_corr, dim3(nb), dim3(256), 0, s, a, pt, N);
+      else hipLaunchKernelGGL(k_psic, dim3(nb), dim3(256), 0, s, a, pt, N);
       hipError_t e2 = hipGetLastError();
       if (e2 != hipSuccess) {
         pq_set_error("pq_inv: launch of the psiC reduction failed: %s", hipGetErrorString(e2));
@@ -410,8 +573,8 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
                       const double *const *coef = nullptr, const int *layers = nullptr) {
   PQ_CHECK_ARG(batch >= 1 && batch <= h->batch_max, "pq_inv_run_dev: batch %d outside 1..%d", batch,
                h->batch_max);
-  PQ_CHECK_ARG(ldz1 >= h->g && ldz2 >= h->g && ldp1 >= h->g && ldp2 >= h->g,
-               "pq_inv_run_dev: leading dimensions smaller than g=%d", h->g);
+  PQ_CHECK_ARG(ldz1 >= h->g && ldz2 >= h->gj && ldp1 >= h->g && ldp2 >= h->gj,
+               "pq_inv_run_dev: leading dimensions smaller than the view %d x %d", h->g, h->gj);
   for (int b = 0; b < batch; ++b) {
     PQ_CHECK_ARG(zeta[b] != nullptr && psi[b] != nullptr, "pq_inv_run_dev: null member pointer %d", b);
     PQ_CHECK_ARG(!layers || (layers[b] >= 1 && layers[b] <= h->L),
@@ -439,7 +602,7 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
       }
       pt.lay[b] = (unsigned char)Lb;
       pt.lgc[b] = (unsigned char)ilog2i(Cb);
-      pt.cum_r[b + 1] = pt.cum_r[b] + Lb * (h->N / h->R);
+      pt.cum_r[b + 1] = pt.cum_r[b] + Lb * ((h->per ? 2 * h->N : h->N) / h->R);
       pt.cum_c[b + 1] = pt.cum_c[b] + h->N / Cb;
     }
     pt.ldz1 = ldz1;
@@ -474,9 +637,10 @@ int inv_run_members(pq_inv *h, int batch, const void *const *zeta, long ldz1, lo
 
 size_t inv_device_bytes(const pq_inv *h) {
   const size_t N = (size_t)h->N, L = (size_t)h->L, B = (size_t)h->batch_max;
-  return h->scr_member * B + sizeof(double) * (h->coef_size * B + (size_t)h->part_cap * L * B +
-                                               L * B + (L + L * L) * B + 2 * (size_t)h->ei.M + N) +
-         (size_t)h->ei.celem * (N / 2 + N) + sizeof(int) * B;
+  return h->scr_member * B +
+         sizeof(double) * (h->coef_size * B + part_per(h) * B + psic_per(h) * B + kb_per(h) * B +
+                           2 * (size_t)h->twm + N + (h->per ? 3 * N : 0)) +
+         (size_t)h->ei.celem * (N / 2 + N + 1) + sizeof(int) * B;
 }
 }  // namespace pq
 
@@ -487,6 +651,18 @@ int pq_inv_create(pq_inv **h, int g, int layers, const double *ss, const double 
   return inv_create(h, g, layers, ss, modes_to_z, z_to_modes, precision, batch_max, 0);
 }
 
+int pq_chan_create(pq_inv **h, int gi, int gj, int layers, const double *ss,
+                   const double *modes_to_z, const double *z_to_modes, int bc_type, double alpha,
+                   int precision, int batch_max) {
+  PQ_CHECK_ARG(gj == 2 * (gi - 1), "pq_chan_create: the period must be gj = 2 (gi - 1) (gi=%d, gj=%d)",
+               gi, gj);
+  PQ_CHECK_ARG(bc_type >= 1 && bc_type <= 3, "pq_chan_create: bad bc_type %d", bc_type);
+  PQ_CHECK_ARG(bc_type != 1 || (alpha >= 0.0 && std::isfinite(alpha)),
+               "pq_chan_create: alpha must be finite and >= 0");
+  return inv_create(h, gi, layers, ss, modes_to_z, z_to_modes, precision, batch_max, 0, 1, bc_type,
+                    alpha);
+}
+
 int pq_inv_destroy(pq_inv *h) {
   if (!h) return PQ_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -494,6 +670,7 @@ int pq_inv_destroy(pq_inv *h) {
   (void)hipFree(h->was);
   (void)hipFree(h->lam4);
   (void)hipFree(h->shat);
+  (void)hipFree(h->ctab);
   (void)hipFree(h->coef);
   (void)hipFree(h->scr);
   (void)hipFree(h->part);
@@ -571,7 +748,7 @@ int pq_inv_status(pq_inv *h, double *psic) {
   PQ_HIP(hipDeviceSynchronize());
   std::vector<int> st(h->batch_max);
   PQ_HIP(hipMemcpy(st.data(), h->status, sizeof(int) * h->batch_max, hipMemcpyDeviceToHost));
-  if (psic) PQ_HIP(hipMemcpy(psic, h->psic, sizeof(double) * h->L * h->batch_max, hipMemcpyDeviceToHost));
+  if (psic) PQ_HIP(hipMemcpy(psic, h->psic, sizeof(double) * psic_per(h) * h->batch_max, hipMemcpyDeviceToHost));
   int worst = 0;
   for (int v : st) worst |= v;
   PQ_HIP(hipMemset(h->status, 0, sizeof(int) * h->batch_max));
@@ -590,50 +767,50 @@ int pq_inv_status(pq_inv *h, double *psic) {
 
 // Host-view copy helpers: g x g x L block of a (ld1, ld2) host view <-> a
 // packed device array of FP64 (el 8) or FP32 (el 4).
-static int copy_view_h2d(void *dev, const double *host, int g, int L, long ld1, long ld2, size_t el,
-                         hipStream_t s) {
+static int copy_view_h2d(void *dev, const double *host, int gi, int gj, int L, long ld1, long ld2,
+                         size_t el, hipStream_t s) {
   if (el == 8) {
     for (int l = 0; l < L; ++l)
-      PQ_HIP(hipMemcpy2DAsync((char *)dev + (size_t)l * g * g * 8, (size_t)g * 8,
-                              host + (size_t)l * ld1 * ld2, (size_t)ld1 * 8, (size_t)g * 8, g,
+      PQ_HIP(hipMemcpy2DAsync((char *)dev + (size_t)l * gi * gj * 8, (size_t)gi * 8,
+                              host + (size_t)l * ld1 * ld2, (size_t)ld1 * 8, (size_t)gi * 8, gj,
                               hipMemcpyHostToDevice, s));
     PQ_HIP(hipStreamSynchronize(s));
     return PQ_OK;
   }
-  std::vector<float> tmp((size_t)g * g * L);
+  std::vector<float> tmp((size_t)gi * gj * L);
   for (int l = 0; l < L; ++l)
-    for (int j = 0; j < g; ++j)
-      for (int i = 0; i < g; ++i)
-        tmp[((size_t)l * g + j) * g + i] = (float)host[i + ld1 * (j + ld2 * (size_t)l)];
+    for (int j = 0; j < gj; ++j)
+      for (int i = 0; i < gi; ++i)
+        tmp[((size_t)l * gj + j) * gi + i] = (float)host[i + ld1 * (j + ld2 * (size_t)l)];
   PQ_HIP(hipMemcpyAsync(dev, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, s));
   PQ_HIP(hipStreamSynchronize(s));
   return PQ_OK;
 }
 
-static int copy_view_d2h(double *host, const void *dev, int g, int L, long ld1, long ld2, size_t el,
-                         hipStream_t s) {
+static int copy_view_d2h(double *host, const void *dev, int gi, int gj, int L, long ld1, long ld2,
+                         size_t el, hipStream_t s) {
   if (el == 8) {
     for (int l = 0; l < L; ++l)
       PQ_HIP(hipMemcpy2DAsync(host + (size_t)l * ld1 * ld2, (size_t)ld1 * 8,
-                              (const char *)dev + (size_t)l * g * g * 8, (size_t)g * 8, (size_t)g * 8, g,
-                              hipMemcpyDeviceToHost, s));
+                              (const char *)dev + (size_t)l * gi * gj * 8, (size_t)gi * 8,
+                              (size_t)gi * 8, gj, hipMemcpyDeviceToHost, s));
     PQ_HIP(hipStreamSynchronize(s));
     return PQ_OK;
   }
-  std::vector<float> tmp((size_t)g * g * L);
+  std::vector<float> tmp((size_t)gi * gj * L);
   PQ_HIP(hipMemcpyAsync(tmp.data(), dev, tmp.size() * 4, hipMemcpyDeviceToHost, s));
   PQ_HIP(hipStreamSynchronize(s));
   for (int l = 0; l < L; ++l)
-    for (int j = 0; j < g; ++j)
-      for (int i = 0; i < g; ++i) host[i + ld1 * (j + ld2 * (size_t)l)] = tmp[((size_t)l * g + j) * g + i];
+    for (int j = 0; j < gj; ++j)
+      for (int i = 0; i < gi; ++i) host[i + ld1 * (j + ld2 * (size_t)l)] = tmp[((size_t)l * gj + j) * gi + i];
   return PQ_OK;
 }
 
 static int run_host(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *psi, int ldp1, int ldp2) {
   PQ_CHECK_ARG(zeta && psi, "pq_inv_run: null array");
-  PQ_CHECK_ARG(ldz1 >= h->g && ldz2 >= h->g && ldp1 >= h->g && ldp2 >= h->g,
-               "pq_inv_run: leading dimensions smaller than g=%d", h->g);
-  const int g = h->g, L = h->L;
+  PQ_CHECK_ARG(ldz1 >= h->g && ldz2 >= h->gj && ldp1 >= h->g && ldp2 >= h->gj,
+               "pq_inv_run: leading dimensions smaller than the view %d x %d", h->g, h->gj);
+  const int g = h->g, gj = h->gj, L = h->L;
   // FP64 storage for FP64 and mixed handles, FP32 for FP32 handles
   const size_t el = (h->prec == PQ_PREC_FP64 || h->prec == PQ_PREC_MIXED) ? 8 : 4;
   void *mz = pq_mirror_device_ptr(zeta);
@@ -645,12 +822,12 @@ static int run_host(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *p
     if (rc) return rc;
     return pq_inv_status(h, nullptr);
   }
-  const size_t bytes = (size_t)g * g * L * el;
+  const size_t bytes = (size_t)g * gj * L * el;
   if (!h->dz) {
     PQ_HIP(hipMalloc(&h->dz, bytes));
     PQ_HIP(hipMalloc(&h->dp, bytes));
   }
-  int rc = copy_view_h2d(h->dz, zeta, g, L, ldz1, ldz2, el, h->stream);
+  int rc = copy_view_h2d(h->dz, zeta, g, gj, L, ldz1, ldz2, el, h->stream);
   if (rc) return rc;
   if (h->mode == 1) {
     // helm works in place: the output keeps input values on part of the
@@ -659,9 +836,9 @@ static int run_host(pq_inv *h, const double *zeta, int ldz1, int ldz2, double *p
   }
   const void *zt[1] = {h->dz};
   void *pt[1] = {h->dp};
-  rc = inv_launch(h, 1, zt, g, g, pt, g, g, h->stream);
+  rc = inv_launch(h, 1, zt, g, gj, pt, g, gj, h->stream);
   if (rc) return rc;
-  rc = copy_view_d2h(psi, h->dp, g, L, ldp1, ldp2, el, h->stream);
+  rc = copy_view_d2h(psi, h->dp, g, gj, L, ldp1, ldp2, el, h->stream);
   if (rc) return rc;
   return pq_inv_status(h, nullptr);
 }
@@ -703,6 +880,20 @@ int pq_helm_create(pq_helm **out, int ii, int jj, int layers, const double *ss, 
   PQ_CHECK_ARG(ii == jj, "pq_helm_create: only square closed basins are supported (ii=%d, jj=%d)", ii, jj);
   pq_inv *inv = nullptr;
   int rc = inv_create(&inv, ii, layers, ss, nullptr, nullptr, precision, 1, 1);
+  if (rc) return rc;
+  pq_helm *h = new pq_helm();
+  h->inv = inv;
+  *out = h;
+  return PQ_OK;
+}
+
+int pq_helm_create_periodic(pq_helm **out, int ii, int jj, int layers, const double *ss,
+                            int precision) {
+  PQ_CHECK_ARG(out != nullptr, "pq_helm_create_periodic: null handle pointer");
+  PQ_CHECK_ARG(jj == 2 * (ii - 1),
+               "pq_helm_create_periodic: the period must be jj = 2 (ii - 1) (ii=%d, jj=%d)", ii, jj);
+  pq_inv *inv = nullptr;
+  int rc = inv_create(&inv, ii, layers, ss, nullptr, nullptr, precision, 1, 1, 1);
   if (rc) return rc;
   pq_helm *h = new pq_helm();
   h->inv = inv;

@@ -53,13 +53,14 @@ class HelmCache:
     """allocate(helm_c, decomp, ss) / helm(helm_c, zeta) / deallocate(helm_c)
     (qg3p_helmholtz.F90:292, 614, 389)."""
 
-    def __init__(self, ii: int, jj: int, ss, precision: str = "fp64"):
+    def __init__(self, ii: int, jj: int, ss, precision: str = "fp64", periodic: bool = False):
+        """periodic: the channel's decomposition (periodic in j, jj = 2 (ii - 1))."""
         self.ss = np.ascontiguousarray(ss, dtype=np.float64)
         self.g = ii
         self.layers = len(self.ss)
         h = _vp()
-        check(load().pq_helm_create(ctypes.byref(h), ii, jj, self.layers, _np_ptr(self.ss),
-                                    PREC[precision]))
+        create = load().pq_helm_create_periodic if periodic else load().pq_helm_create
+        check(create(ctypes.byref(h), ii, jj, self.layers, _np_ptr(self.ss), PREC[precision]))
         self._h = h
         self.precision = precision
 
@@ -177,6 +178,41 @@ class HelmholtzCache:
             self._h = None
 
     __del__ = deallocate
+
+
+class ChannelCache(HelmholtzCache):
+    """helmholtz_cache of the periodic channel (qg3p_aux.F90:152-193,
+    233-286): gi x gj nodes with gj = 2 (gi - 1), walls at i = 1 and i = gi,
+    periodic in j; the baroclinic modes get the momentum-conserving
+    correction for the velocity boundary condition bc_type (BC_PARTIAL_SLIP
+    with alpha, BC_NO_SLIP, BC_FREE_SLIP)."""
+
+    def __init__(self, gi: int, strat: Stratification, bc_type: int = 1, alpha: float = 0.0,
+                 precision: str = "fp64", batch_max: int = 1, tiles=None):
+        self.g = gi
+        self.gj = 2 * (gi - 1)
+        self.strat = strat
+        self.layers = strat.layers
+        self.batch_max = batch_max
+        ss = np.ascontiguousarray(strat.ss, dtype=np.float64)
+        m2z = np.asfortranarray(strat.modes_to_z, dtype=np.float64)
+        z2m = np.asfortranarray(strat.z_to_modes, dtype=np.float64)
+        if precision == "fp32auto":
+            precision = _lib.PREC_NAME[load().pq_prec_fp32_choice(self.layers, _np_ptr(m2z), _np_ptr(z2m))]
+        self.precision = precision
+        h = _vp()
+        check(load().pq_chan_create(ctypes.byref(h), gi, self.gj, self.layers, _np_ptr(ss),
+                                    _np_ptr(m2z), _np_ptr(z2m), bc_type, alpha, PREC[precision],
+                                    batch_max))
+        self._h = h
+        if tiles is not None and not self.set_tiles(*tiles):
+            raise ValueError(f"tile shape {tiles} not supported for gi={gi}, L={self.layers}")
+
+    def status(self):
+        """(status, psiC pairs (batch, L, 2); the barotropic mode's are 0)."""
+        psic = np.zeros(2 * self.layers * self.batch_max)
+        rc = load().pq_inv_status(self._h, _np_ptr(psic))
+        return rc, psic.reshape(self.batch_max, self.layers, 2)
 
 
 def pv_inversion(zeta, hcache: HelmholtzCache, psi) -> None:

@@ -448,3 +448,185 @@ def test_psi_av_add_is_dt_weighted_sum():
         O.psi_av_add(psi, av, dt)
         ref = ref + dt * psi[1:g + 1, 1:g + 1, :]
     assert np.array_equal(av, ref)
+
+
+# ------------------------------------------------------------ periodic channel
+def _per_order_errors(ss, gs):
+    m = []
+    for gi in gs:
+        gj = 2 * (gi - 1)
+        i = np.arange(1, gi + 1)[:, None]
+        j = np.arange(1, gj + 1)[None, :]
+        x = (j - 1) / gj
+        y = (gi - i) / (gi - 1)
+        s = np.cos(2 * np.pi * x) * np.sin(np.pi * y)
+        if ss == 0.0:
+            phi = -np.pi ** 2 * s * (1.0 / (gi - 1) ** 2 + 4.0 / gj ** 2)
+        else:
+            phi = -s * (np.pi ** 2 / (gi - 1) ** 2 + 4 * np.pi ** 2 / gj ** 2 + ss)
+        z = np.asfortranarray(phi[:, :, None])
+        O.PeriodicHelm(gi, gj, [ss]).solve(z)
+        z[0, :, 0] = 0.0
+        z[-1, :, 0] = 0.0
+        m.append(np.max(np.abs(z[:, :, 0] - s)))
+    return np.array(m)
+
+
+def test_periodic_helm_convergence_orders():
+    """unittest/test_helmholtz.F90:248-356: the channel (periodic in j,
+    g_jj = 2 (g_ii - 1)) Poisson order >= 2 and Helmholtz (ss = 0.5)
+    order >= 3.9."""
+    ka = KA["helmholtz_periodic_order"]
+    gs = ka["g"][:6]
+    for ss, lim in [(0.0, ka["poisson_min_order"]), (ka["helmholtz_ss"], ka["helmholtz_min_order"])]:
+        m = _per_order_errors(ss, gs)
+        o = -np.log10(m[1:] / m[:-1]) / np.log10(np.array(gs[1:]) / np.array(gs[:-1]))
+        assert np.all(o >= lim), (ss, o)
+
+
+@pytest.mark.parametrize("gi,gj", [(9, 16), (17, 32), (12, 7)])
+@pytest.mark.parametrize("ss", [0.0, 0.7])
+def test_periodic_helm_eigenfunctions(gi, gj, ss):
+    """sin(p pi (i-1) / (gi-1)) cos / sin(2 pi q (j-1) / gj) are exact discrete
+    eigenfunctions: the solve divides by lambda_p + mu_q - ss."""
+    N = gi - 1
+    i = np.arange(gi)[:, None]
+    j = np.arange(gj)[None, :]
+    lam = -4 * np.sin(np.arange(N) * np.pi / (2 * N)) ** 2
+    for p, q, f in [(1, 0, np.cos), (2, 1, np.cos), (N - 1, 3, np.sin), (3, gj // 2, np.cos)]:
+        rhs = np.sin(np.pi * p * i / N) * f(2 * np.pi * q * j / gj)
+        z = np.asfortranarray(rhs[:, :, None].copy())
+        O.PeriodicHelm(gi, gj, [ss]).solve(z)
+        z[0, :, 0] = z[-1, :, 0] = 0.0
+        mu = -4 * np.sin(np.pi * q / gj) ** 2
+        exp = rhs / (lam[p] + mu - ss)
+        assert np.max(np.abs(z[:, :, 0] - exp)) < 1e-13 * max(1.0, np.max(np.abs(exp))), (p, q)
+
+
+def _periodic_dense(gi, gj, ss):
+    """Dense 5-point operator (Delta - ss) on the interior rows 2..gi-1,
+    periodic in j, zero on the walls: unknown (i, j) at (i - 2) + (gi - 2) j."""
+    n = gi - 2
+    A = np.zeros((n * gj, n * gj))
+    for j in range(gj):
+        for i in range(n):
+            k = i + n * j
+            A[k, k] = -4.0 - ss
+            if i > 0:
+                A[k, k - 1] = 1.0
+            if i < n - 1:
+                A[k, k + 1] = 1.0
+            A[k, i + n * ((j + 1) % gj)] += 1.0
+            A[k, i + n * ((j - 1) % gj)] += 1.0
+    return A
+
+
+@pytest.mark.parametrize("gi,gj,ss", [(6, 7, 0.0), (7, 8, 0.4), (5, 3, 1.3)])
+def test_periodic_helm_brute_force(gi, gj, ss):
+    rng = np.random.default_rng(gi * gj)
+    rhs = rng.standard_normal((gi, gj))
+    z = np.asfortranarray(rhs[:, :, None].copy())
+    O.PeriodicHelm(gi, gj, [ss]).solve(z)
+    ref = np.linalg.solve(_periodic_dense(gi, gj, ss), rhs[1:-1, :].reshape(-1, order="F"))
+    got = z[1:-1, :, 0].reshape(-1, order="F")
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+    # the wall rows: i = gi is zeroed, i = 1 keeps the input (helm's output
+    # boundary semantics, qg3p_helmholtz.F90:757-761)
+    assert np.all(z[-1, :, 0] == 0.0) and np.array_equal(z[0, :, 0], rhs[0, :])
+
+
+def _channel_case(gi, gj, L, seed):
+    rng = np.random.default_rng(seed)
+    s1, s2 = random_strat(L, rng)
+    st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
+    z = np.asfortranarray(rng.standard_normal((gi, gj, L)))
+    return st, z
+
+
+def _lap(p):
+    """5-point Laplacian on the interior rows, periodic in j."""
+    return (p[2:, :] + p[:-2, :] + np.roll(p[1:-1, :], 1, axis=1) + np.roll(p[1:-1, :], -1, axis=1)
+            - 4 * p[1:-1, :])
+
+
+@pytest.mark.parametrize("bc_type,alpha", [(1, 0.2), (2, 0.0), (3, 0.0)])
+def test_channel_inversion_invariants(bc_type, alpha):
+    """Channel pv_inversion: (i) each mode solves (Delta - ss_m) phi_m =
+    (z_to_modes zeta)_m on the interior; (ii) every baroclinic mode's mean
+    boundary relative PV vanishes on both walls (the momentum constraint,
+    qg3p_aux.F90:233-286, 524-575); (iii) the barotropic mode is zero on
+    the walls and every mode is constant along each wall."""
+    gi, gj, L = 13, 24, 3
+    st, z = _channel_case(gi, gj, L, 5)
+    ch = O.Channel(gi, gj, st.ss, st.modes_to_z, st.z_to_modes, bc_type, alpha)
+    psi, rc = ch.run(z)
+    assert rc == 0
+    phi = np.einsum("ml,ijl->ijm", st.z_to_modes, psi)
+    zm = np.einsum("ml,ijl->ijm", st.z_to_modes, z)
+    scale = np.max(np.abs(zm))
+    for m in range(L):
+        r = _lap(phi[:, :, m]) - st.ss[m] * phi[1:-1, :, m] - zm[1:-1, :, m]
+        assert np.max(np.abs(r)) < 1e-11 * scale, m
+        for w in (0, -1):
+            assert np.ptp(phi[w, :, m]) < 1e-13 * np.max(np.abs(phi))
+    assert np.max(np.abs(phi[[0, -1], :, 0])) < 1e-14 * np.max(np.abs(phi))
+    zb, zd = O.zeta_boundary(phi, st.ss, bc_type, alpha)
+    assert np.max(np.abs(zb[1:])) < 1e-12 * scale and np.max(np.abs(zd[1:])) < 1e-12 * scale
+    if bc_type != 3:  # free slip: the homogeneous solution already satisfies it
+        assert np.any(np.abs(ch.psic()) > 1e-3 * scale)  # the correction does something
+
+
+def test_channel_correction_functions_closed_form():
+    """The correction functions are uniform along the channel and solve the
+    1D problem c(i-1) + c(i+1) - (2 + ss) c(i) = 0 between the walls:
+    c1 = sinh(k (gi - i)) / sinh(k (gi - 1)), cosh k = 1 + ss / 2."""
+    gi, gj, L = 17, 8, 3
+    st, _ = _channel_case(gi, gj, L, 2)
+    ch = O.Channel(gi, gj, st.ss, st.modes_to_z, st.z_to_modes, 1, 0.5)
+    corr = ch.corr()
+    i = np.arange(1, gi + 1)
+    for m in range(1, L):
+        k = np.arccosh(1 + st.ss[m] / 2)
+        c1 = np.sinh(k * (gi - i)) / np.sinh(k * (gi - 1))
+        c2 = np.sinh(k * (i - 1)) / np.sinh(k * (gi - 1))
+        assert np.max(np.abs(corr[:, :, m, 0] - c1[:, None])) < 1e-13
+        assert np.max(np.abs(corr[:, :, m, 1] - c2[:, None])) < 1e-13
+
+
+def test_channel_brute_force():
+    """The whole constrained problem as one dense system per baroclinic
+    mode on a tiny channel: interior Helmholtz unknowns plus the two wall
+    values, with the two boundary constraints (partial slip)."""
+    gi, gj, L, alpha = 6, 5, 2, 0.7
+    st, z = _channel_case(gi, gj, L, 8)
+    psi, rc = O.Channel(gi, gj, st.ss, st.modes_to_z, st.z_to_modes, 1, alpha).run(z)
+    phi = np.einsum("ml,ijl->ijm", st.z_to_modes, psi)
+    zm = np.einsum("ml,ijl->ijm", st.z_to_modes, z)
+    n = gi - 2
+    cff = alpha / (alpha + 3.0)
+    for m in range(L):
+        ss = st.ss[m]
+        A0 = _periodic_dense(gi, gj, ss)
+        nu = n * gj + (2 if m > 0 else 0)
+        A = np.zeros((nu, nu))
+        A[:n * gj, :n * gj] = A0
+        rhs = np.zeros(nu)
+        rhs[:n * gj] = zm[1:-1, :, m].reshape(-1, order="F")
+        if m > 0:
+            top, bot = n * gj, n * gj + 1
+            for j in range(gj):
+                A[0 + n * j, top] += 1.0          # wall value enters row i = 2
+                A[n - 1 + n * j, bot] += 1.0      # and row i = gi - 1
+                # mean boundary PV: cff (-3.5 w + 4 phi_2 - 0.5 phi_3) - ss w
+                A[top, 0 + n * j] += 4.0 * cff / gj
+                A[top, 1 + n * j] += -0.5 * cff / gj
+                A[bot, n - 1 + n * j] += 4.0 * cff / gj
+                A[bot, n - 2 + n * j] += -0.5 * cff / gj
+            A[top, top] = -3.5 * cff - ss
+            A[bot, bot] = -3.5 * cff - ss
+        x = np.linalg.solve(A, rhs)
+        ref = np.zeros((gi, gj))
+        ref[1:-1, :] = x[:n * gj].reshape((n, gj), order="F")
+        if m > 0:
+            ref[0, :], ref[-1, :] = x[n * gj], x[n * gj + 1]
+        assert np.max(np.abs(phi[:, :, m] - ref)) < 1e-12 * np.max(np.abs(ref)), m
