@@ -41,6 +41,15 @@ module pequod_hip_c
       integer(c_int) :: rc
     end function pq_helm_create
 
+    function pq_helm_create_periodic(h, ii, jj, layers, ss, precision) &
+      & bind(C, name = "pq_helm_create_periodic") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: ii, jj, layers, precision
+      real(c_double), dimension(*), intent(in) :: ss
+      integer(c_int) :: rc
+    end function pq_helm_create_periodic
+
     function pq_helm_destroy(h) bind(C, name = "pq_helm_destroy") result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: h
@@ -65,6 +74,17 @@ module pequod_hip_c
       real(c_double), dimension(*), intent(in) :: ss, modes_to_z, z_to_modes
       integer(c_int) :: rc
     end function pq_inv_create
+
+    ! the periodic channel (qg3p_aux.F90:152-193, 233-379, 452-479, 524-575)
+    function pq_chan_create(h, gi, gj, layers, ss, modes_to_z, z_to_modes, &
+      & bc_type, alpha, precision, batch_max) bind(C, name = "pq_chan_create") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: gi, gj, layers, bc_type, precision, batch_max
+      real(c_double), dimension(*), intent(in) :: ss, modes_to_z, z_to_modes
+      real(c_double), value :: alpha
+      integer(c_int) :: rc
+    end function pq_chan_create
 
     function pq_inv_destroy(h) bind(C, name = "pq_inv_destroy") result(rc)
       import :: c_ptr, c_int
