@@ -380,7 +380,7 @@ def main():
         fp64_rate = B * max(20, args.steps // 4) / w64
 
     # device-resident FP64 CABARET model (same grid): steps/s
-    cab = None
+    cab = lfr = None
     if not args.no_cabaret:
         nd = c.nondim()
         m = qg.Model(g, st, nd["beta"], c.wind(), viscosity=nd["viscosity"],
@@ -393,6 +393,18 @@ def main():
         m.step(nst)
         m.get("zeta")  # synchronises the model stream
         cab = nst / (time.perf_counter() - t0)
+        m.close()
+        # the centred (leapfrog) scheme on the same grid
+        m = qg.Model(g, st, nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                     visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1, alpha=nd["alpha"],
+                     scheme="leapfrog")
+        m.init(q)
+        m.step(5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.step(nst)
+        m.get("zeta")
+        lfr = nst / (time.perf_counter() - t0)
         m.close()
 
     # 64-member mixed ensemble (sizes {255, 511, 767, 1023}^2, 2..8 layers,
@@ -459,6 +471,7 @@ def main():
         "fp64_dropin_inversions_per_s": None if fp64_rate is None else round(fp64_rate, 1),
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
+        "leapfrog_fp64_steps_per_s": None if lfr is None else round(lfr, 2),
         "ensemble_64_mixed": ens,
         "channel": chan,
         "serving": serving,

@@ -230,6 +230,19 @@ int pq_rel_from_zeta_and_psi_dev(int g, int layers, const double *s1,
                                  const double *s2, int bc_type, double alpha,
                                  const double *zeta, const double *psi,
                                  double *rel, void *stream);
+/* The centred scheme's step (zeta_centred, qg3p_aux.F90:2027-2202, and
+ * zeta_tendency / zeta_forward_euler, qg3p_aux.F90:1789-2024): dst = src +
+ * T F(psi, rel, rel_p) on the interior nodes (src may equal dst); beta[L]
+ * (beta_y formed inside), force_wind (G, G).                              */
+int pq_zeta_step_dev(int g, int layers, double T, const double *beta, const double *s1,
+                     const double *s2, const double *force_wind, double viscosity,
+                     double visc_bot, const double *psi, const double *rel,
+                     const double *rel_p, const double *src, double *dst, void *stream);
+/* robert_asselin_filter with Williams' modification (nu = 0.02, alpha =
+ * 0.5; qg3p_aux.F90:1716-1786): phi_l level n+1, phi_m level n, phi_old_l
+ * level n-1; phi_old_m receives the filtered level n.                      */
+int pq_ra_filter_dev(int g, int layers, double *phi_l, double *phi_m, const double *phi_old_l,
+                     double *phi_old_m, void *stream);
 /* update_zeta_bc (qg3p_aux.F90:643-734). */
 int pq_update_zeta_bc_dev(int g, int layers, const double *s1,
                           const double *s2, const double *rel,
@@ -240,6 +253,12 @@ int pq_update_zeta_bc_dev(int g, int layers, const double *s1,
  * (qg3.F90:590-967), with the PV inversion, for state that stays in HBM.
  * ==================================================================== */
 typedef struct pq_model pq_model;
+/* Time stepping scheme of a model: the CABARET scheme (the reference's
+ * default build), or the centred (leapfrog) scheme with the
+ * Robert-Asselin-Williams filter and a forward Euler start, the driver built
+ * without CABARET (qg3.F90:456-490, 612-660).                              */
+#define PQ_SCHEME_CABARET 0
+#define PQ_SCHEME_LEAPFROG 1
 typedef struct {
   int g, layers;
   const double *s1, *s2;   /* [L] stratification (non-dimensional)        */
@@ -253,6 +272,7 @@ typedef struct {
   double target_cfl, max_dt, max_dt_increase;
   double dt;               /* initial (non-dimensional) time step          */
   int precision;           /* PQ_PREC_* of the inversion                   */
+  int scheme;              /* PQ_SCHEME_* (adapt_dt needs CABARET)         */
 } pq_model_params;
 
 int pq_model_create(pq_model **m, const pq_model_params *p);
@@ -262,11 +282,16 @@ int pq_model_destroy(pq_model *m);
  * CABARET work arrays zeroed, zeta_old = zeta, model_time = dt/2.         */
 int pq_model_init(pq_model *m, const double *zeta_sdata_host);
 /* Run nsteps full steps; if final_partial, the last step stops after the
- * flux stage (the driver's exit point, qg3.F90:815-946).                  */
+ * flux stage (the driver's exit point, qg3.F90:815-946).  A centred-scheme
+ * model runs nsteps leapfrog steps of dt, alternating the time level they
+ * write (1, 2, 1, ...; pq_model_init leaves level 1 next, after the
+ * forward Euler start into level 2); final_partial must be 0.             */
 int pq_model_step(pq_model *m, int nsteps, int final_partial);
 /* Copy a field to the host: which = 0 psi, 1 zeta, 2 rel, 3 zeta_old,
  * 4 ui, 5 uj, 6 zeta_fi, 7 zeta_fj, 8 res, 9 res_beta, 10 uimo, 11 ujmo,
- * 12 flux_i, 13 flux_j.  Sizes follow the reference shapes.               */
+ * 12 flux_i, 13 flux_j (CABARET); a centred-scheme model has 0-3 for time
+ * level 1 and 14 psi, 15 zeta, 16 rel, 17 zeta_old for level 2.  Sizes
+ * follow the reference shapes.                                            */
 int pq_model_get(pq_model *m, int which, double *host, size_t count);
 int pq_model_set(pq_model *m, int which, const double *host, size_t count);
 /* dt, tweight_a, tweight_u, model_time */

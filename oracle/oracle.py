@@ -57,6 +57,8 @@ def lib():
             "pqo_chan_psic": (_dp, [vp]),
             "pqo_zeta_boundary": (None, [i, i, i, _dp, i, d, _dp, _dp, _dp]),
             "pqo_chan_run": (i, [vp, _dp, i, i, _dp, i, i]),
+            "pqo_zeta_step": (None, [i, i, d, _dp, _dp, _dp, _dp, d, d, _dp, _dp, _dp, _dp, _dp]),
+            "pqo_ra_filter": (None, [i, i, _dp, _dp, _dp, _dp]),
             "pqo_psi_av_add": (None, [i, i, i, d, _dp, _dp]),
             "pqo_strat_modes": (i, [i, _dp, _dp, _dp, _dp, _dp]),
             "pqo_strat_l2": (None, [d, d, d, _dp, _dp]),
@@ -475,3 +477,67 @@ class Model:
 
     def energy(self, h):
         return energy(self.a["psi"], h, self.s2)
+
+
+def zeta_step(T, beta, s1, s2, force_wind, viscosity, visc_bot, psi, rel, rel_p, src, dst):
+    """dst = src + T F (zeta_centred / zeta_forward_euler, qg3p_aux.F90:1789-2202)."""
+    g, L = _g_L_node(dst)
+    lib().pqo_zeta_step(g, L, T, _p(f64(beta)), _p(f64(s1)), _p(f64(s2)), _p(f64(force_wind)),
+                        viscosity, visc_bot, _p(psi), _p(rel), _p(rel_p), _p(src), _p(dst))
+
+
+def ra_filter(phi_l, phi_m, phi_old_l, phi_old_m):
+    """Robert-Asselin-Williams filter in place (qg3p_aux.F90:1716-1786)."""
+    g, L = _g_L_node(phi_l)
+    lib().pqo_ra_filter(g, L, _p(phi_l), _p(phi_m), _p(phi_old_l), _p(phi_old_m))
+
+
+class LFState(ctypes.Structure):
+    _fields_ = [("g", ctypes.c_int), ("layers", ctypes.c_int),
+                ("psi", _dp * 2), ("rel", _dp * 2), ("zeta", _dp * 2), ("zeta_old", _dp * 2),
+                ("force_wind", _dp), ("dt", ctypes.c_double), ("model_time", ctypes.c_double),
+                ("leapfrog", ctypes.c_int)]
+
+
+class LeapfrogModel:
+    """The reference's time loop without CABARET (centred scheme with the
+    Robert-Asselin-Williams filter, forward Euler start; qg3.F90:456-490,
+    612-660) on one domain, driving the oracle's stages."""
+
+    NAMES = ("psi", "rel", "zeta", "zeta_old")
+
+    def __init__(self, g, strat_s1, strat_s2, ss, m2z, z2m, beta, force_wind, *, viscosity,
+                 visc_bot, dt, bc_type=1, alpha=0.0):
+        L = len(ss)
+        self.g, self.layers = g, L
+        self.a = {f"{n}{k}": np.zeros((g + 2, g + 2, L), order="F") for n in self.NAMES
+                  for k in (1, 2)}
+        self.a["force_wind"] = f64(force_wind)
+        self.s1, self.s2, self.beta = f64(strat_s1), f64(strat_s2), f64(beta)
+        self.inv = Inversion(g, ss, m2z, z2m)
+        arr = lambda n: (_dp * 2)(_p(self.a[n + "1"]), _p(self.a[n + "2"]))
+        self.st = LFState(g, L, arr("psi"), arr("rel"), arr("zeta"), arr("zeta_old"),
+                          _p(self.a["force_wind"]), dt, 0.0, 1)
+        self.pr = Params(_p(self.beta), _p(self.s1), _p(self.s2), viscosity, visc_bot, bc_type,
+                         alpha, 0, 0.0, 0.0, 0.0)
+        L_ = lib()
+        for name in ("pqo_lf_init", "pqo_lf_step"):
+            f = getattr(L_, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.POINTER(LFState), ctypes.POINTER(Params), ctypes.c_void_p]
+
+    def init(self, zeta_sdata):
+        g = self.g
+        z = self.a["zeta1"]
+        z[:] = 0.0
+        z[2:g, 2:g, :] = f64(zeta_sdata)[1:g - 1, 1:g - 1, :]
+        return lib().pqo_lf_init(ctypes.byref(self.st), ctypes.byref(self.pr), self.inv.h)
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps):
+            rc = lib().pqo_lf_step(ctypes.byref(self.st), ctypes.byref(self.pr), self.inv.h)
+            if rc == 3:
+                raise FloatingPointError("NaN psiC encountered")
+
+    def scalars(self):
+        return dict(dt=self.st.dt, model_time=self.st.model_time, leapfrog=self.st.leapfrog)

@@ -1420,6 +1420,113 @@ void pqo_psi_av_add(int gi, int gj, int L, double dt, const double *psi, double 
       }
 }
 
+/* zeta_centred (qg3p_aux.F90:2027-2202) / zeta_tendency with
+ * zeta_forward_euler (qg3p_aux.F90:1789-2024): dst = src + T * F on
+ * f_i, f_j = 3..G, with beta_y(i) = beta ((G + 1) / 2 - (i - 1))
+ * (qg3.F90:302-307) and the bottom friction on rel_p
+ * (CENTRED_BOTTOM_FRICTION undefined); one layer keeps the reference's
+ * visc_bot inside the viscosity bracket.                                    */
+void pqo_zeta_step(int g, int L, double T, const double *beta, const double *s1,
+                   const double *s2, const double *force_wind, double viscosity,
+                   double visc_bot, const double *psi, const double *rel,
+                   const double *rel_p, const double *src, double *dst) {
+#define FW(i, j) force_wind[((i)-1) + (size_t)g * ((j)-1)]
+#define BY(i, l) (beta[l] * (((double)g + 1.0) / 2.0 - (double)((i)-1)))
+#define P(i, j, l) ND(psi, i, j, l)
+#define R(i, j, l) ND(rel, i, j, l)
+#define LAPP(l)                                                                \
+  (-4.0 * ND(rel_p, i, j, l) + ND(rel_p, i - 1, j, l) + ND(rel_p, i + 1, j, l) + \
+   ND(rel_p, i, j - 1, l) + ND(rel_p, i, j + 1, l))
+  for (int l = 0; l < L; ++l)
+    for (int j = 3; j <= g; ++j)
+      for (int i = 3; i <= g; ++i) {
+        /* q(ii, jj): beta_y + rel + the coupling to the neighbouring layers */
+#define Q(ii, jj)                                                              \
+  (L == 1 ? BY(ii, l) + R(ii, jj, l)                                           \
+   : l == 0 ? BY(ii, l) + R(ii, jj, l) + s2[l] * P(ii, jj, l + 1)              \
+   : l == L - 1 ? BY(ii, l) + R(ii, jj, l) + s1[l] * P(ii, jj, l - 1)          \
+                : BY(ii, l) + R(ii, jj, l) + s1[l] * P(ii, jj, l - 1) + s2[l] * P(ii, jj, l + 1))
+        const double jac = Q(i, j + 1) * (P(i - 1, j + 1, l) - P(i + 1, j + 1, l)) -
+                           Q(i, j - 1) * (P(i - 1, j - 1, l) - P(i + 1, j - 1, l)) -
+                           Q(i - 1, j) * (P(i - 1, j + 1, l) - P(i - 1, j - 1, l)) +
+                           Q(i + 1, j) * (P(i + 1, j + 1, l) - P(i + 1, j - 1, l));
+#undef Q
+        double f;
+        if (L == 1)
+          f = 0.25 * jac + FW(i - 1, j - 1) + viscosity * (LAPP(0) - visc_bot * ND(rel_p, i, j, 0));
+        else if (l == 0)
+          f = 0.25 * jac + FW(i - 1, j - 1) + viscosity * LAPP(0);
+        else if (l < L - 1)
+          f = 0.25 * jac + viscosity * LAPP(l);
+        else
+          f = 0.25 * jac + viscosity * LAPP(l) - visc_bot * ND(rel_p, i, j, l);
+        ND(dst, i, j, l) = ND(src, i, j, l) + T * f;
+      }
+#undef LAPP
+#undef R
+#undef P
+#undef BY
+#undef FW
+}
+
+/* robert_asselin_filter_rank3 with RA_DOUBLE_FILTER (qg3p_aux.F90:44-62,
+ * 1716-1755) over fa_i, fa_j = 2..G+1 (qg3p_aux.F90:1758-1786).            */
+void pqo_ra_filter(int g, int L, double *phi_l, double *phi_m, const double *phi_old_l,
+                   double *phi_old_m) {
+  const double ra_nu = 0.02, ra_alpha = 0.5;
+  const double w21 = 0.5 * ra_nu * ra_alpha, w22 = 1.0 - 2.0 * w21;
+  const double w11 = -0.5 * ra_nu * (1.0 - ra_alpha), w12 = -2.0 * w11, w13 = 1.0 + w11;
+  for (int l = 0; l < L; ++l)
+    for (int j = 2; j <= g + 1; ++j)
+      for (int i = 2; i <= g + 1; ++i) {
+        const double lphi_l = ND(phi_l, i, j, l);
+        ND(phi_l, i, j, l) = w11 * ND(phi_old_l, i, j, l) + w12 * ND(phi_m, i, j, l) + w13 * lphi_l;
+        ND(phi_m, i, j, l) = w21 * (ND(phi_old_l, i, j, l) + lphi_l) + w22 * ND(phi_m, i, j, l);
+        ND(phi_old_m, i, j, l) = ND(phi_m, i, j, l);
+      }
+}
+
+static int lf_invert_level(pqo_lf_state *s, const pqo_params *p, pqo_inv *inv, int k) {
+  const int g = s->g, L = s->layers, ld = g + 2;
+  int st = pqo_inv_run(inv, s->zeta[k] + 1 + ld, ld, ld, s->psi[k] + 1 + ld, ld, ld);
+  if (st == 3) return st;
+  pqo_rel_from_zeta_and_psi(g, L, p->s1, p->s2, p->bc_type, p->alpha, s->zeta[k], s->psi[k],
+                            s->rel[k]);
+  pqo_update_zeta_bc(g, L, p->s1, p->s2, s->rel[k], s->psi[k], s->zeta[k]);
+  return st;
+}
+
+/* qg3.F90:383-391 (level 1), 449-452, 456-490 (FORWARD_EULER_START).      */
+int pqo_lf_init(pqo_lf_state *s, const pqo_params *p, pqo_inv *inv) {
+  const int g = s->g, L = s->layers;
+  const size_t n = (size_t)(g + 2) * (g + 2) * L;
+  int st = lf_invert_level(s, p, inv, 0);
+  if (st == 3) return st;
+  memcpy(s->zeta_old[0], s->zeta[0], sizeof(double) * n);
+  pqo_zeta_step(g, L, s->dt, p->beta, p->s1, p->s2, s->force_wind, p->viscosity, p->visc_bot,
+                s->psi[0], s->rel[0], s->rel[0], s->zeta[0], s->zeta[1]);
+  int st2 = lf_invert_level(s, p, inv, 1);
+  if (st2 == 3) return st2;
+  memcpy(s->zeta_old[1], s->zeta[1], sizeof(double) * n);
+  s->model_time = 0.0 + s->dt;
+  s->leapfrog = 1;
+  return st > st2 ? st : st2;
+}
+
+/* One pass of DOLEAPFROG in the time loop (qg3.F90:612-660).              */
+int pqo_lf_step(pqo_lf_state *s, const pqo_params *p, pqo_inv *inv) {
+  const int g = s->g, L = s->layers;
+  const int l = s->leapfrog - 1, m = 1 - l;
+  const double T = 2.0 * s->dt;
+  pqo_zeta_step(g, L, T, p->beta, p->s1, p->s2, s->force_wind, p->viscosity, p->visc_bot,
+                s->psi[m], s->rel[m], s->rel[l], s->zeta[l], s->zeta[l]);
+  pqo_ra_filter(g, L, s->zeta[l], s->zeta[m], s->zeta_old[l], s->zeta_old[m]);
+  int st = lf_invert_level(s, p, inv, l);
+  s->model_time = s->model_time + s->dt;
+  s->leapfrog = s->leapfrog % 2 + 1;
+  return st;
+}
+
 /* One time step in the order of the CABARET time loop (qg3.F90:590-967). */
 int pqo_step(pqo_state *s, const pqo_params *p, pqo_inv *inv, int do_corrector) {
   const int g = s->g, L = s->layers;

@@ -185,6 +185,38 @@ typedef struct {
 int pqo_step(pqo_state *s, const pqo_params *p, pqo_inv *inv,
              int do_corrector);
 
+/* ---- centred (leapfrog) integration, the driver without CABARET
+ * (qg3.F90:456-490, 612-632; qg3p_aux.F90:1644-2202), with WIND_FORCING,
+ * RA_DOUBLE_FILTER and FORWARD_EULER_START, no background flow ---- */
+/* dst = src + T F(psi, rel, rel_p) on the interior nodes: zeta_centred
+ * (T = 2 dt, psi and rel of level m, rel_p = rel of level l, src = dst =
+ * zeta(l)) and zeta_forward_euler (T = dt, one level).  Node arrays
+ * (g+2, g+2, L); force_wind (g, g); beta [L] (beta_y formed inside).       */
+void pqo_zeta_step(int g, int layers, double T, const double *beta, const double *s1,
+                   const double *s2, const double *force_wind, double viscosity,
+                   double visc_bot, const double *psi, const double *rel,
+                   const double *rel_p, const double *src, double *dst);
+/* robert_asselin_filter with RA_DOUBLE_FILTER (nu = 0.02, alpha = 0.5) on
+ * the owned nodes of every layer: phi_l = level n+1, phi_m = level n,
+ * phi_old_l = level n-1 (all updated as the reference does).               */
+void pqo_ra_filter(int g, int layers, double *phi_l, double *phi_m, const double *phi_old_l,
+                   double *phi_old_m);
+
+typedef struct {
+  int g, layers;
+  double *psi[2], *rel[2], *zeta[2], *zeta_old[2]; /* (g+2)(g+2)L each */
+  double *force_wind;                               /* g g */
+  double dt, model_time;
+  int leapfrog; /* 1 or 2: the level the next step writes */
+} pqo_lf_state;
+/* START_PERTURB-style start from zeta(1) (interior set by the caller,
+ * boundary zero) followed by the forward Euler start; returns the worst
+ * inversion status.                                                        */
+int pqo_lf_init(pqo_lf_state *s, const pqo_params *p, pqo_inv *inv);
+/* One leapfrog step: centred step, filter, inversion, rel, zeta_bc,
+ * model_time += dt.                                                        */
+int pqo_lf_step(pqo_lf_state *s, const pqo_params *p, pqo_inv *inv);
+
 #ifdef __cplusplus
 }
 #endif

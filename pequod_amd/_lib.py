@@ -78,6 +78,8 @@ SIGNATURES = {
     "pq_cabaret_corrector_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "pq_rel_from_zeta_and_psi_dev": (_i, [_i, _i, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp]),
     "pq_update_zeta_bc_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_zeta_step_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_ra_filter_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp]),
     "pq_model_create": (_i, [ctypes.POINTER(_vp), _vp]),
     "pq_model_destroy": (_i, [_vp]),
     "pq_model_init": (_i, [_vp, _dp]),
@@ -155,5 +157,5 @@ class PqModelParams(ctypes.Structure):
         ("viscosity", _d), ("visc_bot", _d),
         ("bc_type", _i), ("alpha", _d),
         ("adapt_dt", _i), ("target_cfl", _d), ("max_dt", _d), ("max_dt_increase", _d),
-        ("dt", _d), ("precision", _i),
+        ("dt", _d), ("precision", _i), ("scheme", _i),
     ]

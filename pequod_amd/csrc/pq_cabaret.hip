@@ -112,6 +112,71 @@ __global__ void k_forcing(Dims d, double dt, const double *__restrict__ fw, doub
   zeta[d.nd(i, j, l)] = zeta[d.nd(i, j, l)] + rv;
 }
 
+// ---- centred (leapfrog) scheme, the driver without CABARET
+// zeta_centred (qg3p_aux.F90:2027-2202) and zeta_tendency /
+// zeta_forward_euler (qg3p_aux.F90:1789-2024) share one form on the interior
+// nodes 3..G: dst = src + T F with
+//   F = 0.25 J(psi, q) + force_wind (top) + viscosity Lap(rel_p) - visc_bot rel_p (bottom),
+// q = beta_y + rel + the coupling to the neighbouring layers'
+// stream functions, beta_y(i) = beta ((G + 1) / 2 - (i - 1)) (qg3.F90:302-307).
+// One layer keeps visc_bot inside the viscosity bracket, as the reference.
+// zeta_centred: T = 2 dt, psi and rel of level n, rel_p of level n-1, src =
+// dst = zeta of level n-1; forward Euler: T = dt, one level, dst the other.
+__global__ void k_zeta_step(Dims d, double T, const double *__restrict__ beta,
+                            const double *__restrict__ s1, const double *__restrict__ s2,
+                            const double *__restrict__ fw, double visc, double visc_bot,
+                            const double *__restrict__ psi, const double *__restrict__ rel,
+                            const double *__restrict__ rel_p, const double *src, double *dst) {
+  PQ_IDX(3, d.g - 2, 3, d.g - 2);
+  const int L = d.L;
+  const double hb = ((double)d.g + 1.0) / 2.0;
+  auto q = [&](int ii, int jj) {
+    const double by = beta[l] * (hb - (double)(ii - 1));
+    double v = by + rel[d.nd(ii, jj, l)];
+    if (L > 1) {
+      if (l == 0) v = v + s2[0] * psi[d.nd(ii, jj, 1)];
+      else if (l == L - 1) v = v + s1[l] * psi[d.nd(ii, jj, l - 1)];
+      else v = v + s1[l] * psi[d.nd(ii, jj, l - 1)] + s2[l] * psi[d.nd(ii, jj, l + 1)];
+    }
+    return v;
+  };
+  auto P = [&](int ii, int jj) { return psi[d.nd(ii, jj, l)]; };
+  const double jac = q(i, j + 1) * (P(i - 1, j + 1) - P(i + 1, j + 1)) -
+                     q(i, j - 1) * (P(i - 1, j - 1) - P(i + 1, j - 1)) -
+                     q(i - 1, j) * (P(i - 1, j + 1) - P(i - 1, j - 1)) +
+                     q(i + 1, j) * (P(i + 1, j + 1) - P(i + 1, j - 1));
+  const double lap = -4.0 * rel_p[d.nd(i, j, l)] + rel_p[d.nd(i - 1, j, l)] +
+                     rel_p[d.nd(i + 1, j, l)] + rel_p[d.nd(i, j - 1, l)] + rel_p[d.nd(i, j + 1, l)];
+  const double f0 = fw[(size_t)(i - 2) + (size_t)d.g * (j - 2)];
+  double f;
+  if (L == 1)
+    f = 0.25 * jac + f0 + visc * (lap - visc_bot * rel_p[d.nd(i, j, 0)]);
+  else if (l == 0)
+    f = 0.25 * jac + f0 + visc * lap;
+  else if (l < L - 1)
+    f = 0.25 * jac + visc * lap;
+  else
+    f = 0.25 * jac + visc * lap - visc_bot * rel_p[d.nd(i, j, l)];
+  dst[d.nd(i, j, l)] = src[d.nd(i, j, l)] + T * f;
+}
+
+// robert_asselin_filter with RA_DOUBLE_FILTER (Williams' filter, nu = 0.02,
+// alpha = 0.5; qg3p_aux.F90:44-62, 1716-1786) on the owned nodes 2..G+1:
+// phi_l = level n+1, phi_m = level n, phi_old_l = level n-1.
+__global__ void k_ra_filter(Dims d, double *__restrict__ phi_l, double *__restrict__ phi_m,
+                            const double *__restrict__ phi_old_l, double *__restrict__ phi_old_m) {
+  PQ_IDX(2, d.g, 2, d.g);
+  const double ra_nu = 0.02, ra_alpha = 0.5;
+  const double w21 = 0.5 * ra_nu * ra_alpha, w22 = 1.0 - 2.0 * w21;
+  const double w11 = -0.5 * ra_nu * (1.0 - ra_alpha), w12 = -2.0 * w11, w13 = 1.0 + w11;
+  const size_t k = d.nd(i, j, l);
+  const double lphi_l = phi_l[k], old = phi_old_l[k], pm = phi_m[k];
+  phi_l[k] = w11 * old + w12 * pm + w13 * lphi_l;
+  const double nm = w21 * (old + lphi_l) + w22 * pm;
+  phi_m[k] = nm;
+  phi_old_m[k] = nm;
+}
+
 __device__ __forceinline__ double psi_edge(const Dims &d, const double *psi, int i, int j, int l) {
   return 0.25 * (psi[d.nd(i, j, l)] + psi[d.nd(i, j - 1, l)] + psi[d.nd(i - 1, j, l)] +
                  psi[d.nd(i - 1, j - 1, l)]);
@@ -490,6 +555,24 @@ int pq_update_zeta_bc_dev(int g, int L, const double *s1, const double *s2, cons
   return check_launch("pq_update_zeta_bc_dev");
 }
 
+int pq_zeta_step_dev(int g, int L, double T, const double *beta, const double *s1,
+                     const double *s2, const double *force_wind, double viscosity,
+                     double visc_bot, const double *psi, const double *rel, const double *rel_p,
+                     const double *src, double *dst, void *stream) {
+  CHECK_GL(g, L, "pq_zeta_step_dev");
+  hipLaunchKernelGGL(k_zeta_step, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+                     T, beta, s1, s2, force_wind, viscosity, visc_bot, psi, rel, rel_p, src, dst);
+  return check_launch("pq_zeta_step_dev");
+}
+
+int pq_ra_filter_dev(int g, int L, double *phi_l, double *phi_m, const double *phi_old_l,
+                     double *phi_old_m, void *stream) {
+  CHECK_GL(g, L, "pq_ra_filter_dev");
+  hipLaunchKernelGGL(k_ra_filter, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, phi_l,
+                     phi_m, phi_old_l, phi_old_m);
+  return check_launch("pq_ra_filter_dev");
+}
+
 // ------------------------------------------------------------------ model
 struct pq_model {
   int g = 0, L = 0;
@@ -503,6 +586,11 @@ struct pq_model {
   double *res = nullptr, *res_beta = nullptr, *fw = nullptr;
   double *fin = nullptr, *fjn = nullptr;
   double *s1 = nullptr, *s2 = nullptr, *beta = nullptr;
+  // centred (leapfrog) scheme: the second time level; lf is the level the
+  // next step writes (1: psi/rel/zeta/zeta_old, 2: the *2 arrays)
+  int scheme = PQ_SCHEME_CABARET, lf = 1;
+  double *psi2 = nullptr, *rel2 = nullptr, *zeta2 = nullptr, *zeta_old2 = nullptr;
+  size_t bytes = 0;  // device bytes of the fields above
   unsigned long long *maxu = nullptr;
   unsigned long long *maxu_host = nullptr;
   // scalars
@@ -535,6 +623,10 @@ static double **model_field(pq_model *m, int which, size_t *count) {
     case 11: *count = d.n_fj(); return &m->ujmo;
     case 12: *count = d.n_fi(); return &m->flux_i;
     case 13: *count = d.n_fj(); return &m->flux_j;
+    case 14: *count = d.n_nd(); return &m->psi2;
+    case 15: *count = d.n_nd(); return &m->zeta2;
+    case 16: *count = d.n_nd(); return &m->rel2;
+    case 17: *count = d.n_nd(); return &m->zeta_old2;
     default: *count = 0; return nullptr;
   }
 }
@@ -544,7 +636,7 @@ int pq_model_destroy(pq_model *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (double *p : {m->psi, m->rel, m->zeta, m->zeta_old, m->zfi, m->ui, m->uimo, m->flux_i,
                     m->zfj, m->uj, m->ujmo, m->flux_j, m->res, m->res_beta, m->fw, m->fin, m->fjn,
-                    m->s1, m->s2, m->beta})
+                    m->s1, m->s2, m->beta, m->psi2, m->rel2, m->zeta2, m->zeta_old2})
     (void)hipFree(p);
   (void)hipFree(m->psi_av);
   pq_diag_destroy(m->diag);
@@ -560,7 +652,12 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
   PQ_CHECK_ARG(p->s1 && p->s2 && p->ss && p->modes_to_z && p->z_to_modes && p->beta && p->force_wind,
                "pq_model_create: null parameter array");
   PQ_CHECK_ARG(p->dt > 0.0, "pq_model_create: dt must be > 0");
+  PQ_CHECK_ARG(p->scheme == PQ_SCHEME_CABARET || p->scheme == PQ_SCHEME_LEAPFROG,
+               "pq_model_create: unknown scheme %d", p->scheme);
+  PQ_CHECK_ARG(p->scheme == PQ_SCHEME_CABARET || !p->adapt_dt,
+               "pq_model_create: adapt_dt is a CABARET option");
   pq_model *m = new pq_model();
+  m->scheme = p->scheme;
   m->g = p->g;
   m->L = p->layers;
   m->d = Dims{p->g, p->layers};
@@ -577,16 +674,23 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
   struct A {
     double **p;
     size_t n;
-  } allocs[] = {{&m->psi, d.n_nd()},  {&m->rel, d.n_nd()},    {&m->zeta, d.n_nd()},
-                {&m->zeta_old, d.n_nd()}, {&m->zfi, d.n_fi()}, {&m->ui, d.n_fi()},
-                {&m->uimo, d.n_fi()}, {&m->flux_i, d.n_fi()}, {&m->zfj, d.n_fj()},
-                {&m->uj, d.n_fj()},   {&m->ujmo, d.n_fj()},   {&m->flux_j, d.n_fj()},
-                {&m->res, d.n_rs()},  {&m->res_beta, d.n_rs()}, {&m->fw, (size_t)p->g * p->g},
-                {&m->fin, d.n_fi()},  {&m->fjn, d.n_fj()},    {&m->s1, (size_t)p->layers},
-                {&m->s2, (size_t)p->layers}, {&m->beta, (size_t)p->layers}};
+  };
+  std::vector<A> allocs = {{&m->psi, d.n_nd()},  {&m->rel, d.n_nd()},    {&m->zeta, d.n_nd()},
+                           {&m->zeta_old, d.n_nd()}, {&m->fw, (size_t)p->g * p->g},
+                           {&m->s1, (size_t)p->layers}, {&m->s2, (size_t)p->layers},
+                           {&m->beta, (size_t)p->layers}};
+  if (m->scheme == PQ_SCHEME_CABARET)
+    allocs.insert(allocs.end(), {{&m->zfi, d.n_fi()}, {&m->ui, d.n_fi()}, {&m->uimo, d.n_fi()},
+                                 {&m->flux_i, d.n_fi()}, {&m->zfj, d.n_fj()}, {&m->uj, d.n_fj()},
+                                 {&m->ujmo, d.n_fj()}, {&m->flux_j, d.n_fj()}, {&m->res, d.n_rs()},
+                                 {&m->res_beta, d.n_rs()}, {&m->fin, d.n_fi()}, {&m->fjn, d.n_fj()}});
+  else
+    allocs.insert(allocs.end(), {{&m->psi2, d.n_nd()}, {&m->rel2, d.n_nd()}, {&m->zeta2, d.n_nd()},
+                                 {&m->zeta_old2, d.n_nd()}});
   for (auto &a : allocs) {
     if (hipMalloc((void **)a.p, a.n * sizeof(double)) != hipSuccess) return fail(PQ_ERR_NOMEM);
     if (hipMemset(*a.p, 0, a.n * sizeof(double)) != hipSuccess) return fail(PQ_ERR_HIP);
+    m->bytes += a.n * sizeof(double);
   }
   if (hipMalloc((void **)&m->maxu, sizeof(unsigned long long)) != hipSuccess) return fail(PQ_ERR_NOMEM);
   if (hipHostMalloc((void **)&m->maxu_host, sizeof(unsigned long long), 0) != hipSuccess)
@@ -612,11 +716,93 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
 
 void *pq_model_stream(pq_model *m) { return m ? (void *)m->stream : nullptr; }
 
-static int model_invert(pq_model *m) {
+static int invert_into(pq_model *m, const double *zeta, double *psi) {
   const int ld = m->g + 2;
-  const void *z[1] = {m->zeta + 1 + ld};
-  void *p[1] = {m->psi + 1 + ld};
+  const void *z[1] = {zeta + 1 + ld};
+  void *p[1] = {psi + 1 + ld};
   return pq_inv_run_dev(m->inv, 1, z, ld, ld, p, ld, ld, m->stream);
+}
+
+static int model_invert(pq_model *m) { return invert_into(m, m->zeta, m->psi); }
+
+// Arrays of time level k (1 or 2) of a centred-scheme model.
+struct Level {
+  double *psi, *rel, *zeta, *zeta_old;
+};
+static Level level(pq_model *m, int k) {
+  return k == 1 ? Level{m->psi, m->rel, m->zeta, m->zeta_old}
+                : Level{m->psi2, m->rel2, m->zeta2, m->zeta_old2};
+}
+
+// pv_inversion, rel_from_zeta_and_psi and update_zeta_bc of one level
+// (qg3.F90:383-391, 640-645).
+static int lf_diagnose(pq_model *m, const Level &v) {
+  int rc = invert_into(m, v.zeta, v.psi);
+  if (rc) return rc;
+  rc = pq_rel_from_zeta_and_psi_dev(m->g, m->L, m->s1, m->s2, m->bc_type, m->alpha, v.zeta, v.psi,
+                                    v.rel, m->stream);
+  if (rc) return rc;
+  return pq_update_zeta_bc_dev(m->g, m->L, m->s1, m->s2, v.rel, v.psi, v.zeta, m->stream);
+}
+
+// The time-mean accumulation after the inversion (qg3.F90:672-681).
+static int model_accumulate(pq_model *m, const double *psi, hipStream_t s) {
+  if (m->average && m->model_time > m->max_spin) {
+    const int ld = m->g + 2;
+    int rc = pq_psi_av_add_dev(m->g, m->L, m->dt, psi + 1 + ld, ld, ld, m->psi_av, m->g, m->g, s);
+    if (rc) return rc;
+    m->diag_time += m->dt;
+  }
+  return PQ_OK;
+}
+
+// One pass of the centred scheme's DOLEAPFROG body (qg3.F90:612-660):
+// zeta_centred into level lf, the Robert-Asselin-Williams filter, then
+// the level's inversion, rel and zeta boundary, model_time += dt.
+static int lf_step(pq_model *m) {
+  const Level vl = level(m, m->lf), vm = level(m, 3 - m->lf);
+  const int g = m->g, L = m->L;
+  const hipStream_t s = m->stream;
+  hipLaunchKernelGGL(k_zeta_step, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d, 2.0 * m->dt,
+                     m->beta, m->s1, m->s2, m->fw, m->visc, m->visc_bot, vm.psi, vm.rel, vl.rel,
+                     vl.zeta, vl.zeta);
+  hipLaunchKernelGGL(k_ra_filter, grid_for(g, g, L), dim3(BX, BY), 0, s, m->d, vl.zeta, vm.zeta,
+                     vl.zeta_old, vm.zeta_old);
+  int rc = check_launch("pq_model_step");
+  if (rc) return rc;
+  rc = lf_diagnose(m, vl);
+  if (rc) return rc;
+  m->model_time += m->dt;
+  rc = model_accumulate(m, vl.psi, s);
+  if (rc) return rc;
+  m->lf = m->lf % 2 + 1;
+  return PQ_OK;
+}
+
+// Start of the centred scheme (qg3.F90:383-391, 449-452, 456-490): level 1
+// from the installed zeta, zeta_old(1) = zeta(1), the forward Euler step
+// into level 2 (FORWARD_EULER_START), zeta_old(2) = zeta(2), model_time = dt.
+static int lf_init(pq_model *m) {
+  const Dims &d = m->d;
+  const Level v1 = level(m, 1), v2 = level(m, 2);
+  int rc = lf_diagnose(m, v1);
+  if (rc) return rc;
+  PQ_HIP(hipMemcpyAsync(v1.zeta_old, v1.zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice,
+                        m->stream));
+  PQ_HIP(hipMemsetAsync(v2.zeta, 0, sizeof(double) * d.n_nd(), m->stream));
+  hipLaunchKernelGGL(k_zeta_step, grid_for(m->g - 2, m->g - 2, m->L), dim3(BX, BY), 0, m->stream,
+                     m->d, m->dt, m->beta, m->s1, m->s2, m->fw, m->visc, m->visc_bot, v1.psi, v1.rel,
+                     v1.rel, v1.zeta, v2.zeta);
+  rc = check_launch("pq_model_init");
+  if (rc) return rc;
+  rc = lf_diagnose(m, v2);
+  if (rc) return rc;
+  PQ_HIP(hipMemcpyAsync(v2.zeta_old, v2.zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice,
+                        m->stream));
+  m->model_time = 0.0 + m->dt;
+  m->lf = 1;
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  return pq_inv_status(m->inv, nullptr);
 }
 
 int pq_model_init(pq_model *m, const double *zeta_sdata) {
@@ -628,6 +814,13 @@ int pq_model_init(pq_model *m, const double *zeta_sdata) {
     for (int j = 2; j <= g - 1; ++j)
       for (int i = 2; i <= g - 1; ++i)
         z[d.nd(i + 1, j + 1, l)] = zeta_sdata[(size_t)(i - 1) + (size_t)g * ((j - 1) + (size_t)g * l)];
+  if (m->scheme == PQ_SCHEME_LEAPFROG) {
+    for (double *p : {m->psi, m->rel, m->psi2, m->rel2})
+      PQ_HIP(hipMemsetAsync(p, 0, sizeof(double) * d.n_nd(), m->stream));
+    PQ_HIP(hipMemcpyAsync(m->zeta, z.data(), sizeof(double) * d.n_nd(), hipMemcpyHostToDevice, m->stream));
+    (void)ld;
+    return lf_init(m);
+  }
   // zero CABARET arrays (qg3.F90:413-424)
   for (double *p : {m->zfi, m->zfj, m->ui, m->uj, m->uimo, m->ujmo, m->flux_i, m->flux_j})
     PQ_HIP(hipMemsetAsync(p, 0, sizeof(double) * (p == m->zfi || p == m->ui || p == m->uimo || p == m->flux_i ? d.n_fi() : d.n_fj()), m->stream));
@@ -703,11 +896,9 @@ static int model_post(pq_model *m, hipStream_t s, bool last) {
   hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,
                      m->psi, m->zeta);
   m->model_time += 0.5 * dt;
-  if (m->average && m->model_time > m->max_spin) {
-    const int ld = g + 2;
-    int rc = pq_psi_av_add_dev(g, L, dt, m->psi + 1 + ld, ld, ld, m->psi_av, g, g, s);
+  {
+    int rc = model_accumulate(m, m->psi, s);
     if (rc) return rc;
-    m->diag_time += dt;
   }
   hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw, m->visc,
                      m->visc_bot, m->rel, m->zeta);
@@ -732,6 +923,14 @@ static int model_post(pq_model *m, hipStream_t s, bool last) {
 
 int pq_model_step(pq_model *m, int nsteps, int final_partial) {
   PQ_CHECK_ARG(m != nullptr && nsteps >= 0, "pq_model_step: bad arguments");
+  if (m->scheme == PQ_SCHEME_LEAPFROG) {
+    PQ_CHECK_ARG(!final_partial, "pq_model_step: final_partial is a CABARET option");
+    for (int it = 0; it < nsteps; ++it) {
+      int rc = lf_step(m);
+      if (rc) return rc;
+    }
+    return PQ_OK;
+  }
   for (int it = 0; it < nsteps; ++it) {
     int rc = model_pre(m, m->stream);
     if (rc) return rc;
@@ -769,11 +968,9 @@ int model_step_kind(const pq_model *m) {
   return m->model_time + 0.5 * m->dt > m->max_spin ? 1 : -1;
 }
 size_t model_device_bytes(const pq_model *m) {
-  const Dims &d = m->d;
-  return sizeof(double) * (4 * d.n_nd() + 5 * d.n_fi() + 5 * d.n_fj() + 2 * d.n_rs() +
-                           (size_t)m->g * m->g + 3 * (size_t)m->L + (m->psi_av ? d.n_rs() : 0)) +
-         inv_device_bytes(m->inv);
+  return m->bytes + (m->psi_av ? sizeof(double) * m->d.n_rs() : 0) + inv_device_bytes(m->inv);
 }
+int model_scheme(const pq_model *m) { return m->scheme; }
 // the same two half-step additions as model_post, so times round alike
 void model_account_step(pq_model *m) {
   m->model_time += 0.5 * m->dt;
@@ -859,7 +1056,9 @@ int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot) {
   // synchronous call then copies the results
   const int ld = m->g + 2;
   PQ_HIP(hipStreamSynchronize(m->stream));
-  return pq::diag_energy_sync(m->diag, m->psi + 1 + ld, ld, ld, ekin, epot);
+  // the level written last (centred scheme: the one before lf)
+  const double *psi = m->scheme == PQ_SCHEME_LEAPFROG ? level(m, 3 - m->lf).psi : m->psi;
+  return pq::diag_energy_sync(m->diag, psi + 1 + ld, ld, ld, ekin, epot);
 }
 
 }  // extern "C"

@@ -156,6 +156,8 @@ int pq_ens_add_model(pq_ens *e, pq_model *model, const double *ss, const double 
                      const double *z_to_modes, int *index) {
   PQ_CHECK_ARG(e && !e->finalized && model, "pq_ens_add_model: null argument or finalized ensemble");
   PQ_CHECK_ARG(!model_adaptive(model), "pq_ens_add_model: models with adapt_dt are stepped alone");
+  PQ_CHECK_ARG(model_scheme(model) == PQ_SCHEME_CABARET,
+               "pq_ens_add_model: centred-scheme models are stepped alone");
   PQ_CHECK_ARG(e->prec == PQ_PREC_FP64 || e->prec == PQ_PREC_MIXED,
                "pq_ens_add_model: model fields are FP64; use PQ_PREC_FP64 or PQ_PREC_MIXED");
   int g, L;

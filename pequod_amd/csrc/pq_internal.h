@@ -33,6 +33,7 @@ int model_step_post(pq_model *m, hipStream_t s, bool last);
 int model_step_invert(pq_model *m);  // the step's inversion, on the model's stream
 hipStream_t model_stream(pq_model *m);
 bool model_adaptive(const pq_model *m);
+int model_scheme(const pq_model *m);  // PQ_SCHEME_*
 void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld);
 // Host-side clocks of a model: model time and the time-mean weight.
 struct ModelClock {

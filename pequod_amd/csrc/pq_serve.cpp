@@ -345,6 +345,7 @@ int pq_srv_open_field(pq_srv *s, int g, int layers, const double *ss, const doub
 int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sdata_host, int *member) {
   PQ_CHECK_ARG(s && p && zeta_sdata_host && member, "pq_srv_open_model: null argument");
   PQ_CHECK_ARG(!p->adapt_dt, "pq_srv_open_model: models with adapt_dt are stepped alone");
+  PQ_CHECK_ARG(p->scheme == PQ_SCHEME_CABARET, "pq_srv_open_model: centred-scheme models are stepped alone");
   auto m = std::make_unique<Member>();
   m->kind = 1;
   m->g = p->g;

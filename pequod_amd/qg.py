@@ -304,15 +304,33 @@ def update_zeta_bc(s1, s2, rel, psi, zeta):
                                        _ptr(zeta), _stream_of(zeta)))
 
 
+def zeta_step(T, beta, s1, s2, force_wind, viscosity, visc_bot, psi, rel, rel_p, src, dst):
+    """dst = src + T F on the interior (zeta_centred / zeta_forward_euler,
+    qg3p_aux.F90:1789-2202); node fields (L, G+2, G+2), force_wind (G, G)
+    as a (G, G) tensor in Fortran order transposed (j, i)."""
+    g = dst.shape[2] - 2
+    check(load().pq_zeta_step_dev(g, _L(dst), T, _ptr(beta), _ptr(s1), _ptr(s2), _ptr(force_wind),
+                                  viscosity, visc_bot, _ptr(psi), _ptr(rel), _ptr(rel_p), _ptr(src),
+                                  _ptr(dst), _stream_of(dst)))
+
+
+def ra_filter(phi_l, phi_m, phi_old_l, phi_old_m):
+    """Robert-Asselin-Williams filter in place (qg3p_aux.F90:1716-1786)."""
+    g = phi_l.shape[2] - 2
+    check(load().pq_ra_filter_dev(g, _L(phi_l), _ptr(phi_l), _ptr(phi_m), _ptr(phi_old_l),
+                                  _ptr(phi_old_m), _stream_of(phi_l)))
+
+
 # --------------------------------------------------------------- model
 FIELDS = {"psi": 0, "zeta": 1, "rel": 2, "zeta_old": 3, "ui": 4, "uj": 5, "zeta_fi": 6,
           "zeta_fj": 7, "res": 8, "res_beta": 9, "uimo": 10, "ujmo": 11, "flux_i": 12,
-          "flux_j": 13}
+          "flux_j": 13, "psi2": 14, "zeta2": 15, "rel2": 16, "zeta_old2": 17}
+SCHEMES = {"cabaret": 0, "leapfrog": 1}
 
 
 def field_shape(name: str, g: int, L: int):
     """Shape of a model field (the reference's array shapes)."""
-    if name in ("psi", "zeta", "rel", "zeta_old"):
+    if name in ("psi", "zeta", "rel", "zeta_old", "psi2", "zeta2", "rel2", "zeta_old2"):
         return (g + 2, g + 2, L)
     if name in ("ui", "zeta_fi", "uimo", "flux_i"):
         return (g + 3, g + 2, L)
@@ -324,7 +342,8 @@ def field_shape(name: str, g: int, L: int):
 def model_params(g: int, strat: Stratification, beta, force_wind: np.ndarray, *,
                  viscosity: float, visc_bot: float, dt: float, bc_type: int = BC_PARTIAL_SLIP,
                  alpha: float = 0.0, adapt_dt: bool = False, target_cfl: float = 0.1,
-                 max_dt: float = 0.0, max_dt_increase: float = 1.5, precision: str = "fp64"):
+                 max_dt: float = 0.0, max_dt_increase: float = 1.5, precision: str = "fp64",
+                 scheme: str = "cabaret"):
     """pq_model_params for a model; returns (params, arrays it points into),
     the arrays to be kept alive while the params are used."""
     keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (strat.s1, strat.s2, strat.ss)]
@@ -335,7 +354,7 @@ def model_params(g: int, strat: Stratification, beta, force_wind: np.ndarray, *,
     p = _lib.PqModelParams(g, strat.layers, _np_ptr(s1), _np_ptr(s2), _np_ptr(ss), _np_ptr(m2z),
                            _np_ptr(z2m), _np_ptr(b), _np_ptr(fw), viscosity, visc_bot, bc_type,
                            alpha, int(adapt_dt), target_cfl, max_dt, max_dt_increase, dt,
-                           PREC[precision])
+                           PREC[precision], SCHEMES[scheme])
     return p, keep
 
 

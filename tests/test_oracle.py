@@ -630,3 +630,161 @@ def test_channel_brute_force():
         if m > 0:
             ref[0, :], ref[-1, :] = x[n * gj], x[n * gj + 1]
         assert np.max(np.abs(phi[:, :, m] - ref)) < 1e-12 * np.max(np.abs(ref)), m
+
+
+# ------------------------------------------------------------ leapfrog
+def _lf_fields(g, L):
+    return [np.zeros((g + 2, g + 2, L), order="F") for _ in range(4)]
+
+
+def test_zeta_step_zonal_flow_is_steady():
+    """psi, rel functions of i only: every Jacobian term cancels exactly."""
+    g, L = 12, 2
+    psi, rel, relp, src = _lf_fields(g, L)
+    i = np.arange(g + 2, dtype=float)[:, None, None]
+    psi += np.sin(0.3 * i) * np.array([1.0, -0.4])
+    rel += np.cos(0.2 * i) * np.array([0.5, 0.1])
+    src += np.random.default_rng(0).standard_normal(src.shape)
+    dst = src.copy(order="F")
+    O.zeta_step(0.7, [0.3, 0.2], [0.0, 0.4], [0.6, 0.0], np.zeros((g, g)), 0.0, 0.0,
+                psi, rel, relp, src, dst)
+    assert np.array_equal(dst, src)
+
+
+@pytest.mark.parametrize("L", [1, 2, 3])
+def test_zeta_step_uniform_flow_beta(L):
+    """psi = a j in every layer, rel = 0: only beta_y(i) = beta ((G+1)/2 -
+    (i-1)) is advected, 0.25 J = -a beta (the PV gradient points to small
+    i); with layer couplings s2 psi_(l+1) = s2 b i added to the top layer's
+    PV the tendency becomes a (s2 b - beta)."""
+    g, a, b, T = 10, 0.3, 0.7, 2.0
+    psi, rel, relp, src = _lf_fields(g, L)
+    i = np.arange(1, g + 3, dtype=float)[:, None]
+    j = np.arange(1, g + 3, dtype=float)[None, :]
+    beta = np.linspace(0.2, 0.5, L)
+    for l in range(L):
+        psi[:, :, l] = a * j
+    s1 = np.zeros(L)
+    s2 = np.zeros(L)
+    if L > 1:
+        psi[:, :, 1] = b * i + 0 * j   # layer 2 varies along i only
+        s2[0] = 0.9
+    dst = src.copy(order="F")
+    O.zeta_step(T, beta, s1, s2, np.zeros((g, g)), 0.0, 0.0, psi, rel, relp, src, dst)
+    exp0 = a * (s2[0] * b - beta[0]) if L > 1 else -a * beta[0]
+    assert np.allclose(dst[2:g, 2:g, 0], T * exp0, rtol=1e-13, atol=1e-14)
+
+
+def test_zeta_step_wind_viscosity_drag():
+    """Wind in the top layer only; viscosity times the 5-point Laplacian of
+    rel_p (rel_p = i^2: Laplacian 2); bottom drag on rel_p in the bottom
+    layer (one layer: inside the viscosity bracket, as the reference)."""
+    g, T, nu, r = 9, 0.5, 0.3, 0.2
+    fw = np.random.default_rng(1).standard_normal((g, g))
+    i = np.arange(1, g + 3, dtype=float)[:, None]
+    for L in (1, 3):
+        psi, rel, relp, src = _lf_fields(g, L)
+        relp += (i * i)[:, :, None]
+        dst = src.copy(order="F")
+        O.zeta_step(T, np.zeros(L), np.zeros(L), np.zeros(L), fw, nu, r, psi, rel, relp, src, dst)
+        ii = i[2:g, :]
+        top = fw[1:g - 1, 1:g - 1] + nu * 2.0 + (-nu * r * ii * ii if L == 1 else 0.0)
+        assert np.allclose(dst[2:g, 2:g, 0], T * top, rtol=1e-13)
+        if L == 3:
+            assert np.allclose(dst[2:g, 2:g, 1], T * nu * 2.0, rtol=1e-13)
+            assert np.allclose(dst[2:g, 2:g, 2], T * (nu * 2.0 - r * ii * ii), rtol=1e-13)
+        # only the interior is written
+        assert np.all(dst[[0, 1, g, g + 1], :, :] == 0.0) and np.all(dst[:, [0, 1, g, g + 1], :] == 0.0)
+
+
+def test_ra_filter_weights():
+    """Williams' filter with nu = 0.02, alpha = 0.5: the level n value moves
+    by nu alpha / 2 d and the level n+1 value by -nu (1 - alpha) / 2 d,
+    d = phi(n-1) - 2 phi(n) + phi(n+1); sequences linear in time are left
+    unchanged; phi_old(m) receives the filtered level n; halo untouched."""
+    g, L = 6, 2
+    rng = np.random.default_rng(4)
+    a, b, c = (np.asfortranarray(rng.standard_normal((g + 2, g + 2, L))) for _ in range(3))
+    pl, pm, pol, pom = c.copy(order="F"), b.copy(order="F"), a.copy(order="F"), np.zeros_like(a)
+    O.ra_filter(pl, pm, pol, pom)
+    d = a - 2 * b + c
+    s = np.s_[1:g + 1, 1:g + 1, :]
+    assert np.allclose(pm[s], (b + 0.005 * d)[s], rtol=0, atol=1e-14)
+    assert np.allclose(pl[s], (c - 0.005 * d)[s], rtol=0, atol=1e-14)
+    assert np.array_equal(pom[s], pm[s])
+    assert np.array_equal(pl[0], c[0]) and np.array_equal(pm[:, -1], b[:, -1])
+    lin = [a, a + 0.5, a + 1.0]
+    pl, pm, pol, pom = lin[2].copy(order="F"), lin[1].copy(order="F"), lin[0].copy(order="F"), np.zeros_like(a)
+    O.ra_filter(pl, pm, pol, pom)
+    assert np.allclose(pm[s], lin[1][s], atol=1e-14) and np.allclose(pl[s], lin[2][s], atol=1e-14)
+
+
+def _smooth_q(g, L, amp=1e-2):
+    x = np.linspace(0.0, 1.0, g)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    q = np.zeros((g, g, L), order="F")
+    for l in range(L):
+        q[:, :, l] = amp * (np.sin(np.pi * X) * np.sin(2 * np.pi * Y) * (1 + 0.3 * l)
+                            + 0.5 * np.sin(3 * np.pi * X + l) * np.sin(np.pi * Y))
+    return q
+
+
+def _lf_and_cabaret(g=65, dt_frac=0.25):
+    c = synth.config("qg3_255")
+    c.n = g - 2
+    nd = c.nondim()
+    st = nd["strat"]
+    kw = dict(viscosity=nd["viscosity"], visc_bot=nd["visc_bot"], dt=nd["dt"] * dt_frac, bc_type=1,
+              alpha=nd["alpha"])
+    fw = c.wind()
+    lf = O.LeapfrogModel(g, st.s1, st.s2, st.ss, st.modes_to_z, st.z_to_modes, nd["beta"], fw, **kw)
+    cab = O.Model(g, st.s1, st.s2, st.ss, st.modes_to_z, st.z_to_modes, nd["beta"], fw, **kw)
+    return lf, cab, _smooth_q(g, st.layers)
+
+
+def test_leapfrog_follows_cabaret():
+    """Both schemes integrate the same equations: on a smooth 3-layer field
+    (wind, beta, stretching, viscosity and drag all active) the PV change
+    of the leapfrog model over 20 steps follows the CABARET model's per
+    layer (correlation above 0.999, amplitude within 1 %); a wrong sign or
+    layer index in the Jacobian, the coupling, the forcing or the filter
+    breaks this."""
+    lf, cab, q = _lf_and_cabaret()
+    assert lf.init(q) == 0 and cab.init(q) == 0
+    z0 = cab.a["zeta"].copy()
+    n = 20
+    lf.step(n - 1)   # the forward Euler start is the first step
+    cab.step(n)
+    assert abs(lf.scalars()["model_time"] - n * lf.st.dt) < 1e-9 * n * lf.st.dt
+    zl = lf.a["zeta1"] if lf.st.leapfrog == 2 else lf.a["zeta2"]   # the level written last
+    for l in range(q.shape[2]):
+        dl = (zl - z0)[2:-2, 2:-2, l].ravel()
+        dc = (cab.a["zeta"] - z0)[2:-2, 2:-2, l].ravel()
+        corr = np.dot(dl, dc) / np.sqrt(np.dot(dl, dl) * np.dot(dc, dc))
+        assert corr > 0.999, (l, corr)
+        assert abs(np.linalg.norm(dl) / np.linalg.norm(dc) - 1) < 0.01, l
+
+
+def test_leapfrog_second_order_in_time():
+    """Free nonlinear evolution (beta, stretching and the Jacobian; no
+    forcing, viscosity or drag) over a fixed time: halving dt cuts the
+    error against a dt/16 run by about 4 (second order: the centred step
+    uses psi and rel of the middle level, the filter's alpha = 0.5 keeps
+    the order); a first-order slip (e.g. psi of the wrong level) gives 2."""
+    g = 33
+    c = synth.config("qg3_255")
+    c.n = g - 2
+    nd = c.nondim()
+    st = nd["strat"]
+
+    def run(dt, n):
+        m = O.LeapfrogModel(g, st.s1, st.s2, st.ss, st.modes_to_z, st.z_to_modes, nd["beta"],
+                            np.zeros((g, g)), viscosity=0.0, visc_bot=0.0, dt=dt, bc_type=2)
+        assert m.init(_smooth_q(g, st.layers, 0.3)) == 0
+        m.step(n - 1)
+        return (m.a["zeta1"] if m.st.leapfrog == 2 else m.a["zeta2"]).copy()
+
+    D, n = nd["dt"], 16
+    ref = run(D / 16, n * 16)
+    e = [np.max(np.abs(run(D / k, n * k) - ref)) for k in (1, 2, 4)]
+    assert e[0] / e[1] > 3.4 and e[1] / e[2] > 3.4, e
