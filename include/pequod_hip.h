@@ -312,6 +312,30 @@ int pq_model_psi_av(pq_model *m, double *host, size_t count, double *diag_time);
  * and epot on the host.  PQ_ERR_NAN when a result is NaN.                 */
 int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot);
 
+/* ---- model output and checkpoints: the reference's data sets, a
+ * plain-text meta-data file base[.index] and a binary data file
+ * base[.index].d of Fortran-style records (4-byte byte count, doubles in
+ * column-major order, byte count) in dir (qg3p_io.F90:202-317, 405-732;
+ * qg3.F90:821-934; manual/manual.tex:1610-1790).  l_scale / t_scale go to
+ * the meta-data (Fortran E26.17E3).                                        */
+/* Snapshot qgL_G_s: the two times, then psi and zeta per layer (both time
+ * levels per record for the centred scheme).  index < 0 omits the index,
+ * which is the name START_PICKUP reads.  supp (CABARET only) also writes
+ * qgL_G_s_supp: the two times, dt, ui, uj, zeta_fi, zeta_fj, res, res_beta
+ * per layer.  Synchronous.                                                 */
+int pq_model_write_snapshot(pq_model *m, const char *dir, int index, double model_time_day,
+                            double total_time_day, double l_scale, double t_scale, int supp);
+/* Time mean qgL_G_s_av: the two times, psi_av / diagnostic_time per layer. */
+int pq_model_write_mean(pq_model *m, const char *dir, double model_time_day,
+                        double total_time_day, double l_scale, double t_scale);
+/* START_PICKUP (qg3.F90:316-364): reads dir/qgL_G_s.d (and qgL_G_s_supp.d
+ * for CABARET), installs the fields and starts as the reference does
+ * (CABARET: fluxes of the saved faces and the corrector's half step, dt
+ * from the file, model_time = dt / 2; centred scheme: rel from psi,
+ * model_time = 0).  The times read are returned (either may be NULL).      */
+int pq_model_read_checkpoint(pq_model *m, const char *dir, double *model_time_day,
+                             double *total_time_day);
+
 /* ======================================================================
  * Diagnostics of a stream function field: replaces energy(strat, psi, ekin,
  * epot) (qg3p_diagnostics.F90:778-863) for the closed basin on one domain,

@@ -89,6 +89,9 @@ SIGNATURES = {
     "pq_model_scalars": (_i, [_vp, _dp]),
     "pq_model_stream": (_vp, [_vp]),
     "pq_model_average": (_i, [_vp, _i, _d]),
+    "pq_model_write_snapshot": (_i, [_vp, ctypes.c_char_p, _i, _d, _d, _d, _d, _i]),
+    "pq_model_write_mean": (_i, [_vp, ctypes.c_char_p, _d, _d, _d, _d]),
+    "pq_model_read_checkpoint": (_i, [_vp, ctypes.c_char_p, _dp, _dp]),
     "pq_model_psi_av": (_i, [_vp, _dp, _sz, _dp]),
     "pq_model_energy": (_i, [_vp, _dp, _dp, _dp]),
     "pq_diag_create": (_i, [ctypes.POINTER(_vp), _i, _i, _dp, _dp]),

@@ -25,6 +25,7 @@ SOURCES = [
     "pq_cabaret.hip",
     "pq_ensemble.hip",
     "pq_diag.hip",
+    "pq_io.cpp",
     "pq_serve.cpp",
 ]
 

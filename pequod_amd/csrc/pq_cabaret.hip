@@ -344,6 +344,34 @@ __global__ void k_corrector_to(Dims d, double dt, const double *__restrict__ flu
 // rel_from_zeta_and_psi + apply_rel_bc (qg3p_aux.F90:738-915) over owned
 // nodes 2..g+1.  Boundary nodes take the boundary formula; on corners the
 // left/right (j) edge formula wins, as it is applied last in the reference.
+// apply_rel_bc (qg3p_aux.F90:778-915) at a boundary node (i, j) of layer l.
+__device__ __forceinline__ double rel_bc(const Dims &d, int bc_type, double alpha,
+                                         const double *__restrict__ psi, int i, int j, int l) {
+  const int g = d.g;
+  const size_t a = d.nd(i, j, l);
+  const bool jb = (j == 2 || j == g + 1);
+  double r;
+  if (bc_type == 1) {
+    const double cff = alpha / (alpha + 3.0);
+    if (jb) {
+      const int s = (j == 2) ? 1 : -1;
+      r = cff * (-3.5 * psi[a] + 4.0 * psi[d.nd(i, j + s, l)] - 0.5 * psi[d.nd(i, j + 2 * s, l)]);
+    } else {
+      const int s = (i == 2) ? 1 : -1;
+      r = cff * (-3.5 * psi[a] + 4.0 * psi[d.nd(i + s, j, l)] - 0.5 * psi[d.nd(i + 2 * s, j, l)]);
+    }
+  } else if (bc_type == 2) {
+    if (jb) {
+      r = (j == 2) ? 2.0 * (psi[d.nd(i, 3, l)] - psi[a]) : 2.0 * (psi[d.nd(i, g, l)] - psi[a]);
+    } else {
+      r = (i == 2) ? 2.0 * (psi[d.nd(3, j, l)] - psi[a]) : 2.0 * (psi[d.nd(g, j, l)] - psi[a]);
+    }
+  } else {
+    r = 0.0;
+  }
+  return r;
+}
+
 __global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
                       int bc_type, double alpha, const double *__restrict__ zeta,
                       const double *__restrict__ psi, double *__restrict__ rel) {
@@ -353,24 +381,7 @@ __global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__res
   double r;
   const bool jb = (j == 2 || j == g + 1), ib = (i == 2 || i == g + 1);
   if (jb || ib) {
-    if (bc_type == 1) {
-      const double cff = alpha / (alpha + 3.0);
-      if (jb) {
-        const int s = (j == 2) ? 1 : -1;
-        r = cff * (-3.5 * psi[a] + 4.0 * psi[d.nd(i, j + s, l)] - 0.5 * psi[d.nd(i, j + 2 * s, l)]);
-      } else {
-        const int s = (i == 2) ? 1 : -1;
-        r = cff * (-3.5 * psi[a] + 4.0 * psi[d.nd(i + s, j, l)] - 0.5 * psi[d.nd(i + 2 * s, j, l)]);
-      }
-    } else if (bc_type == 2) {
-      if (jb) {
-        r = (j == 2) ? 2.0 * (psi[d.nd(i, 3, l)] - psi[a]) : 2.0 * (psi[d.nd(i, g, l)] - psi[a]);
-      } else {
-        r = (i == 2) ? 2.0 * (psi[d.nd(3, j, l)] - psi[a]) : 2.0 * (psi[d.nd(g, j, l)] - psi[a]);
-      }
-    } else {
-      r = 0.0;
-    }
+    r = rel_bc(d, bc_type, alpha, psi, i, j, l);
   } else if (L == 1) {
     r = zeta[a];
   } else if (l == 0) {
@@ -381,6 +392,20 @@ __global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__res
     r = zeta[a] + s1[L - 1] * (psi[a] - psi[d.nd(i, j, L - 2)]);
   }
   rel[a] = r;
+}
+
+// rel_from_psi (qg3p_aux.F90:579-603): the 5-point Laplacian on the
+// interior (the reference's term order), apply_rel_bc on the boundary.
+__global__ void k_rel_from_psi(Dims d, int bc_type, double alpha, const double *__restrict__ psi,
+                               double *__restrict__ rel) {
+  PQ_IDX(2, d.g, 2, d.g);
+  const int g = d.g;
+  const size_t a = d.nd(i, j, l);
+  if (i == 2 || i == g + 1 || j == 2 || j == g + 1)
+    rel[a] = rel_bc(d, bc_type, alpha, psi, i, j, l);
+  else
+    rel[a] = -4.0 * psi[a] + psi[d.nd(i - 1, j, l)] + psi[d.nd(i, j + 1, l)] +
+             psi[d.nd(i + 1, j, l)] + psi[d.nd(i, j - 1, l)];
 }
 
 // update_zeta_bc (qg3p_aux.F90:643-734): boundary nodes only.
@@ -971,6 +996,65 @@ size_t model_device_bytes(const pq_model *m) {
   return m->bytes + (m->psi_av ? sizeof(double) * m->d.n_rs() : 0) + inv_device_bytes(m->inv);
 }
 int model_scheme(const pq_model *m) { return m->scheme; }
+ModelView model_view(pq_model *m) {
+  ModelView v{};
+  v.g = m->g;
+  v.L = m->L;
+  v.scheme = m->scheme;
+  v.psi = m->psi;
+  v.zeta = m->zeta;
+  v.rel = m->rel;
+  v.zeta_old = m->zeta_old;
+  v.psi2 = m->psi2;
+  v.zeta2 = m->zeta2;
+  v.ui = m->ui;
+  v.uj = m->uj;
+  v.zfi = m->zfi;
+  v.zfj = m->zfj;
+  v.res = m->res;
+  v.res_beta = m->res_beta;
+  v.psi_av = m->psi_av;
+  v.dt = m->dt;
+  v.diag_time = m->diag_time;
+  v.stream = m->stream;
+  return v;
+}
+// START_PICKUP after the checkpoint's fields are in place (qg3.F90:316-364,
+// 449-452, 456-466).  CABARET: uimo, ujmo from psi, the fluxes of the saved
+// face values and the corrector's half step, zeta_old = zeta, model_time =
+// dt / 2, time weights 1/2.  Centred scheme: rel = rel_from_psi of both
+// levels, zeta_old = zeta, model_time = 0, level 1 next.
+int model_pickup(pq_model *m, double dt) {
+  const int g = m->g, L = m->L;
+  const Dims &d = m->d;
+  const hipStream_t s = m->stream;
+  if (m->scheme == PQ_SCHEME_LEAPFROG) {
+    for (int k = 1; k <= 2; ++k) {
+      const Level v = level(m, k);
+      hipLaunchKernelGGL(k_rel_from_psi, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->bc_type, m->alpha,
+                         v.psi, v.rel);
+      PQ_HIP(hipMemcpyAsync(v.zeta_old, v.zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+    }
+    m->model_time = 0.0;
+    m->lf = 1;
+  } else {
+    m->dt = dt;
+    hipLaunchKernelGGL(k_cell_boundaries_velocity, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d,
+                       m->psi, m->uimo, m->ujmo);
+    hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, m->ui, m->uj, m->zfi,
+                       m->zfj, m->flux_i, m->flux_j);
+    hipLaunchKernelGGL(k_corrector, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->flux_i,
+                       m->flux_j, m->res, m->zeta);
+    PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+    m->model_time = dt / 2.0;
+    m->tw_a = 0.5;
+    m->tw_u = 0.5;
+  }
+  int rc = check_launch("pickup");
+  if (rc) return rc;
+  PQ_HIP(hipStreamSynchronize(s));
+  return PQ_OK;
+}
 // the same two half-step additions as model_post, so times round alike
 void model_account_step(pq_model *m) {
   m->model_time += 0.5 * m->dt;

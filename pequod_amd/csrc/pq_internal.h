@@ -34,6 +34,19 @@ int model_step_invert(pq_model *m);  // the step's inversion, on the model's str
 hipStream_t model_stream(pq_model *m);
 bool model_adaptive(const pq_model *m);
 int model_scheme(const pq_model *m);  // PQ_SCHEME_*
+// Device arrays and scalars of a model (the reference's full data arrays;
+// CABARET-only arrays are null for the centred scheme and the second time
+// level is null for CABARET).
+struct ModelView {
+  int g, L, scheme;
+  double *psi, *zeta, *rel, *zeta_old, *psi2, *zeta2;
+  double *ui, *uj, *zfi, *zfj, *res, *res_beta, *psi_av;
+  double dt, diag_time;
+  hipStream_t stream;
+};
+ModelView model_view(pq_model *m);
+// The START_PICKUP start once a checkpoint's fields are on the device.
+int model_pickup(pq_model *m, double dt);
 void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld);
 // Host-side clocks of a model: model time and the time-mean weight.
 struct ModelClock {

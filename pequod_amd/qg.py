@@ -411,6 +411,27 @@ class Model:
         check(load().pq_model_psi_av(self._h, _np_ptr(a), a.size, ctypes.byref(t)))
         return a, t.value
 
+    def write_snapshot(self, directory: str, index: int = -1, model_time_day: float = 0.0,
+                       total_time_day: float = 0.0, l_scale: float = 1.0, t_scale: float = 1.0,
+                       supp: bool = False) -> None:
+        """The reference's snapshot data set qgL_G_s[.index] (and, for CABARET
+        with supp, qgL_G_s_supp) in `directory`; index -1 gives the name
+        a START_PICKUP start reads."""
+        check(load().pq_model_write_snapshot(self._h, directory.encode(), index, model_time_day,
+                                             total_time_day, l_scale, t_scale, int(supp)))
+
+    def write_mean(self, directory: str, model_time_day: float = 0.0, total_time_day: float = 0.0,
+                   l_scale: float = 1.0, t_scale: float = 1.0) -> None:
+        check(load().pq_model_write_mean(self._h, directory.encode(), model_time_day, total_time_day,
+                                         l_scale, t_scale))
+
+    def read_checkpoint(self, directory: str):
+        """START_PICKUP from `directory`; returns (model_time_day, total_time_day)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(load().pq_model_read_checkpoint(self._h, directory.encode(), ctypes.byref(a),
+                                              ctypes.byref(b)))
+        return a.value, b.value
+
     def energy(self, h=None):
         """energy(strat, psi, ekin, epot) of the current psi: (ekin[L], epot)."""
         h = np.ascontiguousarray(self.strat.h if h is None else h, dtype=np.float64)
