@@ -92,13 +92,101 @@ def ensemble_rates(torch, dev, precision, log, members=64, reps=20):
             "max_rel_psi_err_vs_fp64": err}
 
 
-def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11):
+def cfl_dt(c, q, cfl=0.3):
+    """A fixed time step at Courant number `cfl` of the flow that the PV
+    field q inverts to (FP64 inversion on the device; non-dimensional, grid
+    spacing 1), capped at the configuration's dt.  The k^-3 fields at 5 km
+    spacing exceed Courant number 1 at dt = 1800 s."""
+    from pequod_amd import qg
+    nd = c.nondim()
+    m = qg.Model(c.g, nd["strat"], nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                 visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1, alpha=nd["alpha"])
+    m.init(q)
+    psi = m.get("psi")[1:-1, 1:-1, :]
+    m.close()
+    umax = max(np.max(np.abs(np.diff(psi, axis=0))), np.max(np.abs(np.diff(psi, axis=1))))
+    return min(nd["dt"], cfl / umax)
+
+
+def ensemble_step_rates(torch, dev, precision, log, members=64, steps=200, ref_steps=20):
+    """Steps/s of the 64-member mixed ensemble run as CABARET models: one
+    ensemble step advances every member by one time step (predictor, PV
+    inversion, corrector), fields stored in FP32, the inversions batched per
+    size class, the whole step replayed from a HIP graph.  Against FP64
+    models (FP64 fields and transforms) stepped one by one, the plain
+    drop-in, timed over `ref_steps` steps.  Reports the max relative psi
+    error of the FP32 members against FP64 models after the same steps."""
+    from pequod_amd import qg, synth
+    from pequod_amd.ensemble import Ensemble
+
+    cfgs = [synth.ensemble_member(k) for k in range(members)]
+
+    def make(c, prec, dt):
+        nd = c.nondim()
+        m = qg.Model(c.g, nd["strat"], nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                     visc_bot=nd["visc_bot"], dt=dt, bc_type=1, alpha=nd["alpha"],
+                     precision=prec)
+        m.init(c.q(member=c.extra["member"]))
+        return m, nd["strat"]
+
+    dts = [cfl_dt(c, c.q(member=c.extra["member"])) for c in cfgs]
+    log(f"[bench] ensemble step: {members} CABARET members ({precision} models), {steps} steps, "
+        f"dt {min(dts):.3g}..{max(dts):.3g} (non-dimensional)")
+    mods = [make(c, precision, dt) for c, dt in zip(cfgs, dts)]
+    ens = Ensemble(precision)
+    for m, st in mods:
+        ens.add_model(m, st)
+    ens.finalize()
+    ens.capture(Ensemble.STEP)
+    ens.step(1)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.synchronize()
+    t_ens = (time.perf_counter() - t0) / steps
+    assert ens.status() == 0
+    # FP64 drop-in: the same members as FP64 models, one pq_model_step each
+    ref = [make(c, "fp64", dt)[0] for c, dt in zip(cfgs, dts)]
+    for r in ref:
+        r.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(ref_steps):
+        for r in ref:
+            r.step(1)
+    torch.cuda.synchronize()
+    t_64 = (time.perf_counter() - t0) / ref_steps
+    for r in ref:
+        r.step(steps - ref_steps)
+    err = 0.0
+    for (m, _), r in zip(mods, ref):
+        a, b = m.get("psi"), r.get("psi")
+        for l in range(a.shape[2]):
+            err = max(err, float(np.max(np.abs(a[:, :, l] - b[:, :, l])) / np.max(np.abs(b[:, :, l]))))
+        r.close()
+    ens.close()
+    for m, _ in mods:
+        m.close()
+    return {"members": members, "precision": precision, "steps": steps,
+            "batched_graph_steps_per_s": round(1.0 / t_ens, 2),
+            "member_steps_per_s": round(members / t_ens, 1),
+            "fp64_dropin_steps_per_s": round(1.0 / t_64, 2),
+            "speedup_vs_fp64_dropin": round(t_64 / t_ens, 3),
+            "max_rel_psi_err_vs_fp64_after_steps": err,
+            "dt_range_nondim": [round(min(dts), 5), round(max(dts), 5)]}
+
+
+def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11, load=0.5):
     """Serving leg: Poisson arrivals (mean `rate` per s over `window` s), 70 %
     inversions of field members and 30 % `steps`-step advances of CABARET
     model members, members resident on the device.  Field members: grids
     {127, 255, 511, 1023, 2047}^2 x 2..6 layers, FP32 storage; model members
-    (FP64): grids {127, 255, 511, 1023}^2 x {2, 4, 6} layers.  Returns
-    latency percentiles per kind and the completed rate."""
+    (FP32 fields and transforms): grids {127, 255, 511, 1023}^2 x {2, 4, 6}
+    layers.  Returns
+    latency percentiles per kind and the completed rate at the offered rate
+    (a backlog forms when it exceeds the capacity, and latency is then
+    queueing), and again at `load` times the completed rate measured there,
+    a load the engine sustains."""
     from pequod_amd import qg, synth
     from pequod_amd.serve import Server
 
@@ -117,9 +205,11 @@ def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11):
         for L in (2, 4, 6):
             c = synth.steep_member(L, 1.6, n=n, member=200 + L)
             nd = c.nondim()
-            models.append(srv.open_model(c.g, nd["strat"], nd["beta"], c.wind(), c.q(member=200 + L),
+            q = c.q(member=200 + L)
+            models.append(srv.open_model(c.g, nd["strat"], nd["beta"], c.wind(), q,
                                          viscosity=nd["viscosity"], visc_bot=nd["visc_bot"],
-                                         dt=nd["dt"], bc_type=1, alpha=nd["alpha"]))
+                                         dt=cfl_dt(c, q), bc_type=1, alpha=nd["alpha"],
+                                         precision="fp32"))
     # warm-up: one request per member; statistics start after it
     for m in fields:
         srv.submit(m, Server.FIELD)
@@ -129,28 +219,37 @@ def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11):
     first += 1
     used = srv.used_bytes
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * window * 1.5)))
-    arrivals = arrivals[arrivals < window]
-    kinds = rng.random(arrivals.size) < 0.7
-    targets = [int(rng.integers(len(fields))) if k else int(rng.integers(len(models))) for k in kinds]
-    log(f"[bench] serving: {arrivals.size} requests over {window} s ({len(fields)} field members, "
-        f"{len(models)} model members)")
-    t0 = time.perf_counter()
-    for a, inv, tgt in zip(arrivals, kinds, targets):
-        while time.perf_counter() - t0 < a:
-            pass
-        if inv:
-            srv.submit(fields[tgt], Server.FIELD)
-        else:
-            srv.submit(models[tgt], Server.MODEL, steps)
-    srv.drain(1200)
-    makespan = time.perf_counter() - t0
-    out = {"offered_per_s": rate, "window_s": window, "requests": int(arrivals.size),
-           "served_per_s": round(arrivals.size / makespan, 1), "hbm_used_gb": round(used / 1e9, 2)}
-    for name, kind in (("inversion", Server.FIELD), ("advance_50", Server.MODEL), ("all", -1)):
-        st = srv.stats(kind, first)
-        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()
-                     if k in ("count", "p50_ms", "p95_ms", "p99_ms", "max_ms", "failed")}
+
+    def run(rate, window, first):
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * window * 1.5) + 10))
+        arrivals = arrivals[arrivals < window]
+        kinds = rng.random(arrivals.size) < 0.7
+        targets = [int(rng.integers(len(fields))) if k else int(rng.integers(len(models))) for k in kinds]
+        log(f"[bench] serving: {arrivals.size} requests over {window} s at {rate:.0f}/s "
+            f"({len(fields)} field members, {len(models)} model members)")
+        t0 = time.perf_counter()
+        last = first
+        for a, inv, tgt in zip(arrivals, kinds, targets):
+            while time.perf_counter() - t0 < a:
+                pass
+            if inv:
+                last = srv.submit(fields[tgt], Server.FIELD)
+            else:
+                last = srv.submit(models[tgt], Server.MODEL, steps)
+        srv.drain(1200)
+        makespan = time.perf_counter() - t0
+        out = {"offered_per_s": round(rate, 1), "window_s": window, "requests": int(arrivals.size),
+               "served_per_s": round(arrivals.size / makespan, 1)}
+        for name, kind in (("inversion", Server.FIELD), (f"advance_{steps}", Server.MODEL), ("all", -1)):
+            st = srv.stats(kind, first)
+            out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()
+                         if k in ("count", "p50_ms", "p95_ms", "p99_ms", "max_ms", "failed")}
+        return out, last + 1
+
+    out, nxt = run(rate, window, first)
+    out["hbm_used_gb"] = round(used / 1e9, 2)
+    sustained_rate = load * out["served_per_s"]
+    out["sustained"], _ = run(sustained_rate, max(window, 4.0), nxt)
     srv.close()
     return out
 
@@ -219,6 +318,8 @@ def main():
     ap.add_argument("--no-cabaret", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-member mixed-size ensemble leg")
+    ap.add_argument("--no-ensemble-step", action="store_true",
+                    help="skip the 64-member ensemble time-step (CABARET models) leg")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving leg (Poisson request stream, continuous batching)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg: requests per second")
@@ -379,33 +480,31 @@ def main():
         _, _, _, w64, _ = run_rate("fp64", max(20, args.steps // 4), 5, B)
         fp64_rate = B * max(20, args.steps // 4) / w64
 
-    # device-resident FP64 CABARET model (same grid): steps/s
-    cab = lfr = None
+    # device-resident CABARET models (same grid, FP64 and FP32 fields) and
+    # the centred scheme: steps/s at a fixed dt of Courant number 0.3
+    cab = cab32 = lfr = None
     if not args.no_cabaret:
         nd = c.nondim()
-        m = qg.Model(g, st, nd["beta"], c.wind(), viscosity=nd["viscosity"],
-                     visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1, alpha=nd["alpha"])
-        m.init(q)
-        m.step(5)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        nst = 50
-        m.step(nst)
-        m.get("zeta")  # synchronises the model stream
-        cab = nst / (time.perf_counter() - t0)
-        m.close()
-        # the centred (leapfrog) scheme on the same grid
-        m = qg.Model(g, st, nd["beta"], c.wind(), viscosity=nd["viscosity"],
-                     visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1, alpha=nd["alpha"],
-                     scheme="leapfrog")
-        m.init(q)
-        m.step(5)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        m.step(nst)
-        m.get("zeta")
-        lfr = nst / (time.perf_counter() - t0)
-        m.close()
+        dt_run = cfl_dt(c, q)
+        nst = 200
+
+        def model_rate(**extra):
+            m = qg.Model(g, st, nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                         visc_bot=nd["visc_bot"], dt=dt_run, bc_type=1, alpha=nd["alpha"], **extra)
+            m.init(q)
+            m.step(5)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            m.step(nst)
+            torch.cuda.synchronize()  # device-wide: includes the model's stream
+            rate = nst / (time.perf_counter() - t0)
+            assert np.all(np.isfinite(m.get("psi")))
+            m.close()
+            return rate
+
+        cab = model_rate()
+        cab32 = model_rate(precision="fp32")
+        lfr = model_rate(scheme="leapfrog")
 
     # 64-member mixed ensemble (sizes {255, 511, 767, 1023}^2, 2..8 layers,
     # per-member stratification): batched per size class and replayed from a
@@ -413,6 +512,10 @@ def main():
     ens = None
     if not args.no_ensemble:
         ens = ensemble_rates(torch, dev, args.ensemble_precision, log)
+
+    ens_step = None
+    if not args.no_ensemble_step:
+        ens_step = ensemble_step_rates(torch, dev, args.ensemble_precision, log)
 
     chan = None
     if not args.no_channel:
@@ -471,8 +574,10 @@ def main():
         "fp64_dropin_inversions_per_s": None if fp64_rate is None else round(fp64_rate, 1),
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
+        "cabaret_fp32_steps_per_s": None if cab32 is None else round(cab32, 2),
         "leapfrog_fp64_steps_per_s": None if lfr is None else round(lfr, 2),
         "ensemble_64_mixed": ens,
+        "ensemble_64_mixed_steps": ens_step,
         "channel": chan,
         "serving": serving,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(ach, 1),

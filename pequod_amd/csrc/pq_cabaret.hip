@@ -57,17 +57,42 @@ inline dim3 grid_for(int ni, int nj, int L) {
   const int l = (int)blockIdx.z;                                                \
   if (i >= (i0) + (ni) || j >= (j0) + (nj)) return;
 
+// Field accessors: the model's fields are stored as TS (double, or float
+// for the FP32-storage model); every read converts to double and every
+// write rounds to TS, so the arithmetic is FP64 either way (and for TS =
+// double the code is the plain FP64 code).
+template <typename TS> struct R {
+  const TS *__restrict__ p;
+  __host__ __device__ R(const TS *q) : p(q) {}
+  __device__ __forceinline__ double operator[](size_t k) const { return double(p[k]); }
+};
+template <typename TS> struct W {
+  TS *__restrict__ p;
+  __host__ __device__ W(TS *q) : p(q) {}
+  struct Ref {
+    TS *q;
+    __device__ __forceinline__ Ref &operator=(double v) {
+      *q = TS(v);
+      return *this;
+    }
+    __device__ __forceinline__ operator double() const { return double(*q); }
+  };
+  __device__ __forceinline__ Ref operator[](size_t k) const { return Ref{p + k}; }
+};
+
 // cabaret_predictor (qg3p_aux.F90:993-1020): f range 3..g
-__global__ void k_predictor(Dims d, double dt, const double *__restrict__ zeta_old,
-                            const double *__restrict__ res, double *__restrict__ zeta) {
+template <typename TS>
+__global__ void k_predictor(Dims d, double dt, R<TS> zeta_old,
+                            R<TS> res, W<TS> zeta) {
   PQ_IDX(3, d.g - 2, 3, d.g - 2);
   zeta[d.nd(i, j, l)] = zeta_old[d.nd(i, j, l)] + 0.5 * dt * res[d.rs(i - 1, j - 1, l)];
 }
 
 // cabaret_beta (qg3p_aux.F90:1023-1061): l range 2..g-1 of sdata
+template <typename TS>
 __global__ void k_beta(Dims d, double dt, double tw_a, const double *__restrict__ beta,
-                       const double *__restrict__ ui, double *__restrict__ zeta,
-                       double *__restrict__ res_beta) {
+                       R<TS> ui, W<TS> zeta,
+                       W<TS> res_beta) {
   PQ_IDX(2, d.g - 2, 2, d.g - 2);
   const double rb = dt * (0.5 * (ui[d.fi(i + 2, j + 1, l)] + ui[d.fi(i + 1, j + 1, l)]) * beta[l]);
   const size_t z = d.nd(i + 1, j + 1, l);
@@ -77,12 +102,13 @@ __global__ void k_beta(Dims d, double dt, double tw_a, const double *__restrict_
 
 // cabaret_predictor followed by cabaret_beta, one pass (the beta update of
 // data node (i, j) only reads that node's predicted value).
+template <typename TS>
 __global__ void k_predictor_beta(Dims d, double dt, double tw_a,
                                  const double *__restrict__ beta,
-                                 const double *__restrict__ zeta_old,
-                                 const double *__restrict__ res,
-                                 const double *__restrict__ ui, double *__restrict__ zeta,
-                                 double *__restrict__ res_beta) {
+                                 R<TS> zeta_old,
+                                 R<TS> res,
+                                 R<TS> ui, W<TS> zeta,
+                                 W<TS> res_beta) {
   PQ_IDX(3, d.g - 2, 3, d.g - 2);
   double z = zeta_old[d.nd(i, j, l)] + 0.5 * dt * res[d.rs(i - 1, j - 1, l)];
   const double rb = dt * (0.5 * (ui[d.fi(i + 1, j, l)] + ui[d.fi(i, j, l)]) * beta[l]);
@@ -92,9 +118,10 @@ __global__ void k_predictor_beta(Dims d, double dt, double tw_a,
 }
 
 // cabaret_forcing with WIND_FORCING (qg3p_aux.F90:1065-1164)
+template <typename TS>
 __global__ void k_forcing(Dims d, double dt, const double *__restrict__ fw, double visc,
-                          double visc_bot, const double *__restrict__ rel,
-                          double *__restrict__ zeta) {
+                          double visc_bot, R<TS> rel,
+                          W<TS> zeta) {
   PQ_IDX(3, d.g - 2, 3, d.g - 2);
   const int L = d.L;
   const double lap = -4.0 * rel[d.nd(i, j, l)] + rel[d.nd(i - 1, j, l)] + rel[d.nd(i + 1, j, l)] +
@@ -122,11 +149,12 @@ __global__ void k_forcing(Dims d, double dt, const double *__restrict__ fw, doub
 // One layer keeps visc_bot inside the viscosity bracket, as the reference.
 // zeta_centred: T = 2 dt, psi and rel of level n, rel_p of level n-1, src =
 // dst = zeta of level n-1; forward Euler: T = dt, one level, dst the other.
+template <typename TS>
 __global__ void k_zeta_step(Dims d, double T, const double *__restrict__ beta,
                             const double *__restrict__ s1, const double *__restrict__ s2,
                             const double *__restrict__ fw, double visc, double visc_bot,
-                            const double *__restrict__ psi, const double *__restrict__ rel,
-                            const double *__restrict__ rel_p, const double *src, double *dst) {
+                            R<TS> psi, R<TS> rel,
+                            R<TS> rel_p, R<TS> src, W<TS> dst) {
   PQ_IDX(3, d.g - 2, 3, d.g - 2);
   const int L = d.L;
   const double hb = ((double)d.g + 1.0) / 2.0;
@@ -163,8 +191,9 @@ __global__ void k_zeta_step(Dims d, double T, const double *__restrict__ beta,
 // robert_asselin_filter with RA_DOUBLE_FILTER (Williams' filter, nu = 0.02,
 // alpha = 0.5; qg3p_aux.F90:44-62, 1716-1786) on the owned nodes 2..G+1:
 // phi_l = level n+1, phi_m = level n, phi_old_l = level n-1.
-__global__ void k_ra_filter(Dims d, double *__restrict__ phi_l, double *__restrict__ phi_m,
-                            const double *__restrict__ phi_old_l, double *__restrict__ phi_old_m) {
+template <typename TS>
+__global__ void k_ra_filter(Dims d, W<TS> phi_l, W<TS> phi_m,
+                            R<TS> phi_old_l, W<TS> phi_old_m) {
   PQ_IDX(2, d.g, 2, d.g);
   const double ra_nu = 0.02, ra_alpha = 0.5;
   const double w21 = 0.5 * ra_nu * ra_alpha, w22 = 1.0 - 2.0 * w21;
@@ -177,16 +206,18 @@ __global__ void k_ra_filter(Dims d, double *__restrict__ phi_l, double *__restri
   phi_old_m[k] = nm;
 }
 
-__device__ __forceinline__ double psi_edge(const Dims &d, const double *psi, int i, int j, int l) {
+template <typename A>
+__device__ __forceinline__ double psi_edge(const Dims &d, A psi, int i, int j, int l) {
   return 0.25 * (psi[d.nd(i, j, l)] + psi[d.nd(i, j - 1, l)] + psi[d.nd(i - 1, j, l)] +
                  psi[d.nd(i - 1, j - 1, l)]);
 }
 
 // cabaret_cell_boundaries_time_integrated (qg3p_aux.F90:1235-1313).  One
 // thread per (i, j) in 3..g+1 x 3..g+1 covers both face sets.
-__global__ void k_cell_boundaries(Dims d, double tw, const double *__restrict__ psi,
-                                  double *__restrict__ ui, double *__restrict__ uj,
-                                  double *__restrict__ uimo, double *__restrict__ ujmo) {
+template <typename TS>
+__global__ void k_cell_boundaries(Dims d, double tw, R<TS> psi,
+                                  W<TS> ui, W<TS> uj,
+                                  W<TS> uimo, W<TS> ujmo) {
   PQ_IDX(3, d.g - 1, 3, d.g - 1);
   const int g = d.g;
   const double pe = psi_edge(d, psi, i, j, l);
@@ -205,8 +236,9 @@ __global__ void k_cell_boundaries(Dims d, double tw, const double *__restrict__ 
 }
 
 // cabaret_cell_boundaries_velocity (qg3p_aux.F90:1168-1231)
-__global__ void k_cell_boundaries_velocity(Dims d, const double *__restrict__ psi,
-                                           double *__restrict__ ui, double *__restrict__ uj) {
+template <typename TS>
+__global__ void k_cell_boundaries_velocity(Dims d, R<TS> psi,
+                                           W<TS> ui, W<TS> uj) {
   PQ_IDX(3, d.g - 1, 3, d.g - 1);
   const int g = d.g;
   const double pe = psi_edge(d, psi, i, j, l);
@@ -263,11 +295,12 @@ __device__ __forceinline__ double extrap_face(double dt, double u1, double fi1, 
 }
 
 // cabaret_extrapolator, new face values into fin / fjn.
-__global__ void k_extrapolator(Dims d, double dt, const double *__restrict__ zeta,
-                               const double *__restrict__ zeta_old, const double *__restrict__ ui,
-                               const double *__restrict__ uj, const double *__restrict__ zfi,
-                               const double *__restrict__ zfj, double *__restrict__ fin,
-                               double *__restrict__ fjn) {
+template <typename TS>
+__global__ void k_extrapolator(Dims d, double dt, R<TS> zeta,
+                               R<TS> zeta_old, R<TS> ui,
+                               R<TS> uj, R<TS> zfi,
+                               R<TS> zfj, W<TS> fin,
+                               W<TS> fjn) {
   PQ_IDX(3, d.g - 1, 3, d.g - 1);
   const int g = d.g;
   if (j <= g) {  // i-face (i, j), i in 3..g+1
@@ -293,8 +326,9 @@ __global__ void k_extrapolator(Dims d, double dt, const double *__restrict__ zet
 }
 
 // Copy the extrapolated faces back (qg3p_aux.F90:1526-1543).
-__global__ void k_face_copy(Dims d, const double *__restrict__ fin, const double *__restrict__ fjn,
-                            double *__restrict__ zfi, double *__restrict__ zfj) {
+template <typename TS>
+__global__ void k_face_copy(Dims d, R<TS> fin, R<TS> fjn,
+                            W<TS> zfi, W<TS> zfj) {
   PQ_IDX(3, d.g - 1, 3, d.g - 1);
   if (j <= d.g) zfi[d.fi(i, j, l)] = fin[d.fi(i, j, l)];
   if (i <= d.g) zfj[d.fj(i, j, l)] = fjn[d.fj(i, j, l)];
@@ -302,9 +336,10 @@ __global__ void k_face_copy(Dims d, const double *__restrict__ fin, const double
 
 // cabaret_flux (qg3p_aux.F90:1547-1594): i-faces 2..g+2 x 3..g, j-faces
 // 3..g x 2..g+2.
-__global__ void k_flux(Dims d, const double *__restrict__ ui, const double *__restrict__ uj,
-                       const double *__restrict__ zfi, const double *__restrict__ zfj,
-                       double *__restrict__ flux_i, double *__restrict__ flux_j) {
+template <typename TS>
+__global__ void k_flux(Dims d, R<TS> ui, R<TS> uj,
+                       R<TS> zfi, R<TS> zfj,
+                       W<TS> flux_i, W<TS> flux_j) {
   PQ_IDX(2, d.g + 1, 2, d.g + 1);
   const int g = d.g;
   if (j >= 3 && j <= g) flux_i[d.fi(i, j, l)] = ui[d.fi(i, j, l)] * zfi[d.fi(i, j, l)];
@@ -312,9 +347,10 @@ __global__ void k_flux(Dims d, const double *__restrict__ ui, const double *__re
 }
 
 // cabaret_corrector (qg3p_aux.F90:1597-1642)
-__global__ void k_corrector(Dims d, double dt, const double *__restrict__ flux_i,
-                            const double *__restrict__ flux_j, double *__restrict__ res,
-                            double *__restrict__ zeta) {
+template <typename TS>
+__global__ void k_corrector(Dims d, double dt, R<TS> flux_i,
+                            R<TS> flux_j, W<TS> res,
+                            W<TS> zeta) {
   PQ_IDX(3, d.g - 2, 3, d.g - 2);
   const double r = -(flux_i[d.fi(i + 1, j, l)] - flux_i[d.fi(i, j, l)] + flux_j[d.fj(i, j + 1, l)] -
                      flux_j[d.fj(i, j, l)]);
@@ -325,9 +361,10 @@ __global__ void k_corrector(Dims d, double dt, const double *__restrict__ flux_i
 // cabaret_corrector writing the corrected field into zeta_out (the step's
 // zeta_old for the next step) from zeta_in; boundary values are copied.
 // Covers all owned nodes 2..g+1.
-__global__ void k_corrector_to(Dims d, double dt, const double *__restrict__ flux_i,
-                               const double *__restrict__ flux_j, double *__restrict__ res,
-                               const double *__restrict__ zeta_in, double *__restrict__ zeta_out) {
+template <typename TS>
+__global__ void k_corrector_to(Dims d, double dt, R<TS> flux_i,
+                               R<TS> flux_j, W<TS> res,
+                               R<TS> zeta_in, W<TS> zeta_out) {
   PQ_IDX(1, d.g + 2, 1, d.g + 2);
   const int g = d.g;
   const size_t a = d.nd(i, j, l);
@@ -345,8 +382,9 @@ __global__ void k_corrector_to(Dims d, double dt, const double *__restrict__ flu
 // nodes 2..g+1.  Boundary nodes take the boundary formula; on corners the
 // left/right (j) edge formula wins, as it is applied last in the reference.
 // apply_rel_bc (qg3p_aux.F90:778-915) at a boundary node (i, j) of layer l.
-__device__ __forceinline__ double rel_bc(const Dims &d, int bc_type, double alpha,
-                                         const double *__restrict__ psi, int i, int j, int l) {
+template <typename A>
+__device__ __forceinline__ double rel_bc(const Dims &d, int bc_type, double alpha, A psi, int i,
+                                         int j, int l) {
   const int g = d.g;
   const size_t a = d.nd(i, j, l);
   const bool jb = (j == 2 || j == g + 1);
@@ -372,9 +410,10 @@ __device__ __forceinline__ double rel_bc(const Dims &d, int bc_type, double alph
   return r;
 }
 
+template <typename TS>
 __global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
-                      int bc_type, double alpha, const double *__restrict__ zeta,
-                      const double *__restrict__ psi, double *__restrict__ rel) {
+                      int bc_type, double alpha, R<TS> zeta,
+                      R<TS> psi, W<TS> rel) {
   PQ_IDX(2, d.g, 2, d.g);
   const int g = d.g, L = d.L;
   const size_t a = d.nd(i, j, l);
@@ -396,8 +435,9 @@ __global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__res
 
 // rel_from_psi (qg3p_aux.F90:579-603): the 5-point Laplacian on the
 // interior (the reference's term order), apply_rel_bc on the boundary.
-__global__ void k_rel_from_psi(Dims d, int bc_type, double alpha, const double *__restrict__ psi,
-                               double *__restrict__ rel) {
+template <typename TS>
+__global__ void k_rel_from_psi(Dims d, int bc_type, double alpha, R<TS> psi,
+                               W<TS> rel) {
   PQ_IDX(2, d.g, 2, d.g);
   const int g = d.g;
   const size_t a = d.nd(i, j, l);
@@ -409,9 +449,10 @@ __global__ void k_rel_from_psi(Dims d, int bc_type, double alpha, const double *
 }
 
 // update_zeta_bc (qg3p_aux.F90:643-734): boundary nodes only.
+template <typename TS>
 __global__ void k_zeta_bc(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
-                          const double *__restrict__ rel, const double *__restrict__ psi,
-                          double *__restrict__ zeta) {
+                          R<TS> rel, R<TS> psi,
+                          W<TS> zeta) {
   PQ_IDX(2, d.g, 2, d.g);
   const int g = d.g, L = d.L;
   if (!(i == 2 || i == g + 1 || j == 2 || j == g + 1)) return;
@@ -426,7 +467,8 @@ __global__ void k_zeta_bc(Dims d, const double *__restrict__ s1, const double *_
 
 // max |u| over the cabaret_cfl face ranges (qg3p_aux.F90:937-940); the
 // non-negative doubles are ordered like their bit patterns.
-__global__ void k_maxabs(Dims d, const double *__restrict__ ui, const double *__restrict__ uj,
+template <typename TS>
+__global__ void k_maxabs(Dims d, R<TS> ui, R<TS> uj,
                          unsigned long long *__restrict__ out) {
   const int i = 3 + (int)(blockIdx.x * BX + threadIdx.x);
   const int j = 3 + (int)(blockIdx.y * BY + threadIdx.y);
@@ -477,7 +519,7 @@ int pq_cabaret_cfl_dev(int g, int L, double dt, const double *ui, const double *
   unsigned long long *dm = nullptr;
   PQ_HIP(hipMallocAsync((void **)&dm, sizeof(unsigned long long), S(stream)));
   PQ_HIP(hipMemsetAsync(dm, 0, sizeof(unsigned long long), S(stream)));
-  hipLaunchKernelGGL(k_maxabs, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, ui, uj, dm);
+  hipLaunchKernelGGL(k_maxabs<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, ui, uj, dm);
   unsigned long long bits = 0;
   PQ_HIP(hipMemcpyAsync(&bits, dm, sizeof(bits), hipMemcpyDeviceToHost, S(stream)));
   PQ_HIP(hipFreeAsync(dm, S(stream)));
@@ -491,7 +533,7 @@ int pq_cabaret_cfl_dev(int g, int L, double dt, const double *ui, const double *
 int pq_cabaret_predictor_dev(int g, int L, double dt, const double *zeta_old, const double *res,
                              double *zeta, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_predictor_dev");
-  hipLaunchKernelGGL(k_predictor, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+  hipLaunchKernelGGL(k_predictor<double>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
                      dt, zeta_old, res, zeta);
   return check_launch("pq_cabaret_predictor_dev");
 }
@@ -499,7 +541,7 @@ int pq_cabaret_predictor_dev(int g, int L, double dt, const double *zeta_old, co
 int pq_cabaret_beta_dev(int g, int L, double dt, double tw_a, const double *beta, const double *ui,
                         double *zeta, double *res_beta, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_beta_dev");
-  hipLaunchKernelGGL(k_beta, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, dt,
+  hipLaunchKernelGGL(k_beta<double>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, dt,
                      tw_a, beta, ui, zeta, res_beta);
   return check_launch("pq_cabaret_beta_dev");
 }
@@ -507,7 +549,7 @@ int pq_cabaret_beta_dev(int g, int L, double dt, double tw_a, const double *beta
 int pq_cabaret_forcing_dev(int g, int L, double dt, const double *fw, double visc,
                            double visc_bot, const double *rel, double *zeta, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_forcing_dev");
-  hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+  hipLaunchKernelGGL(k_forcing<double>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
                      dt, fw, visc, visc_bot, rel, zeta);
   return check_launch("pq_cabaret_forcing_dev");
 }
@@ -515,7 +557,7 @@ int pq_cabaret_forcing_dev(int g, int L, double dt, const double *fw, double vis
 int pq_cabaret_cell_boundaries_dev(int g, int L, double tw, const double *psi, double *ui,
                                    double *uj, double *uimo, double *ujmo, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_cell_boundaries_dev");
-  hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream),
+  hipLaunchKernelGGL(k_cell_boundaries<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream),
                      Dims{g, L}, tw, psi, ui, uj, uimo, ujmo);
   return check_launch("pq_cabaret_cell_boundaries_dev");
 }
@@ -523,7 +565,7 @@ int pq_cabaret_cell_boundaries_dev(int g, int L, double tw, const double *psi, d
 int pq_cabaret_cell_boundaries_velocity_dev(int g, int L, const double *psi, double *ui,
                                             double *uj, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_cell_boundaries_velocity_dev");
-  hipLaunchKernelGGL(k_cell_boundaries_velocity, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0,
+  hipLaunchKernelGGL(k_cell_boundaries_velocity<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0,
                      S(stream), Dims{g, L}, psi, ui, uj);
   return check_launch("pq_cabaret_cell_boundaries_velocity_dev");
 }
@@ -535,9 +577,9 @@ int pq_cabaret_extrapolator_dev(int g, int L, double dt, const double *zeta,
   Dims d{g, L};
   double *tmp = scratch;
   if (!tmp) PQ_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (d.n_fi() + d.n_fj()), S(stream)));
-  hipLaunchKernelGGL(k_extrapolator, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, dt,
+  hipLaunchKernelGGL(k_extrapolator<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, dt,
                      zeta, zeta_old, ui, uj, zfi, zfj, tmp, tmp + d.n_fi());
-  hipLaunchKernelGGL(k_face_copy, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, tmp,
+  hipLaunchKernelGGL(k_face_copy<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, tmp,
                      tmp + d.n_fi(), zfi, zfj);
   if (!scratch) PQ_HIP(hipFreeAsync(tmp, S(stream)));
   return check_launch("pq_cabaret_extrapolator_dev");
@@ -546,7 +588,7 @@ int pq_cabaret_extrapolator_dev(int g, int L, double dt, const double *zeta,
 int pq_cabaret_flux_dev(int g, int L, const double *ui, const double *uj, const double *zfi,
                         const double *zfj, double *flux_i, double *flux_j, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_flux_dev");
-  hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, ui,
+  hipLaunchKernelGGL(k_flux<double>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, ui,
                      uj, zfi, zfj, flux_i, flux_j);
   return check_launch("pq_cabaret_flux_dev");
 }
@@ -554,7 +596,7 @@ int pq_cabaret_flux_dev(int g, int L, const double *ui, const double *uj, const 
 int pq_cabaret_corrector_dev(int g, int L, double dt, const double *flux_i, const double *flux_j,
                              double *res, double *zeta, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_corrector_dev");
-  hipLaunchKernelGGL(k_corrector, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+  hipLaunchKernelGGL(k_corrector<double>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
                      dt, flux_i, flux_j, res, zeta);
   return check_launch("pq_cabaret_corrector_dev");
 }
@@ -567,7 +609,7 @@ int pq_rel_from_zeta_and_psi_dev(int g, int L, const double *s1, const double *s
   // one layer: set(rel, zeta) copies the whole data array, halo included
   if (L == 1)
     PQ_HIP(hipMemcpyAsync(rel, zeta, sizeof(double) * Dims{g, L}.n_nd(), hipMemcpyDeviceToDevice, S(stream)));
-  hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
+  hipLaunchKernelGGL(k_rel<double>, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
                      bc_type, alpha, zeta, psi, rel);
   return check_launch("pq_rel_from_zeta_and_psi_dev");
 }
@@ -575,7 +617,7 @@ int pq_rel_from_zeta_and_psi_dev(int g, int L, const double *s1, const double *s
 int pq_update_zeta_bc_dev(int g, int L, const double *s1, const double *s2, const double *rel,
                           const double *psi, double *zeta, void *stream) {
   CHECK_GL(g, L, "pq_update_zeta_bc_dev");
-  hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
+  hipLaunchKernelGGL(k_zeta_bc<double>, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, s1, s2,
                      rel, psi, zeta);
   return check_launch("pq_update_zeta_bc_dev");
 }
@@ -585,7 +627,7 @@ int pq_zeta_step_dev(int g, int L, double T, const double *beta, const double *s
                      double visc_bot, const double *psi, const double *rel, const double *rel_p,
                      const double *src, double *dst, void *stream) {
   CHECK_GL(g, L, "pq_zeta_step_dev");
-  hipLaunchKernelGGL(k_zeta_step, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
+  hipLaunchKernelGGL(k_zeta_step<double>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, S(stream), Dims{g, L},
                      T, beta, s1, s2, force_wind, viscosity, visc_bot, psi, rel, rel_p, src, dst);
   return check_launch("pq_zeta_step_dev");
 }
@@ -593,7 +635,7 @@ int pq_zeta_step_dev(int g, int L, double T, const double *beta, const double *s
 int pq_ra_filter_dev(int g, int L, double *phi_l, double *phi_m, const double *phi_old_l,
                      double *phi_old_m, void *stream) {
   CHECK_GL(g, L, "pq_ra_filter_dev");
-  hipLaunchKernelGGL(k_ra_filter, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, phi_l,
+  hipLaunchKernelGGL(k_ra_filter<double>, grid_for(g, g, L), dim3(BX, BY), 0, S(stream), Dims{g, L}, phi_l,
                      phi_m, phi_old_l, phi_old_m);
   return check_launch("pq_ra_filter_dev");
 }
@@ -616,6 +658,10 @@ struct pq_model {
   int scheme = PQ_SCHEME_CABARET, lf = 1;
   double *psi2 = nullptr, *rel2 = nullptr, *zeta2 = nullptr, *zeta_old2 = nullptr;
   size_t bytes = 0;  // device bytes of the fields above
+  // bytes per value of the fields above (8, or 4 for the FP32-storage
+  // precisions); fw, s1, s2 and beta are always FP64.  The stage kernels
+  // compute in FP64 either way.
+  int es = 8;
   unsigned long long *maxu = nullptr;
   unsigned long long *maxu_host = nullptr;
   // scalars
@@ -681,7 +727,10 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
                "pq_model_create: unknown scheme %d", p->scheme);
   PQ_CHECK_ARG(p->scheme == PQ_SCHEME_CABARET || !p->adapt_dt,
                "pq_model_create: adapt_dt is a CABARET option");
+  PQ_CHECK_ARG(p->precision >= 0 && p->precision <= PQ_PREC_FP32_WIDE,
+               "pq_model_create: bad precision %d", p->precision);
   pq_model *m = new pq_model();
+  m->es = (p->precision == PQ_PREC_FP64 || p->precision == PQ_PREC_MIXED) ? 8 : 4;
   m->scheme = p->scheme;
   m->g = p->g;
   m->L = p->layers;
@@ -700,10 +749,8 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
     double **p;
     size_t n;
   };
-  std::vector<A> allocs = {{&m->psi, d.n_nd()},  {&m->rel, d.n_nd()},    {&m->zeta, d.n_nd()},
-                           {&m->zeta_old, d.n_nd()}, {&m->fw, (size_t)p->g * p->g},
-                           {&m->s1, (size_t)p->layers}, {&m->s2, (size_t)p->layers},
-                           {&m->beta, (size_t)p->layers}};
+  std::vector<A> allocs = {{&m->psi, d.n_nd()}, {&m->rel, d.n_nd()}, {&m->zeta, d.n_nd()},
+                           {&m->zeta_old, d.n_nd()}};
   if (m->scheme == PQ_SCHEME_CABARET)
     allocs.insert(allocs.end(), {{&m->zfi, d.n_fi()}, {&m->ui, d.n_fi()}, {&m->uimo, d.n_fi()},
                                  {&m->flux_i, d.n_fi()}, {&m->zfj, d.n_fj()}, {&m->uj, d.n_fj()},
@@ -712,9 +759,15 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
   else
     allocs.insert(allocs.end(), {{&m->psi2, d.n_nd()}, {&m->rel2, d.n_nd()}, {&m->zeta2, d.n_nd()},
                                  {&m->zeta_old2, d.n_nd()}});
+  std::vector<A> fp64 = {{&m->fw, (size_t)p->g * p->g}, {&m->s1, (size_t)p->layers},
+                         {&m->s2, (size_t)p->layers}, {&m->beta, (size_t)p->layers}};
   for (auto &a : allocs) {
+    if (hipMalloc((void **)a.p, a.n * m->es) != hipSuccess) return fail(PQ_ERR_NOMEM);
+    if (hipMemset(*a.p, 0, a.n * m->es) != hipSuccess) return fail(PQ_ERR_HIP);
+    m->bytes += a.n * m->es;
+  }
+  for (auto &a : fp64) {
     if (hipMalloc((void **)a.p, a.n * sizeof(double)) != hipSuccess) return fail(PQ_ERR_NOMEM);
-    if (hipMemset(*a.p, 0, a.n * sizeof(double)) != hipSuccess) return fail(PQ_ERR_HIP);
     m->bytes += a.n * sizeof(double);
   }
   if (hipMalloc((void **)&m->maxu, sizeof(unsigned long long)) != hipSuccess) return fail(PQ_ERR_NOMEM);
@@ -741,14 +794,33 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
 
 void *pq_model_stream(pq_model *m) { return m ? (void *)m->stream : nullptr; }
 
-static int invert_into(pq_model *m, const double *zeta, double *psi) {
+}  // extern "C"
+
+// The model's steps, templated on the field storage type; the C entry
+// points below keep the C linkage of their declarations.
+// A model field viewed with its storage type.
+template <typename TS> static inline TS *fld(double *p) { return reinterpret_cast<TS *>(p); }
+#define PQ_BY_STORAGE(m, f, ...) ((m)->es == 4 ? f<float>(__VA_ARGS__) : f<double>(__VA_ARGS__))
+
+template <typename TS> static int invert_into(pq_model *m, double *zeta, double *psi) {
   const int ld = m->g + 2;
-  const void *z[1] = {zeta + 1 + ld};
-  void *p[1] = {psi + 1 + ld};
+  const void *z[1] = {fld<TS>(zeta) + 1 + ld};
+  void *p[1] = {fld<TS>(psi) + 1 + ld};
   return pq_inv_run_dev(m->inv, 1, z, ld, ld, p, ld, ld, m->stream);
 }
 
-static int model_invert(pq_model *m) { return invert_into(m, m->zeta, m->psi); }
+static int model_invert(pq_model *m) { return PQ_BY_STORAGE(m, invert_into, m, m->zeta, m->psi); }
+
+// rel_from_zeta_and_psi and update_zeta_bc (qg3p_aux.F90:606-640, 643-734)
+// on the model's fields.
+template <typename TS>
+static void launch_rel(pq_model *m, double *zeta, double *psi, double *rel, hipStream_t s) {
+  const int g = m->g, L = m->L;
+  hipLaunchKernelGGL(k_rel<TS>, grid_for(g, g, L), dim3(BX, BY), 0, s, m->d, m->s1, m->s2,
+                     m->bc_type, m->alpha, fld<TS>(zeta), fld<TS>(psi), fld<TS>(rel));
+  hipLaunchKernelGGL(k_zeta_bc<TS>, grid_for(g, g, L), dim3(BX, BY), 0, s, m->d, m->s1, m->s2,
+                     fld<TS>(rel), fld<TS>(psi), fld<TS>(zeta));
+}
 
 // Arrays of time level k (1 or 2) of a centred-scheme model.
 struct Level {
@@ -760,17 +832,19 @@ static Level level(pq_model *m, int k) {
 }
 
 // pv_inversion, rel_from_zeta_and_psi and update_zeta_bc of one level
-// (qg3.F90:383-391, 640-645).
-static int lf_diagnose(pq_model *m, const Level &v) {
-  int rc = invert_into(m, v.zeta, v.psi);
+// (qg3.F90:383-391, 640-645).  A one-layer model's rel is zeta itself
+// (set(rel, zeta), halo included) before the boundary values.
+template <typename TS> static int lf_diagnose(pq_model *m, const Level &v) {
+  int rc = invert_into<TS>(m, v.zeta, v.psi);
   if (rc) return rc;
-  rc = pq_rel_from_zeta_and_psi_dev(m->g, m->L, m->s1, m->s2, m->bc_type, m->alpha, v.zeta, v.psi,
-                                    v.rel, m->stream);
-  if (rc) return rc;
-  return pq_update_zeta_bc_dev(m->g, m->L, m->s1, m->s2, v.rel, v.psi, v.zeta, m->stream);
+  if (m->L == 1)
+    PQ_HIP(hipMemcpyAsync(v.rel, v.zeta, sizeof(TS) * m->d.n_nd(), hipMemcpyDeviceToDevice, m->stream));
+  launch_rel<TS>(m, v.zeta, v.psi, v.rel, m->stream);
+  return check_launch("pq_model_step");
 }
 
-// The time-mean accumulation after the inversion (qg3.F90:672-681).
+// The time-mean accumulation after the inversion (qg3.F90:672-681); only
+// FP64-storage models average (pq_model_average).
 static int model_accumulate(pq_model *m, const double *psi, hipStream_t s) {
   if (m->average && m->model_time > m->max_spin) {
     const int ld = m->g + 2;
@@ -784,18 +858,19 @@ static int model_accumulate(pq_model *m, const double *psi, hipStream_t s) {
 // One pass of the centred scheme's DOLEAPFROG body (qg3.F90:612-660):
 // zeta_centred into level lf, the Robert-Asselin-Williams filter, then
 // the level's inversion, rel and zeta boundary, model_time += dt.
-static int lf_step(pq_model *m) {
+template <typename TS> static int lf_step(pq_model *m) {
   const Level vl = level(m, m->lf), vm = level(m, 3 - m->lf);
   const int g = m->g, L = m->L;
   const hipStream_t s = m->stream;
-  hipLaunchKernelGGL(k_zeta_step, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d, 2.0 * m->dt,
-                     m->beta, m->s1, m->s2, m->fw, m->visc, m->visc_bot, vm.psi, vm.rel, vl.rel,
-                     vl.zeta, vl.zeta);
-  hipLaunchKernelGGL(k_ra_filter, grid_for(g, g, L), dim3(BX, BY), 0, s, m->d, vl.zeta, vm.zeta,
-                     vl.zeta_old, vm.zeta_old);
+  hipLaunchKernelGGL(k_zeta_step<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d,
+                     2.0 * m->dt, m->beta, m->s1, m->s2, m->fw, m->visc, m->visc_bot,
+                     fld<TS>(vm.psi), fld<TS>(vm.rel), fld<TS>(vl.rel), fld<TS>(vl.zeta),
+                     fld<TS>(vl.zeta));
+  hipLaunchKernelGGL(k_ra_filter<TS>, grid_for(g, g, L), dim3(BX, BY), 0, s, m->d, fld<TS>(vl.zeta),
+                     fld<TS>(vm.zeta), fld<TS>(vl.zeta_old), fld<TS>(vm.zeta_old));
   int rc = check_launch("pq_model_step");
   if (rc) return rc;
-  rc = lf_diagnose(m, vl);
+  rc = lf_diagnose<TS>(m, vl);
   if (rc) return rc;
   m->model_time += m->dt;
   rc = model_accumulate(m, vl.psi, s);
@@ -807,61 +882,43 @@ static int lf_step(pq_model *m) {
 // Start of the centred scheme (qg3.F90:383-391, 449-452, 456-490): level 1
 // from the installed zeta, zeta_old(1) = zeta(1), the forward Euler step
 // into level 2 (FORWARD_EULER_START), zeta_old(2) = zeta(2), model_time = dt.
-static int lf_init(pq_model *m) {
-  const Dims &d = m->d;
+template <typename TS> static int lf_init(pq_model *m) {
+  const size_t nb = sizeof(TS) * m->d.n_nd();
   const Level v1 = level(m, 1), v2 = level(m, 2);
-  int rc = lf_diagnose(m, v1);
+  int rc = lf_diagnose<TS>(m, v1);
   if (rc) return rc;
-  PQ_HIP(hipMemcpyAsync(v1.zeta_old, v1.zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice,
-                        m->stream));
-  PQ_HIP(hipMemsetAsync(v2.zeta, 0, sizeof(double) * d.n_nd(), m->stream));
-  hipLaunchKernelGGL(k_zeta_step, grid_for(m->g - 2, m->g - 2, m->L), dim3(BX, BY), 0, m->stream,
-                     m->d, m->dt, m->beta, m->s1, m->s2, m->fw, m->visc, m->visc_bot, v1.psi, v1.rel,
-                     v1.rel, v1.zeta, v2.zeta);
+  PQ_HIP(hipMemcpyAsync(v1.zeta_old, v1.zeta, nb, hipMemcpyDeviceToDevice, m->stream));
+  PQ_HIP(hipMemsetAsync(v2.zeta, 0, nb, m->stream));
+  hipLaunchKernelGGL(k_zeta_step<TS>, grid_for(m->g - 2, m->g - 2, m->L), dim3(BX, BY), 0,
+                     m->stream, m->d, m->dt, m->beta, m->s1, m->s2, m->fw, m->visc, m->visc_bot,
+                     fld<TS>(v1.psi), fld<TS>(v1.rel), fld<TS>(v1.rel), fld<TS>(v1.zeta),
+                     fld<TS>(v2.zeta));
   rc = check_launch("pq_model_init");
   if (rc) return rc;
-  rc = lf_diagnose(m, v2);
+  rc = lf_diagnose<TS>(m, v2);
   if (rc) return rc;
-  PQ_HIP(hipMemcpyAsync(v2.zeta_old, v2.zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice,
-                        m->stream));
+  PQ_HIP(hipMemcpyAsync(v2.zeta_old, v2.zeta, nb, hipMemcpyDeviceToDevice, m->stream));
   m->model_time = 0.0 + m->dt;
   m->lf = 1;
   PQ_HIP(hipStreamSynchronize(m->stream));
   return pq_inv_status(m->inv, nullptr);
 }
 
-int pq_model_init(pq_model *m, const double *zeta_sdata) {
-  PQ_CHECK_ARG(m && zeta_sdata, "pq_model_init: null argument");
-  const int g = m->g, L = m->L, ld = g + 2;
+// START_PERTURB (qg3.F90:393-425) with zeta's data array already on the
+// device: CABARET work arrays zeroed, psi from the inversion, rel and the
+// zeta boundary, zeta_old = zeta, model_time = dt / 2.
+template <typename TS> static int cabaret_init(pq_model *m) {
   const Dims &d = m->d;
-  std::vector<double> z(d.n_nd(), 0.0);
-  for (int l = 0; l < L; ++l)
-    for (int j = 2; j <= g - 1; ++j)
-      for (int i = 2; i <= g - 1; ++i)
-        z[d.nd(i + 1, j + 1, l)] = zeta_sdata[(size_t)(i - 1) + (size_t)g * ((j - 1) + (size_t)g * l)];
-  if (m->scheme == PQ_SCHEME_LEAPFROG) {
-    for (double *p : {m->psi, m->rel, m->psi2, m->rel2})
-      PQ_HIP(hipMemsetAsync(p, 0, sizeof(double) * d.n_nd(), m->stream));
-    PQ_HIP(hipMemcpyAsync(m->zeta, z.data(), sizeof(double) * d.n_nd(), hipMemcpyHostToDevice, m->stream));
-    (void)ld;
-    return lf_init(m);
-  }
-  // zero CABARET arrays (qg3.F90:413-424)
-  for (double *p : {m->zfi, m->zfj, m->ui, m->uj, m->uimo, m->ujmo, m->flux_i, m->flux_j})
-    PQ_HIP(hipMemsetAsync(p, 0, sizeof(double) * (p == m->zfi || p == m->ui || p == m->uimo || p == m->flux_i ? d.n_fi() : d.n_fj()), m->stream));
-  PQ_HIP(hipMemsetAsync(m->res, 0, sizeof(double) * d.n_rs(), m->stream));
-  PQ_HIP(hipMemsetAsync(m->res_beta, 0, sizeof(double) * d.n_rs(), m->stream));
-  PQ_HIP(hipMemsetAsync(m->psi, 0, sizeof(double) * d.n_nd(), m->stream));
-  PQ_HIP(hipMemsetAsync(m->rel, 0, sizeof(double) * d.n_nd(), m->stream));
-  PQ_HIP(hipMemcpyAsync(m->zeta, z.data(), sizeof(double) * d.n_nd(), hipMemcpyHostToDevice, m->stream));
-  (void)ld;
-  int rc = model_invert(m);
+  for (double *p : {m->zfi, m->ui, m->uimo, m->flux_i})
+    PQ_HIP(hipMemsetAsync(p, 0, sizeof(TS) * d.n_fi(), m->stream));
+  for (double *p : {m->zfj, m->uj, m->ujmo, m->flux_j})
+    PQ_HIP(hipMemsetAsync(p, 0, sizeof(TS) * d.n_fj(), m->stream));
+  PQ_HIP(hipMemsetAsync(m->res, 0, sizeof(TS) * d.n_rs(), m->stream));
+  PQ_HIP(hipMemsetAsync(m->res_beta, 0, sizeof(TS) * d.n_rs(), m->stream));
+  int rc = lf_diagnose<TS>(m, level(m, 1));
   if (rc) return rc;
-  rc = pq_rel_from_zeta_and_psi_dev(g, L, m->s1, m->s2, m->bc_type, m->alpha, m->zeta, m->psi, m->rel, m->stream);
-  if (rc) return rc;
-  rc = pq_update_zeta_bc_dev(g, L, m->s1, m->s2, m->rel, m->psi, m->zeta, m->stream);
-  if (rc) return rc;
-  PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, m->stream));
+  PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice,
+                        m->stream));
   m->tw_a = 0.5;
   m->tw_u = 0.5;
   m->model_time = m->dt / 2.0;
@@ -869,12 +926,55 @@ int pq_model_init(pq_model *m, const double *zeta_sdata) {
   return pq_inv_status(m->inv, nullptr);
 }
 
+// Host values (FP64) into / out of a model field of either storage type.
+static int field_upload(pq_model *m, double *dst, const double *src, size_t n, hipStream_t s) {
+  if (m->es == 8) {
+    PQ_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    PQ_HIP(hipStreamSynchronize(s));
+    return PQ_OK;
+  }
+  std::vector<float> f(src, src + n);
+  PQ_HIP(hipMemcpyAsync(dst, f.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+  PQ_HIP(hipStreamSynchronize(s));
+  return PQ_OK;
+}
+static int field_download(pq_model *m, double *dst, const double *src, size_t n, hipStream_t s) {
+  if (m->es == 8) {
+    PQ_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    PQ_HIP(hipStreamSynchronize(s));
+    return PQ_OK;
+  }
+  std::vector<float> f(n);
+  PQ_HIP(hipMemcpyAsync(f.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  PQ_HIP(hipStreamSynchronize(s));
+  std::copy(f.begin(), f.end(), dst);
+  return PQ_OK;
+}
+
+int pq_model_init(pq_model *m, const double *zeta_sdata) {
+  PQ_CHECK_ARG(m && zeta_sdata, "pq_model_init: null argument");
+  const int g = m->g, L = m->L;
+  const Dims &d = m->d;
+  std::vector<double> z(d.n_nd(), 0.0);
+  for (int l = 0; l < L; ++l)
+    for (int j = 2; j <= g - 1; ++j)
+      for (int i = 2; i <= g - 1; ++i)
+        z[d.nd(i + 1, j + 1, l)] = zeta_sdata[(size_t)(i - 1) + (size_t)g * ((j - 1) + (size_t)g * l)];
+  std::vector<double *> zero = {m->psi, m->rel};
+  if (m->scheme == PQ_SCHEME_LEAPFROG) zero.insert(zero.end(), {m->psi2, m->rel2});
+  for (double *p : zero) PQ_HIP(hipMemsetAsync(p, 0, m->es * d.n_nd(), m->stream));
+  int rc = field_upload(m, m->zeta, z.data(), d.n_nd(), m->stream);
+  if (rc) return rc;
+  if (m->scheme == PQ_SCHEME_LEAPFROG) return PQ_BY_STORAGE(m, lf_init, m);
+  return PQ_BY_STORAGE(m, cabaret_init, m);
+}
+
 // cabaret_adapt_dt (qg3p_aux.F90:946-990) with the scalar update on the host.
-static int model_adapt_dt(pq_model *m) {
+template <typename TS> static int model_adapt_dt(pq_model *m) {
   const Dims &d = m->d;
   PQ_HIP(hipMemsetAsync(m->maxu, 0, sizeof(unsigned long long), m->stream));
-  hipLaunchKernelGGL(k_maxabs, grid_for(m->g - 1, m->g - 1, m->L), dim3(BX, BY), 0, m->stream, d,
-                     m->ui, m->uj, m->maxu);
+  hipLaunchKernelGGL(k_maxabs<TS>, grid_for(m->g - 1, m->g - 1, m->L), dim3(BX, BY), 0, m->stream,
+                     d, fld<TS>(m->ui), fld<TS>(m->uj), m->maxu);
   PQ_HIP(hipMemcpyAsync(m->maxu_host, m->maxu, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         m->stream));
   PQ_HIP(hipStreamSynchronize(m->stream));
@@ -896,54 +996,60 @@ static int model_adapt_dt(pq_model *m) {
 
 // First half of a step: adapt_dt (host round trip when enabled), predictor
 // and beta term (qg3.F90:782-790).  The PV inversion follows.
-static int model_pre(pq_model *m, hipStream_t s) {
+template <typename TS> static int model_pre_t(pq_model *m, hipStream_t s) {
   if (m->adapt) {
-    int rc = model_adapt_dt(m);
+    int rc = model_adapt_dt<TS>(m);
     if (rc) return rc;
   }
   const int g = m->g, L = m->L;
-  hipLaunchKernelGGL(k_predictor_beta, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d, m->dt,
-                     m->tw_a, m->beta, m->zeta_old, m->res, m->ui, m->zeta, m->res_beta);
+  hipLaunchKernelGGL(k_predictor_beta<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d,
+                     m->dt, m->tw_a, m->beta, fld<TS>(m->zeta_old), fld<TS>(m->res),
+                     fld<TS>(m->ui), fld<TS>(m->zeta), fld<TS>(m->res_beta));
   return PQ_OK;
 }
+static int model_pre(pq_model *m, hipStream_t s) { return PQ_BY_STORAGE(m, model_pre_t, m, s); }
 
 // Second half, after psi is up to date: relative vorticity and boundary
 // values, forcing, cell boundaries, extrapolator, fluxes and (unless this is
 // a final partial step) the corrector with zeta_old <- zeta (qg3.F90:790-960).
-static int model_post(pq_model *m, hipStream_t s, bool last) {
+template <typename TS> static int model_post_t(pq_model *m, hipStream_t s, bool last) {
   const int g = m->g, L = m->L;
   const Dims &d = m->d;
   const double dt = m->dt;
   if (L == 1)
-    PQ_HIP(hipMemcpyAsync(m->rel, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_rel, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->bc_type,
-                     m->alpha, m->zeta, m->psi, m->rel);
-  hipLaunchKernelGGL(k_zeta_bc, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2, m->rel,
-                     m->psi, m->zeta);
+    PQ_HIP(hipMemcpyAsync(m->rel, m->zeta, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+  launch_rel<TS>(m, m->zeta, m->psi, m->rel, s);
   m->model_time += 0.5 * dt;
   {
     int rc = model_accumulate(m, m->psi, s);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_forcing, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw, m->visc,
-                     m->visc_bot, m->rel, m->zeta);
-  hipLaunchKernelGGL(k_cell_boundaries, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->tw_u,
-                     m->psi, m->ui, m->uj, m->uimo, m->ujmo);
-  hipLaunchKernelGGL(k_extrapolator, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, dt, m->zeta,
-                     m->zeta_old, m->ui, m->uj, m->zfi, m->zfj, m->fin, m->fjn);
-  hipLaunchKernelGGL(k_face_copy, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, m->fin, m->fjn,
-                     m->zfi, m->zfj);
-  hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, m->ui, m->uj, m->zfi,
-                     m->zfj, m->flux_i, m->flux_j);
+  TS *zeta = fld<TS>(m->zeta), *zeta_old = fld<TS>(m->zeta_old);
+  TS *ui = fld<TS>(m->ui), *uj = fld<TS>(m->uj), *zfi = fld<TS>(m->zfi), *zfj = fld<TS>(m->zfj);
+  TS *fin = fld<TS>(m->fin), *fjn = fld<TS>(m->fjn);
+  TS *flux_i = fld<TS>(m->flux_i), *flux_j = fld<TS>(m->flux_j);
+  hipLaunchKernelGGL(k_forcing<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw,
+                     m->visc, m->visc_bot, fld<TS>(m->rel), zeta);
+  hipLaunchKernelGGL(k_cell_boundaries<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d,
+                     m->tw_u, fld<TS>(m->psi), ui, uj, fld<TS>(m->uimo), fld<TS>(m->ujmo));
+  hipLaunchKernelGGL(k_extrapolator<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, dt, zeta,
+                     zeta_old, ui, uj, zfi, zfj, fin, fjn);
+  hipLaunchKernelGGL(k_face_copy<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, fin, fjn,
+                     zfi, zfj);
+  hipLaunchKernelGGL(k_flux<TS>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, ui, uj, zfi, zfj,
+                     flux_i, flux_j);
   if (!last) {
     // corrector into zeta_old, so that zeta_old <- zeta needs no copy; the
     // next predictor rewrites the interior of zeta and keeps its boundary.
-    hipLaunchKernelGGL(k_corrector_to, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
-                       m->flux_i, m->flux_j, m->res, m->zeta, m->zeta_old);
-    PQ_HIP(hipMemcpyAsync(m->zeta, m->zeta_old, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_corrector_to<TS>, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
+                       flux_i, flux_j, fld<TS>(m->res), zeta, zeta_old);
+    PQ_HIP(hipMemcpyAsync(zeta, zeta_old, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice, s));
     m->model_time += 0.5 * dt;
   }
   return check_launch("pq_model_step");
+}
+static int model_post(pq_model *m, hipStream_t s, bool last) {
+  return PQ_BY_STORAGE(m, model_post_t, m, s, last);
 }
 
 int pq_model_step(pq_model *m, int nsteps, int final_partial) {
@@ -951,7 +1057,7 @@ int pq_model_step(pq_model *m, int nsteps, int final_partial) {
   if (m->scheme == PQ_SCHEME_LEAPFROG) {
     PQ_CHECK_ARG(!final_partial, "pq_model_step: final_partial is a CABARET option");
     for (int it = 0; it < nsteps; ++it) {
-      int rc = lf_step(m);
+      int rc = PQ_BY_STORAGE(m, lf_step, m);
       if (rc) return rc;
     }
     return PQ_OK;
@@ -967,7 +1073,6 @@ int pq_model_step(pq_model *m, int nsteps, int final_partial) {
   return PQ_OK;
 }
 
-}  // extern "C"
 
 // ---- hooks for the ensemble runtime (pq_ensemble.hip) --------------------
 namespace pq {
@@ -976,12 +1081,14 @@ int model_step_post(pq_model *m, hipStream_t s, bool last) { return model_post(m
 int model_step_invert(pq_model *m) { return model_invert(m); }
 hipStream_t model_stream(pq_model *m) { return m->stream; }
 bool model_adaptive(const pq_model *m) { return m->adapt != 0; }
-void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld) {
+void model_views(pq_model *m, int *g, int *L, void **zeta_sdata, void **psi_sdata, long *ld, int *es) {
   *g = m->g;
   *L = m->L;
   *ld = m->g + 2;
-  *zeta_sdata = m->zeta + 1 + (m->g + 2);
-  *psi_sdata = m->psi + 1 + (m->g + 2);
+  *es = m->es;
+  const size_t off = (size_t)m->es * (1 + (m->g + 2));
+  *zeta_sdata = (char *)m->zeta + off;
+  *psi_sdata = (char *)m->psi + off;
 }
 ModelClock model_clock(const pq_model *m) { return {m->model_time, m->diag_time}; }
 void model_set_clock(pq_model *m, ModelClock c) {
@@ -1001,6 +1108,7 @@ ModelView model_view(pq_model *m) {
   v.g = m->g;
   v.L = m->L;
   v.scheme = m->scheme;
+  v.es = m->es;
   v.psi = m->psi;
   v.zeta = m->zeta;
   v.rel = m->rel;
@@ -1024,28 +1132,29 @@ ModelView model_view(pq_model *m) {
 // face values and the corrector's half step, zeta_old = zeta, model_time =
 // dt / 2, time weights 1/2.  Centred scheme: rel = rel_from_psi of both
 // levels, zeta_old = zeta, model_time = 0, level 1 next.
-int model_pickup(pq_model *m, double dt) {
+template <typename TS> static int pickup_t(pq_model *m, double dt) {
   const int g = m->g, L = m->L;
   const Dims &d = m->d;
   const hipStream_t s = m->stream;
   if (m->scheme == PQ_SCHEME_LEAPFROG) {
     for (int k = 1; k <= 2; ++k) {
       const Level v = level(m, k);
-      hipLaunchKernelGGL(k_rel_from_psi, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->bc_type, m->alpha,
-                         v.psi, v.rel);
-      PQ_HIP(hipMemcpyAsync(v.zeta_old, v.zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_rel_from_psi<TS>, grid_for(g, g, L), dim3(BX, BY), 0, s, d, m->bc_type,
+                         m->alpha, fld<TS>(v.psi), fld<TS>(v.rel));
+      PQ_HIP(hipMemcpyAsync(v.zeta_old, v.zeta, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice, s));
     }
     m->model_time = 0.0;
     m->lf = 1;
   } else {
     m->dt = dt;
-    hipLaunchKernelGGL(k_cell_boundaries_velocity, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d,
-                       m->psi, m->uimo, m->ujmo);
-    hipLaunchKernelGGL(k_flux, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, m->ui, m->uj, m->zfi,
-                       m->zfj, m->flux_i, m->flux_j);
-    hipLaunchKernelGGL(k_corrector, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->flux_i,
-                       m->flux_j, m->res, m->zeta);
-    PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta, sizeof(double) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+    TS *flux_i = fld<TS>(m->flux_i), *flux_j = fld<TS>(m->flux_j);
+    hipLaunchKernelGGL(k_cell_boundaries_velocity<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s,
+                       d, fld<TS>(m->psi), fld<TS>(m->uimo), fld<TS>(m->ujmo));
+    hipLaunchKernelGGL(k_flux<TS>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, fld<TS>(m->ui),
+                       fld<TS>(m->uj), fld<TS>(m->zfi), fld<TS>(m->zfj), flux_i, flux_j);
+    hipLaunchKernelGGL(k_corrector<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, flux_i,
+                       flux_j, fld<TS>(m->res), fld<TS>(m->zeta));
+    PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice, s));
     m->model_time = dt / 2.0;
     m->tw_a = 0.5;
     m->tw_u = 0.5;
@@ -1055,6 +1164,7 @@ int model_pickup(pq_model *m, double dt) {
   PQ_HIP(hipStreamSynchronize(s));
   return PQ_OK;
 }
+int model_pickup(pq_model *m, double dt) { return PQ_BY_STORAGE(m, pickup_t, m, dt); }
 // the same two half-step additions as model_post, so times round alike
 void model_account_step(pq_model *m) {
   m->model_time += 0.5 * m->dt;
@@ -1071,9 +1181,7 @@ int pq_model_get(pq_model *m, int which, double *host, size_t count) {
   double **f = model_field(m, which, &n);
   PQ_CHECK_ARG(f != nullptr, "pq_model_get: unknown field %d", which);
   PQ_CHECK_ARG(count >= n, "pq_model_get: field %d needs %zu values, buffer holds %zu", which, n, count);
-  PQ_HIP(hipStreamSynchronize(m->stream));
-  PQ_HIP(hipMemcpy(host, *f, n * sizeof(double), hipMemcpyDeviceToHost));
-  return PQ_OK;
+  return field_download(m, host, *f, n, m->stream);
 }
 
 int pq_model_set(pq_model *m, int which, const double *host, size_t count) {
@@ -1083,8 +1191,7 @@ int pq_model_set(pq_model *m, int which, const double *host, size_t count) {
   PQ_CHECK_ARG(f != nullptr, "pq_model_set: unknown field %d", which);
   PQ_CHECK_ARG(count >= n, "pq_model_set: field %d needs %zu values, buffer holds %zu", which, n, count);
   PQ_HIP(hipStreamSynchronize(m->stream));
-  PQ_HIP(hipMemcpy(*f, host, n * sizeof(double), hipMemcpyHostToDevice));
-  return PQ_OK;
+  return field_upload(m, *f, host, n, m->stream);
 }
 
 int pq_model_scalars(pq_model *m, double *out4) {
@@ -1098,6 +1205,7 @@ int pq_model_scalars(pq_model *m, double *out4) {
 
 int pq_model_average(pq_model *m, int enable, double max_spin) {
   PQ_CHECK_ARG(m != nullptr, "pq_model_average: null model");
+  PQ_CHECK_ARG(!enable || m->es == 8, "pq_model_average: the time mean needs an FP64-storage model");
   PQ_HIP(hipStreamSynchronize(m->stream));
   if (enable) {
     if (!m->psi_av && hipMalloc((void **)&m->psi_av, sizeof(double) * m->d.n_rs()) != hipSuccess) {
@@ -1128,6 +1236,7 @@ int pq_model_psi_av(pq_model *m, double *host, size_t count, double *diag_time) 
 
 int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot) {
   PQ_CHECK_ARG(m && h && ekin && epot, "pq_model_energy: null argument");
+  PQ_CHECK_ARG(m->es == 8, "pq_model_energy: the energy diagnostic needs an FP64-storage model");
   if (!m->diag || !std::equal(m->diag_h.begin(), m->diag_h.end(), h)) {
     pq_diag_destroy(m->diag);
     m->diag = nullptr;

@@ -158,12 +158,15 @@ int pq_ens_add_model(pq_ens *e, pq_model *model, const double *ss, const double 
   PQ_CHECK_ARG(!model_adaptive(model), "pq_ens_add_model: models with adapt_dt are stepped alone");
   PQ_CHECK_ARG(model_scheme(model) == PQ_SCHEME_CABARET,
                "pq_ens_add_model: centred-scheme models are stepped alone");
-  PQ_CHECK_ARG(e->prec == PQ_PREC_FP64 || e->prec == PQ_PREC_MIXED,
-               "pq_ens_add_model: model fields are FP64; use PQ_PREC_FP64 or PQ_PREC_MIXED");
-  int g, L;
+  int g, L, es;
   long ld;
-  double *z, *p;
-  model_views(model, &g, &L, &z, &p, &ld);
+  void *z, *p;
+  model_views(model, &g, &L, &z, &p, &ld, &es);
+  const int ens_es = (e->prec == PQ_PREC_FP64 || e->prec == PQ_PREC_MIXED) ? 8 : 4;
+  PQ_CHECK_ARG(es == ens_es,
+               "pq_ens_add_model: the model stores %s fields and the ensemble's precision takes %s "
+               "(FP64: PQ_PREC_FP64 or PQ_PREC_MIXED; FP32: the FP32-storage precisions)",
+               es == 8 ? "FP64" : "FP32", ens_es == 8 ? "FP64" : "FP32");
   int rc = pq_ens_add(e, g, L, ss, modes_to_z, z_to_modes, z, ld, ld, p, ld, ld, index);
   if (rc) return rc;
   e->mem.back().model = model;

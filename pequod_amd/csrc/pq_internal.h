@@ -36,9 +36,10 @@ bool model_adaptive(const pq_model *m);
 int model_scheme(const pq_model *m);  // PQ_SCHEME_*
 // Device arrays and scalars of a model (the reference's full data arrays;
 // CABARET-only arrays are null for the centred scheme and the second time
-// level is null for CABARET).
+// level is null for CABARET).  The field arrays hold es-byte values (8:
+// FP64, 4: FP32 storage); psi_av is FP64.
 struct ModelView {
-  int g, L, scheme;
+  int g, L, scheme, es;
   double *psi, *zeta, *rel, *zeta_old, *psi2, *zeta2;
   double *ui, *uj, *zfi, *zfj, *res, *res_beta, *psi_av;
   double dt, diag_time;
@@ -47,7 +48,8 @@ struct ModelView {
 ModelView model_view(pq_model *m);
 // The START_PICKUP start once a checkpoint's fields are on the device.
 int model_pickup(pq_model *m, double dt);
-void model_views(pq_model *m, int *g, int *L, double **zeta_sdata, double **psi_sdata, long *ld);
+// zeta / psi sdata views (leading dimension ld) of es-byte values.
+void model_views(pq_model *m, int *g, int *L, void **zeta_sdata, void **psi_sdata, long *ld, int *es);
 // Host-side clocks of a model: model time and the time-mean weight.
 struct ModelClock {
   double t, diag_t;

@@ -114,10 +114,27 @@ struct In {
   }
 };
 
-// Host copy of a device array.
-int d2h(std::vector<double> &h, const double *dev, size_t n) {
+// Host copy of a device array of es-byte values (8: FP64, 4: FP32, widened).
+int d2h(std::vector<double> &h, const double *dev, size_t n, int es = 8) {
   h.resize(n);
-  PQ_HIP(hipMemcpy(h.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (es == 8) {
+    PQ_HIP(hipMemcpy(h.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost));
+    return PQ_OK;
+  }
+  std::vector<float> f(n);
+  PQ_HIP(hipMemcpy(f.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost));
+  std::copy(f.begin(), f.end(), h.begin());
+  return PQ_OK;
+}
+
+// Device copy of host values, rounded to FP32 for es = 4.
+int h2d(double *dev, const std::vector<double> &h, int es) {
+  if (es == 8) {
+    PQ_HIP(hipMemcpy(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return PQ_OK;
+  }
+  std::vector<float> f(h.begin(), h.end());
+  PQ_HIP(hipMemcpy(dev, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
   return PQ_OK;
 }
 
@@ -175,8 +192,8 @@ int pq_model_write_snapshot(pq_model *m, const char *dir, int index, double mode
     o.rec(total_time_day);
     const size_t n = (size_t)G2 * G2 * L;
     for (int which = 0; which < 2; ++which) {
-      rc = d2h(a1, which == 0 ? v.psi : v.zeta, n);
-      if (!rc && lf) rc = d2h(a2, which == 0 ? v.psi2 : v.zeta2, n);
+      rc = d2h(a1, which == 0 ? v.psi : v.zeta, n, v.es);
+      if (!rc && lf) rc = d2h(a2, which == 0 ? v.psi2 : v.zeta2, n, v.es);
       if (rc) {
         std::fclose(o.f);
         return rc;
@@ -219,7 +236,7 @@ int pq_model_write_snapshot(pq_model *m, const char *dir, int index, double mode
   for (int k = 0; k < 6 && o.ok; ++k) {
     const bool fi = k == 0 || k == 2, fj = k == 1 || k == 3;
     const int d1 = fi ? g + 3 : fj ? g + 2 : g, d2 = fi ? g + 2 : fj ? g + 3 : g;
-    rc = d2h(a, dev[k], (size_t)d1 * d2 * L);
+    rc = d2h(a, dev[k], (size_t)d1 * d2 * L, v.es);
     if (rc) {
       std::fclose(o.f);
       return rc;
@@ -326,14 +343,13 @@ int pq_model_read_checkpoint(pq_model *m, const char *dir, double *model_time_da
   }
   // device state: the fields read, zero elsewhere
   double *dst[4] = {v.psi, v.zeta, v.psi2, v.zeta2};
-  for (int k = 0; k < (lf ? 4 : 2); ++k)
-    PQ_HIP(hipMemcpy(dst[k], f[k].data(), n * sizeof(double), hipMemcpyHostToDevice));
+  for (int k = 0; k < (lf ? 4 : 2) && !rc; ++k) rc = h2d(dst[k], f[k], v.es);
   if (!lf) {
     double *sd[6] = {v.ui, v.uj, v.zfi, v.zfj, v.res, v.res_beta};
-    for (int k = 0; k < 6; ++k)
-      PQ_HIP(hipMemcpy(sd[k], sup[k].data(), sup[k].size() * sizeof(double), hipMemcpyHostToDevice));
-    PQ_HIP(hipMemset(v.rel, 0, n * sizeof(double)));
+    for (int k = 0; k < 6 && !rc; ++k) rc = h2d(sd[k], sup[k], v.es);
+    if (!rc) PQ_HIP(hipMemset(v.rel, 0, n * v.es));
   }
+  if (rc) return rc;
   rc = model_pickup(m, dt);
   if (rc) return rc;
   if (model_time_day) *model_time_day = mtd;
