@@ -44,17 +44,39 @@ struct Dims {
   size_t n_rs() const { return (size_t)g * g * L; }
 };
 
-constexpr int BX = 64, BY = 4;
+#ifndef PQ_CAB_BX
+#define PQ_CAB_BX 64
+#define PQ_CAB_BY 4
+#endif
+constexpr int BX = PQ_CAB_BX, BY = PQ_CAB_BY;
 
 inline dim3 grid_for(int ni, int nj, int L) {
   return dim3((ni + BX - 1) / BX, (nj + BY - 1) / BY, L);
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (dispatch id mod 8), and each
+// XCD has its own L2.  The dispatch id is remapped so that every XCD takes
+// one contiguous run of tiles, ordered layer fastest, then i, then j: the
+// stencil rows a tile shares with the next tile in j, and the neighbouring
+// layers' values, are then read through the same L2.
+__device__ __forceinline__ void pq_tile(int &bx, int &by, int &bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const unsigned nb = gx * gy * gz;
+  const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned q = nb / 8u, r = nb % 8u, x = id % 8u, k = id / 8u;
+  const unsigned lid = (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + k;
+  bz = (int)(lid % gz);
+  const unsigned rest = lid / gz;
+  bx = (int)(rest % gx);
+  by = (int)(rest / gx);
+}
+
 // Thread -> (i, j, l) over i in [i0, i0+ni), j in [j0, j0+nj).
 #define PQ_IDX(i0, ni, j0, nj)                                                  \
-  const int i = (i0) + (int)(blockIdx.x * BX + threadIdx.x);                    \
-  const int j = (j0) + (int)(blockIdx.y * BY + threadIdx.y);                    \
-  const int l = (int)blockIdx.z;                                                \
+  int bx_, by_, l;                                                              \
+  pq_tile(bx_, by_, l);                                                         \
+  const int i = (i0) + bx_ * BX + (int)threadIdx.x;                             \
+  const int j = (j0) + by_ * BY + (int)threadIdx.y;                             \
   if (i >= (i0) + (ni) || j >= (j0) + (nj)) return;
 
 // Field accessors: the model's fields are stored as TS (double, or float
@@ -378,6 +400,83 @@ __global__ void k_corrector_to(Dims d, double dt, R<TS> flux_i,
   }
 }
 
+// cabaret_extrapolator, the copy of the new face values and cabaret_flux
+// of one model step in one pass: thread (i, j) over 2..g+2 extrapolates
+// i-face (i, j) (i in 3..g+1, j in 3..g) and j-face (i, j) into fin / fjn,
+// and writes the fluxes of the cabaret_flux ranges with the new face values
+// (the boundary faces i or j = 2, g+2 keep theirs).  zeta_fi / zeta_fj are
+// only read here (k_post_faces copies fin / fjn back).
+template <typename TS>
+__global__ void k_post_extrap_flux(Dims d, double dt, R<TS> zeta, R<TS> zeta_old, R<TS> ui,
+                                   R<TS> uj, R<TS> zfi, R<TS> zfj, W<TS> fin, W<TS> fjn,
+                                   W<TS> flux_i, W<TS> flux_j) {
+  PQ_IDX(2, d.g + 1, 2, d.g + 1);
+  const int g = d.g;
+  if (j >= 3 && j <= g) {  // i-face (i, j), i in 2..g+2
+    const size_t a = d.fi(i, j, l);
+    double f;
+    if (i >= 3 && i <= g + 1) {
+      const bool lo = !(i > 3), hi = !(i < g + 1);
+      const double u1 = ui[a], fi1 = zfi[a];
+      const double um = lo ? 0.0 : ui[d.fi(i - 1, j, l)], fm = lo ? 0.0 : zfi[d.fi(i - 1, j, l)];
+      const double up = hi ? 0.0 : ui[d.fi(i + 1, j, l)], fp = hi ? 0.0 : zfi[d.fi(i + 1, j, l)];
+      const TS v = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, zeta[d.nd(i - 1, j, l)],
+                                  zeta_old[d.nd(i - 1, j, l)], zeta[d.nd(i, j, l)],
+                                  zeta_old[d.nd(i, j, l)], lo, hi));
+      fin[a] = double(v);
+      f = double(v);
+    } else {
+      f = zfi[a];
+    }
+    flux_i[a] = ui[a] * f;
+  }
+  if (i >= 3 && i <= g) {  // j-face (i, j), j in 2..g+2
+    const size_t a = d.fj(i, j, l);
+    double f;
+    if (j >= 3 && j <= g + 1) {
+      const bool lo = !(j > 3), hi = !(j < g + 1);
+      const double u1 = uj[a], fi1 = zfj[a];
+      const double um = lo ? 0.0 : uj[d.fj(i, j - 1, l)], fm = lo ? 0.0 : zfj[d.fj(i, j - 1, l)];
+      const double up = hi ? 0.0 : uj[d.fj(i, j + 1, l)], fp = hi ? 0.0 : zfj[d.fj(i, j + 1, l)];
+      const TS v = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, zeta[d.nd(i, j - 1, l)],
+                                  zeta_old[d.nd(i, j - 1, l)], zeta[d.nd(i, j, l)],
+                                  zeta_old[d.nd(i, j, l)], lo, hi));
+      fjn[a] = double(v);
+      f = double(v);
+    } else {
+      f = zfj[a];
+    }
+    flux_j[a] = uj[a] * f;
+  }
+}
+
+// cabaret_corrector, zeta_old <- zeta and the face copy of one model step
+// in one pass: thread (i, j) over 1..g+2 corrects node (i, j) (interior
+// 3..g; res), writes the result to zeta and zeta_old, and copies the new
+// face values of i- and j-face (i, j) into zeta_fi / zeta_fj.
+template <typename TS>
+__global__ void k_post_correct(Dims d, double dt, R<TS> flux_i, R<TS> flux_j, W<TS> res,
+                               W<TS> zeta, W<TS> zeta_old, R<TS> fin, R<TS> fjn, W<TS> zfi,
+                               W<TS> zfj) {
+  PQ_IDX(1, d.g + 2, 1, d.g + 2);
+  const int g = d.g;
+  const size_t a = d.nd(i, j, l);
+  double v;
+  if (i >= 3 && i <= g && j >= 3 && j <= g) {
+    const double r = -(flux_i[d.fi(i + 1, j, l)] - flux_i[d.fi(i, j, l)] +
+                       flux_j[d.fj(i, j + 1, l)] - flux_j[d.fj(i, j, l)]);
+    res[d.rs(i - 1, j - 1, l)] = r;
+    v = zeta[a] + 0.5 * dt * r;
+  } else {
+    v = zeta[a];
+  }
+  const TS t = TS(v);
+  zeta_old[a] = double(t);
+  zeta[a] = double(t);
+  if (i >= 3 && i <= g + 1 && j >= 3 && j <= g) zfi[d.fi(i, j, l)] = fin[d.fi(i, j, l)];
+  if (i >= 3 && i <= g && j >= 3 && j <= g + 1) zfj[d.fj(i, j, l)] = fjn[d.fj(i, j, l)];
+}
+
 // rel_from_zeta_and_psi + apply_rel_bc (qg3p_aux.F90:738-915) over owned
 // nodes 2..g+1.  Boundary nodes take the boundary formula; on corners the
 // left/right (j) edge formula wins, as it is applied last in the reference.
@@ -410,11 +509,11 @@ __device__ __forceinline__ double rel_bc(const Dims &d, int bc_type, double alph
   return r;
 }
 
+// rel of owned node (i, j) of layer l.
 template <typename TS>
-__global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
-                      int bc_type, double alpha, R<TS> zeta,
-                      R<TS> psi, W<TS> rel) {
-  PQ_IDX(2, d.g, 2, d.g);
+__device__ __forceinline__ double rel_node(const Dims &d, const double *__restrict__ s1,
+                                           const double *__restrict__ s2, int bc_type, double alpha,
+                                           R<TS> zeta, R<TS> psi, int i, int j, int l) {
   const int g = d.g, L = d.L;
   const size_t a = d.nd(i, j, l);
   double r;
@@ -430,7 +529,15 @@ __global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__res
   } else {
     r = zeta[a] + s1[L - 1] * (psi[a] - psi[d.nd(i, j, L - 2)]);
   }
-  rel[a] = r;
+  return r;
+}
+
+template <typename TS>
+__global__ void k_rel(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
+                      int bc_type, double alpha, R<TS> zeta,
+                      R<TS> psi, W<TS> rel) {
+  PQ_IDX(2, d.g, 2, d.g);
+  rel[d.nd(i, j, l)] = rel_node(d, s1, s2, bc_type, alpha, zeta, psi, i, j, l);
 }
 
 // rel_from_psi (qg3p_aux.F90:579-603): the 5-point Laplacian on the
@@ -448,21 +555,72 @@ __global__ void k_rel_from_psi(Dims d, int bc_type, double alpha, R<TS> psi,
              psi[d.nd(i + 1, j, l)] + psi[d.nd(i, j - 1, l)];
 }
 
-// update_zeta_bc (qg3p_aux.F90:643-734): boundary nodes only.
+// update_zeta_bc (qg3p_aux.F90:643-734) at boundary node (i, j) of layer l
+// from its rel value.
+template <typename TS>
+__device__ __forceinline__ double zeta_bc_node(const Dims &d, const double *__restrict__ s1,
+                                               const double *__restrict__ s2, double rel, R<TS> psi,
+                                               int i, int j, int l) {
+  const int L = d.L;
+  const size_t a = d.nd(i, j, l);
+  double z;
+  if (L == 1) z = rel;
+  else if (l == 0) z = rel - s2[0] * (psi[a] - psi[d.nd(i, j, 1)]);
+  else if (l == L - 1) z = rel - s1[L - 1] * (psi[a] - psi[d.nd(i, j, L - 2)]);
+  else z = rel - s1[l] * (psi[a] - psi[d.nd(i, j, l - 1)]) - s2[l] * (psi[a] - psi[d.nd(i, j, l + 1)]);
+  return z;
+}
+
+// update_zeta_bc: boundary nodes only.
 template <typename TS>
 __global__ void k_zeta_bc(Dims d, const double *__restrict__ s1, const double *__restrict__ s2,
                           R<TS> rel, R<TS> psi,
                           W<TS> zeta) {
   PQ_IDX(2, d.g, 2, d.g);
-  const int g = d.g, L = d.L;
+  const int g = d.g;
   if (!(i == 2 || i == g + 1 || j == 2 || j == g + 1)) return;
   const size_t a = d.nd(i, j, l);
-  double z;
-  if (L == 1) z = rel[a];
-  else if (l == 0) z = rel[a] - s2[0] * (psi[a] - psi[d.nd(i, j, 1)]);
-  else if (l == L - 1) z = rel[a] - s1[L - 1] * (psi[a] - psi[d.nd(i, j, L - 2)]);
-  else z = rel[a] - s1[l] * (psi[a] - psi[d.nd(i, j, l - 1)]) - s2[l] * (psi[a] - psi[d.nd(i, j, l + 1)]);
-  zeta[a] = z;
+  zeta[a] = zeta_bc_node(d, s1, s2, rel[a], psi, i, j, l);
+}
+
+// rel_from_zeta_and_psi, update_zeta_bc and
+// cabaret_cell_boundaries_time_integrated of one model step in one pass:
+// thread (i, j) over 1..g+2 writes rel and (on the boundary) zeta of node
+// (i, j) and the i- and j-face velocities of (i, j).  rel of a one-layer
+// model is zeta on the halo ring as well (set(rel, zeta) copies the whole
+// data array).  Same arithmetic as k_rel, k_zeta_bc and k_cell_boundaries;
+// zeta is read only at the thread's own node.
+template <typename TS>
+__global__ void k_post_rel_cells(Dims d, const double *__restrict__ s1,
+                                 const double *__restrict__ s2, int bc_type, double alpha,
+                                 double tw, R<TS> psi, W<TS> zeta, W<TS> rel, W<TS> ui, W<TS> uj,
+                                 W<TS> uimo, W<TS> ujmo) {
+  PQ_IDX(1, d.g + 2, 1, d.g + 2);
+  const int g = d.g;
+  const size_t a = d.nd(i, j, l);
+  if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1) {
+    const TS r = TS(rel_node(d, s1, s2, bc_type, alpha, R<TS>(zeta.p), psi, i, j, l));
+    rel[a] = double(r);
+    if (i == 2 || i == g + 1 || j == 2 || j == g + 1)
+      zeta[a] = zeta_bc_node(d, s1, s2, double(r), psi, i, j, l);
+  } else if (d.L == 1) {
+    rel[a] = zeta[a];
+  }
+  if (i >= 3 && i <= g + 1 && j >= 3 && j <= g + 1) {
+    const double pe = psi_edge(d, psi, i, j, l);
+    if (j <= g) {
+      const double uim = -psi_edge(d, psi, i, j + 1, l) + pe;
+      const size_t f = d.fi(i, j, l);
+      ui[f] = (1.0 + tw) * uim - tw * uimo[f];
+      uimo[f] = uim;
+    }
+    if (i <= g) {
+      const double ujm = +psi_edge(d, psi, i + 1, j, l) - pe;
+      const size_t f = d.fj(i, j, l);
+      uj[f] = (1.0 + tw) * ujm - tw * ujmo[f];
+      ujmo[f] = ujm;
+    }
+  }
 }
 
 // max |u| over the cabaret_cfl face ranges (qg3p_aux.F90:937-940); the
@@ -1016,35 +1174,29 @@ template <typename TS> static int model_post_t(pq_model *m, hipStream_t s, bool 
   const int g = m->g, L = m->L;
   const Dims &d = m->d;
   const double dt = m->dt;
-  if (L == 1)
-    PQ_HIP(hipMemcpyAsync(m->rel, m->zeta, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice, s));
-  launch_rel<TS>(m, m->zeta, m->psi, m->rel, s);
+  TS *zeta = fld<TS>(m->zeta), *zeta_old = fld<TS>(m->zeta_old), *rel = fld<TS>(m->rel);
+  TS *ui = fld<TS>(m->ui), *uj = fld<TS>(m->uj), *zfi = fld<TS>(m->zfi), *zfj = fld<TS>(m->zfj);
+  TS *fin = fld<TS>(m->fin), *fjn = fld<TS>(m->fjn);
+  TS *flux_i = fld<TS>(m->flux_i), *flux_j = fld<TS>(m->flux_j);
+  hipLaunchKernelGGL(k_post_rel_cells<TS>, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, m->s1,
+                     m->s2, m->bc_type, m->alpha, m->tw_u, fld<TS>(m->psi), zeta, rel, ui, uj,
+                     fld<TS>(m->uimo), fld<TS>(m->ujmo));
   m->model_time += 0.5 * dt;
   {
     int rc = model_accumulate(m, m->psi, s);
     if (rc) return rc;
   }
-  TS *zeta = fld<TS>(m->zeta), *zeta_old = fld<TS>(m->zeta_old);
-  TS *ui = fld<TS>(m->ui), *uj = fld<TS>(m->uj), *zfi = fld<TS>(m->zfi), *zfj = fld<TS>(m->zfj);
-  TS *fin = fld<TS>(m->fin), *fjn = fld<TS>(m->fjn);
-  TS *flux_i = fld<TS>(m->flux_i), *flux_j = fld<TS>(m->flux_j);
   hipLaunchKernelGGL(k_forcing<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, d, dt, m->fw,
-                     m->visc, m->visc_bot, fld<TS>(m->rel), zeta);
-  hipLaunchKernelGGL(k_cell_boundaries<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d,
-                     m->tw_u, fld<TS>(m->psi), ui, uj, fld<TS>(m->uimo), fld<TS>(m->ujmo));
-  hipLaunchKernelGGL(k_extrapolator<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, dt, zeta,
-                     zeta_old, ui, uj, zfi, zfj, fin, fjn);
-  hipLaunchKernelGGL(k_face_copy<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, fin, fjn,
-                     zfi, zfj);
-  hipLaunchKernelGGL(k_flux<TS>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, ui, uj, zfi, zfj,
-                     flux_i, flux_j);
+                     m->visc, m->visc_bot, rel, zeta);
+  hipLaunchKernelGGL(k_post_extrap_flux<TS>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, dt,
+                     zeta, zeta_old, ui, uj, zfi, zfj, fin, fjn, flux_i, flux_j);
   if (!last) {
-    // corrector into zeta_old, so that zeta_old <- zeta needs no copy; the
-    // next predictor rewrites the interior of zeta and keeps its boundary.
-    hipLaunchKernelGGL(k_corrector_to<TS>, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
-                       flux_i, flux_j, fld<TS>(m->res), zeta, zeta_old);
-    PQ_HIP(hipMemcpyAsync(zeta, zeta_old, sizeof(TS) * d.n_nd(), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_post_correct<TS>, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
+                       flux_i, flux_j, fld<TS>(m->res), zeta, zeta_old, fin, fjn, zfi, zfj);
     m->model_time += 0.5 * dt;
+  } else {
+    hipLaunchKernelGGL(k_face_copy<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, fin, fjn,
+                       zfi, zfj);
   }
   return check_launch("pq_model_step");
 }

@@ -194,3 +194,70 @@ def test_model_steps_match_oracle(dev, adapt):
         a, b = gm.get(name), om.a[name]
         den = np.max(np.abs(b))
         assert np.max(np.abs(a - b)) <= 1e-9 * (den if den > 0 else 1.0), name
+
+
+@pytest.mark.parametrize("L,bc", [(3, 1), (1, 2), (4, 3)])
+def test_model_step_is_the_stage_sequence(dev, L, bc):
+    """The model's step (fused kernels: predictor + beta, rel + zeta_bc +
+    cell boundaries, extrapolator + flux, corrector + zeta_old + face copy)
+    against the stage entry points called in the reference's order
+    (qg3.F90:782-960) on copies of the same state, with the inversion by an
+    FP64 handle of the same stratification: bit for bit after 3 full steps
+    and after a final partial step."""
+    import torch
+    print(f"model step vs stages L={L} bc={bc}", flush=True)
+    g = 33
+    c = synth.config("qg3_255")
+    c.n = g - 2
+    nd = c.nondim()
+    rng = np.random.default_rng(L + 10 * bc)
+    if L == 3:
+        st = nd["strat"]
+    else:
+        s1, s2 = random_strat(L, rng)
+        st = S.from_parameters(np.ones(L), s1[1:], s2[:-1])
+    beta = np.full(L, float(np.asarray(nd["beta"]).ravel()[0]))
+    fw = c.wind()
+    kw = dict(viscosity=nd["viscosity"], visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=bc,
+              alpha=nd["alpha"])
+    gm = qg.Model(g, st, beta, fw, **kw)
+    q = c.q()[:, :, :L] if L <= 3 else np.concatenate([c.q(), c.q(member=1)[:, :, :L - 3]], axis=2)
+    gm.init(np.asfortranarray(q))
+    names = ("psi", "zeta", "rel", "zeta_old", "ui", "uj", "zeta_fi", "zeta_fj", "res", "res_beta",
+             "uimo", "ujmo", "flux_i", "flux_j")
+    T = {n: to_dev(gm.get(n), dev) for n in names}
+    hc = qg.HelmholtzCache(g, st, "fp64", tiles=None)
+    t1 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    ts1, ts2, tb = t1(st.s1), t1(st.s2), t1(beta)
+    tfw = torch.from_numpy(np.ascontiguousarray(fw.T)).to(dev)
+    sc = gm.scalars()
+    dt, tw_a, tw_u = sc["dt"], sc["tweight_a"], sc["tweight_u"]
+
+    def stage_step(last):
+        qg.cabaret_predictor(dt, T["zeta_old"], T["res"], T["zeta"])
+        qg.cabaret_beta(dt, tw_a, tb, T["ui"], T["zeta"], T["res_beta"])
+        qg.pv_inversion(T["zeta"], hc, T["psi"])
+        qg.rel_from_zeta_and_psi(ts1, ts2, bc, kw["alpha"], T["zeta"], T["psi"], T["rel"])
+        qg.update_zeta_bc(ts1, ts2, T["rel"], T["psi"], T["zeta"])
+        qg.cabaret_forcing(dt, tfw, kw["viscosity"], kw["visc_bot"], T["rel"], T["zeta"])
+        qg.cabaret_cell_boundaries(tw_u, T["psi"], T["ui"], T["uj"], T["uimo"], T["ujmo"])
+        qg.cabaret_extrapolator(dt, T["zeta"], T["zeta_old"], T["ui"], T["uj"], T["zeta_fi"],
+                                T["zeta_fj"])
+        qg.cabaret_flux(T["ui"], T["uj"], T["zeta_fi"], T["zeta_fj"], T["flux_i"], T["flux_j"])
+        if not last:
+            qg.cabaret_corrector(dt, T["flux_i"], T["flux_j"], T["res"], T["zeta"])
+            T["zeta_old"].copy_(T["zeta"])
+
+    for k in range(3):
+        stage_step(False)
+    gm.step(3)
+    torch.cuda.synchronize()
+    for n in names:
+        np.testing.assert_array_equal(gm.get(n), from_dev(T[n]), err_msg=f"full steps: {n}")
+    stage_step(True)
+    gm.step(1, final_partial=True)
+    torch.cuda.synchronize()
+    for n in names:
+        np.testing.assert_array_equal(gm.get(n), from_dev(T[n]), err_msg=f"final partial step: {n}")
+    hc.deallocate()
+    gm.close()
