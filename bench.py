@@ -576,6 +576,15 @@ def main():
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
         "cabaret_fp32_steps_per_s": None if cab32 is None else round(cab32, 2),
         "leapfrog_fp64_steps_per_s": None if lfr is None else round(lfr, 2),
+        # one full CABARET step (inversion included) moves at least 21 L G^2
+        # values: 10 field reads and 9 writes of the step's state plus the
+        # inversion's read and write
+        "cabaret_step_roofline": None if cab is None else {
+            "bytes_per_step": {"fp64": 21 * L * g * g * 8, "fp32": 21 * L * g * g * 4},
+            "achieved_GBs": {"fp64": round(21 * L * g * g * 8 * cab / 1e9, 1),
+                             "fp32": round(21 * L * g * g * 4 * cab32 / 1e9, 1)},
+            "frac": {"fp64": round(21 * L * g * g * 8 * cab / 1e9 / HBM_PEAK_GBS, 4),
+                     "fp32": round(21 * L * g * g * 4 * cab32 / 1e9 / HBM_PEAK_GBS, 4)}},
         "ensemble_64_mixed": ens,
         "ensemble_64_mixed_steps": ens_step,
         "channel": chan,
