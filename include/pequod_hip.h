@@ -271,7 +271,11 @@ typedef struct {
   int adapt_dt;
   double target_cfl, max_dt, max_dt_increase;
   double dt;               /* initial (non-dimensional) time step          */
-  int precision;           /* PQ_PREC_* of the inversion                   */
+  int precision;           /* PQ_PREC_* of the inversion; it also sets the
+                            * field storage: FP64 for PQ_PREC_FP64 and
+                            * PQ_PREC_MIXED, FP32 for the other modes (the
+                            * stages compute in FP64 either way; the time
+                            * mean and energy need FP64 fields)          */
   int scheme;              /* PQ_SCHEME_* (adapt_dt needs CABARET)         */
 } pq_model_params;
 
@@ -386,7 +390,9 @@ int pq_ens_add(pq_ens *e, int g, int layers, const double *ss,
                const void *zeta, long ldz1, long ldz2, void *psi, long ldp1,
                long ldp2, int *index);
 /* A CABARET model member (fixed dt); the ensemble inverts its zeta into its
- * psi and steps it with pq_ens_step.                                      */
+ * psi and steps it with pq_ens_step.  The model's field storage must be the
+ * ensemble's: FP64 fields for PQ_PREC_FP64 / PQ_PREC_MIXED ensembles, FP32
+ * fields for the FP32-storage modes (else PQ_ERR_ARG).                    */
 int pq_ens_add_model(pq_ens *e, pq_model *m, const double *ss,
                      const double *modes_to_z, const double *z_to_modes,
                      int *index);
