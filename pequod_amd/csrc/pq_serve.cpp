@@ -66,12 +66,17 @@ struct Member {
   double *coef = nullptr;
   // model members
   pq_model *model = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr, graph_k = nullptr;
+  hipGraphExec_t exec = nullptr, exec_k = nullptr;  // 1 step, kStepsPerGraph steps
   int step_kind = 0;  // model_step_kind at the capture
   hipEvent_t ev = nullptr;
   long long running = -1;
 };
+
+// An advance of n steps replays a graph of kStepsPerGraph steps n / k times
+// and the one-step graph for the rest: one launch per step would make the
+// scheduler thread's launch rate the engine's limit.
+constexpr int kStepsPerGraph = 10;
 
 struct Class {
   int g = 0;
@@ -199,9 +204,14 @@ bool pq_srv::pass() {
     hipStream_t s = model_stream(m.model);
     // a captured step goes stale when pq_model_average changed what a step does
     int rc = model_step_kind(m.model) == m.step_kind ? PQ_OK : PQ_ERR_ARG;
-    for (int it = 0; it < r.steps && rc == PQ_OK; ++it) {
-      if (hipGraphLaunch(m.exec, s) != hipSuccess) rc = PQ_ERR_HIP;
-      else model_account_step(m.model);
+    for (int left = r.steps; left > 0 && rc == PQ_OK;) {
+      const int k = left >= kStepsPerGraph ? kStepsPerGraph : 1;
+      if (hipGraphLaunch(k > 1 ? m.exec_k : m.exec, s) != hipSuccess) {
+        rc = PQ_ERR_HIP;
+      } else {
+        for (int q = 0; q < k; ++q) model_account_step(m.model);
+        left -= k;
+      }
     }
     if (rc == PQ_OK && hipEventRecord(m.ev, s) != hipSuccess) rc = PQ_ERR_HIP;
     if (rc != PQ_OK) {
@@ -355,6 +365,8 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sd
   auto fail = [&](int code) {
     if (m->exec) (void)hipGraphExecDestroy(m->exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->exec_k) (void)hipGraphExecDestroy(m->exec_k);
+    if (m->graph_k) (void)hipGraphDestroy(m->graph_k);
     if (m->ev) (void)hipEventDestroy(m->ev);
     pq_model_destroy(m->model);
     return code;
@@ -376,28 +388,34 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sd
   };
   rc = pq_model_init(m->model, zeta_sdata_host);
   if (rc) return unreserve(rc);
-  // one full time step, captured once and replayed per step
+  // one full time step, and kStepsPerGraph of them, captured once and
+  // replayed (the inversion's kernels were first launched by
+  // pq_model_init, so no attribute call happens in a capture)
   hipStream_t st = model_stream(m->model);
   const ModelClock t0 = model_clock(m->model);
   m->step_kind = model_step_kind(m->model);
-  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return unreserve(PQ_ERR_HIP);
-  // the three parts of pq_model_step (the inversion's kernels were first
-  // launched by pq_model_init, so no attribute call happens in the capture)
-  rc = model_step_pre(m->model, st);
-  if (!rc) rc = model_step_invert(m->model);
-  if (!rc) rc = model_step_post(m->model, st, false);
-  hipGraph_t gr = nullptr;
-  const hipError_t ee = hipStreamEndCapture(st, &gr);
-  model_set_clock(m->model, t0);
-  if (rc || ee != hipSuccess) {
-    if (gr) (void)hipGraphDestroy(gr);
-    if (!rc) pq_set_error("pq_srv_open_model: capture failed: %s", hipGetErrorString(ee));
-    return unreserve(rc ? rc : PQ_ERR_HIP);
+  for (int k : {1, kStepsPerGraph}) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return unreserve(PQ_ERR_HIP);
+    // the three parts of pq_model_step, k times
+    for (int q = 0; q < k && !rc; ++q) {
+      rc = model_step_pre(m->model, st);
+      if (!rc) rc = model_step_invert(m->model);
+      if (!rc) rc = model_step_post(m->model, st, false);
+    }
+    hipGraph_t gr = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &gr);
+    model_set_clock(m->model, t0);
+    if (rc || ee != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      if (!rc) pq_set_error("pq_srv_open_model: capture failed: %s", hipGetErrorString(ee));
+      return unreserve(rc ? rc : PQ_ERR_HIP);
+    }
+    (k == 1 ? m->graph : m->graph_k) = gr;
+    if (hipGraphInstantiate(k == 1 ? &m->exec : &m->exec_k, gr, nullptr, nullptr, 0) != hipSuccess)
+      return unreserve(PQ_ERR_HIP);
   }
-  m->graph = gr;
-  if (hipGraphInstantiate(&m->exec, gr, nullptr, nullptr, 0) != hipSuccess ||
-      hipEventCreateWithFlags(&m->ev, hipEventDisableTiming) != hipSuccess)
-    return unreserve(PQ_ERR_HIP);
+  if (hipEventCreateWithFlags(&m->ev, hipEventDisableTiming) != hipSuccess) return unreserve(PQ_ERR_HIP);
   std::lock_guard<std::mutex> lk(s->mu);
   s->mem.push_back(std::move(m));
   *member = (int)s->mem.size() - 1;
@@ -540,9 +558,13 @@ int pq_srv_close(pq_srv *s, int member) {
   m.coef = nullptr;
   if (m.exec) (void)hipGraphExecDestroy(m.exec);
   if (m.graph) (void)hipGraphDestroy(m.graph);
+  if (m.exec_k) (void)hipGraphExecDestroy(m.exec_k);
+  if (m.graph_k) (void)hipGraphDestroy(m.graph_k);
   if (m.ev) (void)hipEventDestroy(m.ev);
   m.exec = nullptr;
   m.graph = nullptr;
+  m.exec_k = nullptr;
+  m.graph_k = nullptr;
   m.ev = nullptr;
   if (m.model) pq_model_destroy(m.model);
   m.model = nullptr;
@@ -562,6 +584,8 @@ int pq_srv_destroy(pq_srv *s) {
     if (m.coef) (void)hipFree(m.coef);
     if (m.exec) (void)hipGraphExecDestroy(m.exec);
     if (m.graph) (void)hipGraphDestroy(m.graph);
+    if (m.exec_k) (void)hipGraphExecDestroy(m.exec_k);
+    if (m.graph_k) (void)hipGraphDestroy(m.graph_k);
     if (m.ev) (void)hipEventDestroy(m.ev);
     if (m.model) pq_model_destroy(m.model);
   }

@@ -163,21 +163,27 @@ def test_fp32_model_files_and_restart(dev, tmp_path):
 
 def test_fp32_model_served_matches_standalone(dev):
     """An FP32 model member of the server advances exactly as the same
-    model stepped alone."""
+    model stepped alone (12 + 3 steps: the 10-step graph and single steps)."""
     from pequod_amd.serve import Server
     print("fp32 served model", flush=True)
     g = 65
     c, st, nd, kw, q = _setup(g, 3)
     srv = Server("fp32", cap_bytes=16e9, max_batch=4)
     mid = srv.open_model(g, st, nd["beta"], c.wind(), q, precision="fp32", **kw)
-    for n in (3, 2):
+    for n in (12, 3):
         srv.submit(mid, Server.MODEL, n)
     srv.drain(120)
     assert srv.stats()["failed"] == 0
     ref = qg.Model(g, st, nd["beta"], c.wind(), precision="fp32", **kw)
     ref.init(q)
-    ref.step(5)
+    ref.step(15)
     for name in ("zeta", "psi", "zeta_fi", "ui", "res"):
         np.testing.assert_array_equal(srv.model_get(mid, name, g, st.layers), ref.get(name), err_msg=name)
+    import ctypes
+    from pequod_amd._lib import check, load
+    sc = np.zeros(4)
+    check(load().pq_model_scalars(load().pq_srv_model(srv._h, mid),
+                                  sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    assert sc[3] == ref.scalars()["model_time"]  # host clock advanced per replayed step
     ref.close()
     srv.close()
