@@ -182,6 +182,19 @@ __global__ void __launch_bounds__(512) k_rows_fwd(InvArgs a, PtrTab ptrs) {
 // the q = 0 point accumulates the boundary sums of the modal solution,
 // acc_m += phi_m w_p (wall i = 1) and acc_{8+m} += phi_m (-1)^(p+1) w_p
 // (wall i = G).
+// 1 / x by the hardware's reciprocal estimate and Newton-Raphson steps
+// r <- r + r (1 - x r) (one for FP32, two for FP64; each squares the
+// relative error), in place of an IEEE division per point and mode.
+__device__ __forceinline__ float recip(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
 template <typename Eng, int LT, bool PER = false>
 __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T *lds,
                                               const typename Eng::Tc *cD, const typename Eng::Tc *cM,
@@ -222,7 +235,7 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
       if constexpr (LT == 0) {
         for (int ll = 0; ll < Lm; ++ll) {
           T *x = row_ptr<Eng>(lds, (ll << lgC) + c) + pad16(e);
-          *x = T(Tc(*x) / (lpq - cS[ll]));
+          *x = T(Tc(*x) * recip(lpq - cS[ll]));
         }
       } else {
         if constexpr (!REG) {
@@ -230,7 +243,10 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
           // registers they would spill)
           asm volatile("" ::: "memory");
         }
-        const double wpq = PER ? (e == 0 ? sp : 0.0) : sp * a.shat[q];
+        // the psiC weight: zero unless p and q are odd (closed basin), so
+        // the even-p columns skip the FP64 accumulation
+        const bool wz = !PER && !(p & 1);
+        const double wpq = PER ? (e == 0 ? sp : 0.0) : (wz ? 0.0 : sp * a.shat[q]);
         Tc col[LT], phi[LT];
 #pragma unroll
         for (int ll = 0; ll < LT; ++ll) col[ll] = Tc(row_ptr<Eng>(lds, (ll << lgC) + c)[pad16(e)]);
@@ -240,7 +256,7 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
           if constexpr (REG) {
 #pragma unroll
             for (int ll = 0; ll < LT; ++ll) s += d[m * LT + ll] * col[ll];
-            phi[m] = s / (lpq - sv[m]);
+            phi[m] = s * recip(lpq - sv[m]);
           } else {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -249,9 +265,9 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
               for (int e = 0; e < VN; ++e)
                 if (v * VN + e < LT) s += dv[e] * col[v * VN + e];
             }
-            phi[m] = s / (lpq - cS[m]);
+            phi[m] = s * recip(lpq - cS[m]);
           }
-          acc[m] += double(phi[m]) * wpq;  // closed: wpq = 0 unless p, q odd
+          if (!wz) acc[m] += double(phi[m]) * wpq;
           if constexpr (PER) acc[8 + m] += double(phi[m]) * (wpq * sgn);
         }
 #pragma unroll
