@@ -24,6 +24,12 @@ module pequod_hip_c
   public :: pq_mirror_register, pq_mirror_unregister, pq_mirror_push, &
     & pq_mirror_pull
   public :: pq_model_stream
+  public :: pq_mirror_device_ptr, pq_malloc, pq_free, pq_memcpy_h2d, pq_memcpy_d2h
+  public :: pq_cabaret_cfl_dev, pq_cabaret_predictor_dev, pq_cabaret_beta_dev, &
+    & pq_cabaret_forcing_dev, pq_cabaret_cell_boundaries_dev, &
+    & pq_cabaret_cell_boundaries_velocity_dev, pq_cabaret_extrapolator_dev, &
+    & pq_cabaret_flux_dev, pq_cabaret_corrector_dev, pq_rel_from_zeta_and_psi_dev, &
+    & pq_update_zeta_bc_dev
 
   interface
     function pq_last_error() bind(C, name = "pq_last_error") result(p)
@@ -168,6 +174,139 @@ module pequod_hip_c
       type(c_ptr), value :: m
       type(c_ptr) :: s
     end function pq_model_stream
+
+    ! device memory and mirrors
+    function pq_mirror_device_ptr(host) bind(C, name = "pq_mirror_device_ptr") result(d)
+      import :: c_ptr
+      type(c_ptr), value :: host
+      type(c_ptr) :: d
+    end function pq_mirror_device_ptr
+
+    function pq_malloc(d, bytes) bind(C, name = "pq_malloc") result(rc)
+      import :: c_ptr, c_size_t, c_int
+      type(c_ptr), intent(out) :: d
+      integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function pq_malloc
+
+    function pq_free(d) bind(C, name = "pq_free") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: d
+      integer(c_int) :: rc
+    end function pq_free
+
+    function pq_memcpy_h2d(d, host, bytes) bind(C, name = "pq_memcpy_h2d") result(rc)
+      import :: c_ptr, c_size_t, c_int
+      type(c_ptr), value :: d, host
+      integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function pq_memcpy_h2d
+
+    function pq_memcpy_d2h(host, d, bytes) bind(C, name = "pq_memcpy_d2h") result(rc)
+      import :: c_ptr, c_size_t, c_int
+      type(c_ptr), value :: host, d
+      integer(c_size_t), value :: bytes
+      integer(c_int) :: rc
+    end function pq_memcpy_d2h
+
+    ! CABARET and rel / zeta boundary stages on device arrays of the
+    ! reference's shapes (include/pequod_hip.h)
+    function pq_cabaret_cfl_dev(g, layers, dt, ui, uj, cfl, stream) &
+      & bind(C, name = "pq_cabaret_cfl_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: dt
+      type(c_ptr), value :: ui, uj, stream
+      real(c_double), intent(out) :: cfl
+      integer(c_int) :: rc
+    end function pq_cabaret_cfl_dev
+
+    function pq_cabaret_predictor_dev(g, layers, dt, zeta_old, res, zeta, stream) &
+      & bind(C, name = "pq_cabaret_predictor_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: dt
+      type(c_ptr), value :: zeta_old, res, zeta, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_predictor_dev
+
+    function pq_cabaret_beta_dev(g, layers, dt, tweight_a, beta, ui, zeta, res_beta, stream) &
+      & bind(C, name = "pq_cabaret_beta_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: dt, tweight_a
+      type(c_ptr), value :: beta, ui, zeta, res_beta, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_beta_dev
+
+    function pq_cabaret_forcing_dev(g, layers, dt, force_wind, viscosity, visc_bot, rel, zeta, &
+      & stream) bind(C, name = "pq_cabaret_forcing_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: dt, viscosity, visc_bot
+      type(c_ptr), value :: force_wind, rel, zeta, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_forcing_dev
+
+    function pq_cabaret_cell_boundaries_dev(g, layers, tweight_u, psi, ui, uj, uimo, ujmo, &
+      & stream) bind(C, name = "pq_cabaret_cell_boundaries_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: tweight_u
+      type(c_ptr), value :: psi, ui, uj, uimo, ujmo, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_cell_boundaries_dev
+
+    function pq_cabaret_cell_boundaries_velocity_dev(g, layers, psi, ui, uj, stream) &
+      & bind(C, name = "pq_cabaret_cell_boundaries_velocity_dev") result(rc)
+      import :: c_int, c_ptr
+      integer(c_int), value :: g, layers
+      type(c_ptr), value :: psi, ui, uj, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_cell_boundaries_velocity_dev
+
+    function pq_cabaret_extrapolator_dev(g, layers, dt, zeta, zeta_old, ui, uj, zeta_fi, &
+      & zeta_fj, scratch, stream) bind(C, name = "pq_cabaret_extrapolator_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: dt
+      type(c_ptr), value :: zeta, zeta_old, ui, uj, zeta_fi, zeta_fj, scratch, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_extrapolator_dev
+
+    function pq_cabaret_flux_dev(g, layers, ui, uj, zeta_fi, zeta_fj, flux_i, flux_j, stream) &
+      & bind(C, name = "pq_cabaret_flux_dev") result(rc)
+      import :: c_int, c_ptr
+      integer(c_int), value :: g, layers
+      type(c_ptr), value :: ui, uj, zeta_fi, zeta_fj, flux_i, flux_j, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_flux_dev
+
+    function pq_cabaret_corrector_dev(g, layers, dt, flux_i, flux_j, res, zeta, stream) &
+      & bind(C, name = "pq_cabaret_corrector_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers
+      real(c_double), value :: dt
+      type(c_ptr), value :: flux_i, flux_j, res, zeta, stream
+      integer(c_int) :: rc
+    end function pq_cabaret_corrector_dev
+
+    function pq_rel_from_zeta_and_psi_dev(g, layers, s1, s2, bc_type, alpha, zeta, psi, rel, &
+      & stream) bind(C, name = "pq_rel_from_zeta_and_psi_dev") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g, layers, bc_type
+      real(c_double), value :: alpha
+      type(c_ptr), value :: s1, s2, zeta, psi, rel, stream
+      integer(c_int) :: rc
+    end function pq_rel_from_zeta_and_psi_dev
+
+    function pq_update_zeta_bc_dev(g, layers, s1, s2, rel, psi, zeta, stream) &
+      & bind(C, name = "pq_update_zeta_bc_dev") result(rc)
+      import :: c_int, c_ptr
+      integer(c_int), value :: g, layers
+      type(c_ptr), value :: s1, s2, rel, psi, zeta, stream
+      integer(c_int) :: rc
+    end function pq_update_zeta_bc_dev
   end interface
 
   interface
