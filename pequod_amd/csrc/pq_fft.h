@@ -51,9 +51,13 @@ struct StageArr {
   int lns[12];
 };
 
+// Radices of an FFT of length M with E elements per thread: powers of two
+// (16, 8, 4, 2 as E allows), and for M = 3 * 2^a a final radix 3, or 24
+// (= 8 x 3 in registers, one LDS round trip fewer) when E is a multiple of 24.
 constexpr StageArr make_stages(int M, int E) {
   StageArr a{};
-  int rem = (M % 3 == 0) ? M / 3 : M;
+  const bool r24 = M % 3 == 0 && E % 24 == 0 && (M / 3) % 8 == 0;
+  int rem = (M % 3 == 0) ? (r24 ? M / 24 : M / 3) : M;
   int lg = 0;
   a.nst = 0;
   while (rem > 1) {
@@ -68,7 +72,7 @@ constexpr StageArr make_stages(int M, int E) {
     rem /= r;
   }
   if (M % 3 == 0) {
-    a.rad[a.nst] = 3;
+    a.rad[a.nst] = r24 ? 24 : 3;
     a.lns[a.nst] = lg;
     ++a.nst;
   }
@@ -162,11 +166,46 @@ template <typename T> __device__ __forceinline__ void dft16(cx<T> *v) {
   }
 }
 
+// 24 = 3 x 8 (Cooley-Tukey): DFT-8s of the inputs n1 + 3 n2 (n1 < 3),
+// twiddles exp(-2 pi i n1 k2 / 24), then DFT-3s over n1 give X[k2 + 8 k1].
+template <typename T> __device__ __forceinline__ void dft24(cx<T> *v) {
+  // (cos, -sin) of 2 pi m / 24, m = 0..14
+  const T c15 = T(0.96592582628906828675), s15 = T(0.25881904510252076235);
+  const T c30 = T(0.86602540378443864676), h = T(0.70710678118654752440);
+  const cx<T> w[15] = {{T(1), T(0)},     {c15, -s15},       {c30, T(-0.5)},    {h, -h},
+                       {T(0.5), -c30},   {s15, -c15},       {T(0), T(-1)},     {-s15, -c15},
+                       {T(-0.5), -c30},  {-h, -h},          {-c30, T(-0.5)},   {-c15, -s15},
+                       {T(-1), T(0)},    {-c15, s15},       {-c30, T(0.5)}};
+  cx<T> y[3][8];
+#pragma unroll
+  for (int n1 = 0; n1 < 3; ++n1)
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) y[n1][n2] = v[n1 + 3 * n2];
+  dft8(y[0]);
+  dft8(y[1]);
+  dft8(y[2]);
+#pragma unroll
+  for (int k2 = 1; k2 < 8; ++k2) {
+    y[1][k2] = cmul(y[1][k2], w[k2]);
+    y[2][k2] = cmul(y[2][k2], w[2 * k2]);
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) {
+    cx<T> z[3] = {y[0][k2], y[1][k2], y[2][k2]};
+    dft3(z);
+    v[k2] = z[0];
+    v[k2 + 8] = z[1];
+    v[k2 + 16] = z[2];
+  }
+}
+
 template <typename T, int R> __device__ __forceinline__ void dftR(cx<T> *v) {
+  static_assert(R == 2 || R == 3 || R == 4 || R == 8 || R == 16 || R == 24, "radix");
   if constexpr (R == 2) dft2(v);
   else if constexpr (R == 3) dft3(v);
   else if constexpr (R == 4) dft4(v);
   else if constexpr (R == 8) dft8(v);
+  else if constexpr (R == 24) dft24(v);
   else dft16(v);
 }
 
