@@ -274,8 +274,7 @@ typedef struct {
   int precision;           /* PQ_PREC_* of the inversion; it also sets the
                             * field storage: FP64 for PQ_PREC_FP64 and
                             * PQ_PREC_MIXED, FP32 for the other modes (the
-                            * stages compute in FP64 either way; the time
-                            * mean and energy need FP64 fields)          */
+                            * stages compute in FP64 either way)          */
   int scheme;              /* PQ_SCHEME_* (adapt_dt needs CABARET)         */
 } pq_model_params;
 

@@ -1001,12 +1001,13 @@ template <typename TS> static int lf_diagnose(pq_model *m, const Level &v) {
   return check_launch("pq_model_step");
 }
 
-// The time-mean accumulation after the inversion (qg3.F90:672-681); only
-// FP64-storage models average (pq_model_average).
+// The time-mean accumulation after the inversion (qg3.F90:672-681), from
+// psi's sdata view (FP64 sum; FP32 fields read widened).
 static int model_accumulate(pq_model *m, const double *psi, hipStream_t s) {
   if (m->average && m->model_time > m->max_spin) {
     const int ld = m->g + 2;
-    int rc = pq_psi_av_add_dev(m->g, m->L, m->dt, psi + 1 + ld, ld, ld, m->psi_av, m->g, m->g, s);
+    const void *sd = reinterpret_cast<const char *>(psi) + (size_t)m->es * (1 + ld);
+    int rc = pq::psi_av_add_es(m->g, m->L, m->dt, sd, ld, ld, m->psi_av, m->g, m->g, s, m->es);
     if (rc) return rc;
     m->diag_time += m->dt;
   }
@@ -1357,7 +1358,6 @@ int pq_model_scalars(pq_model *m, double *out4) {
 
 int pq_model_average(pq_model *m, int enable, double max_spin) {
   PQ_CHECK_ARG(m != nullptr, "pq_model_average: null model");
-  PQ_CHECK_ARG(!enable || m->es == 8, "pq_model_average: the time mean needs an FP64-storage model");
   PQ_HIP(hipStreamSynchronize(m->stream));
   if (enable) {
     if (!m->psi_av && hipMalloc((void **)&m->psi_av, sizeof(double) * m->d.n_rs()) != hipSuccess) {
@@ -1388,7 +1388,6 @@ int pq_model_psi_av(pq_model *m, double *host, size_t count, double *diag_time) 
 
 int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot) {
   PQ_CHECK_ARG(m && h && ekin && epot, "pq_model_energy: null argument");
-  PQ_CHECK_ARG(m->es == 8, "pq_model_energy: the energy diagnostic needs an FP64-storage model");
   if (!m->diag || !std::equal(m->diag_h.begin(), m->diag_h.end(), h)) {
     pq_diag_destroy(m->diag);
     m->diag = nullptr;
@@ -1403,7 +1402,8 @@ int pq_model_energy(pq_model *m, const double *h, double *ekin, double *epot) {
   PQ_HIP(hipStreamSynchronize(m->stream));
   // the level written last (centred scheme: the one before lf)
   const double *psi = m->scheme == PQ_SCHEME_LEAPFROG ? level(m, 3 - m->lf).psi : m->psi;
-  return pq::diag_energy_sync(m->diag, psi + 1 + ld, ld, ld, ekin, epot);
+  const void *sd = reinterpret_cast<const char *>(psi) + (size_t)m->es * (1 + ld);
+  return pq::diag_energy_sync(m->diag, sd, ld, ld, ekin, epot, m->es);
 }
 
 }  // extern "C"

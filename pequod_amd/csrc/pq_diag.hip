@@ -37,7 +37,7 @@ constexpr int kRows = 64;   // rows per workgroup (one lane each)
 constexpr int kWaves = 4;   // waves per workgroup, side by side along j
 
 struct EnergyArgs {
-  const double *psi;
+  const void *psi;         // FP64, or FP32 (read widened) for FP32-field models
   long ld1, lstride;       // (i, j, l) at psi[(i-1) + ld1 (j-1) + lstride l]
   int g, L, nblk;
   double fac[kMaxL];       // 0.5 s2(k) h(k) / H for the interfaces k = 1..L-1
@@ -60,6 +60,7 @@ __device__ __forceinline__ double pick(const double (&a)[kMaxL], int k) {
   return r;
 }
 
+template <typename T>
 __global__ __launch_bounds__(kRows *kWaves) void k_energy_part(EnergyArgs a, double *part) {
   __shared__ double red[kWaves][kMaxL + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -74,14 +75,14 @@ __global__ __launch_bounds__(kRows *kWaves) void k_energy_part(EnergyArgs a, dou
   for (int q = 0; q < kCols; ++q) prev[q] = 0.0;
   double ep = 0.0;
   for (int l = 0; l < a.L; ++l) {
-    const double *p = a.psi + a.lstride * l + (i - 1);
+    const T *p = static_cast<const T *>(a.psi) + a.lstride * l + (i - 1);
     const double f = l > 0 ? pick(a.fac, l - 1) : 0.0;
     double ek = 0.0;
     // west corners of the first cell: column jb - 1
     double ap = 0.0, bp = 0.0;
     if (row && jb > 1 && jb <= g) {
-      ap = p[a.ld1 * (jb - 2)];
-      if (up) bp = p[a.ld1 * (jb - 2) - 1];
+      ap = double(p[a.ld1 * (jb - 2)]);
+      if (up) bp = double(p[a.ld1 * (jb - 2) - 1]);
     }
 #pragma unroll
     for (int q = 0; q < kCols; ++q) {
@@ -89,8 +90,8 @@ __global__ __launch_bounds__(kRows *kWaves) void k_energy_part(EnergyArgs a, dou
       const bool ok = row && j <= g;
       double x = 0.0, y = 0.0;  // psi(i, j), psi(i-1, j)
       if (ok) {
-        x = p[a.ld1 * (j - 1)];
-        if (up) y = p[a.ld1 * (j - 1) - 1];
+        x = double(p[a.ld1 * (j - 1)]);
+        if (up) y = double(p[a.ld1 * (j - 1) - 1]);
       }
       if (ok && up && j > 1) {
         const double d1 = y + x - bp - ap;  // 2 d(psi)/dj on the cell
@@ -138,8 +139,10 @@ __global__ __launch_bounds__(256) void k_energy_final(EnergyArgs a, const double
   }
 }
 
-// psi_av(i, j, l) += dt * psi(i, j, l) over all g x g nodes.
-__global__ __launch_bounds__(256) void k_psi_av_add(int g, double dt, const double *psi, long ld1,
+// psi_av(i, j, l) += dt * psi(i, j, l) over all g x g nodes (psi FP64, or
+// FP32 read widened).
+template <typename T>
+__global__ __launch_bounds__(256) void k_psi_av_add(int g, double dt, const T *psi, long ld1,
                                                     long ls, double *av, long la1, long las) {
   // a product and a sum, each rounded (no fused multiply-add), as written
 #pragma clang fp contract(off)
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(256) void k_psi_av_add(int g, double dt, const doub
   const int l = blockIdx.z;
   if (i >= g || j >= g) return;
   double &o = av[i + la1 * j + las * l];
-  o = o + dt * psi[i + ld1 * j + ls * l];
+  o = o + dt * double(psi[i + ld1 * j + ls * l]);
 }
 
 }  // namespace
@@ -167,9 +170,10 @@ struct pq_diag {
 // Energy of a device field into host ekin[L], *epot on the handle's stream
 // (after the work already queued there), synchronously; PQ_ERR_NAN when a
 // result is NaN (the reference aborts, qg3p_diagnostics.F90:857-860).
-int pq::diag_energy_sync(pq_diag *d, const void *psi, long ld1, long ld2, double *ekin, double *epot) {
+int pq::diag_energy_sync(pq_diag *d, const void *psi, long ld1, long ld2, double *ekin, double *epot,
+                         int es) {
   const int L = d->L;
-  int rc = pq_diag_energy_dev(d, psi, ld1, ld2, d->out, d->stream);
+  int rc = pq::diag_energy_dev_es(d, psi, ld1, ld2, d->out, d->stream, es);
   if (rc) return rc;
   PQ_HIP(hipMemcpyAsync(d->out_host, d->out, sizeof(double) * (L + 1), hipMemcpyDeviceToHost, d->stream));
   PQ_HIP(hipStreamSynchronize(d->stream));
@@ -238,15 +242,25 @@ int pq_diag_create(pq_diag **out, int g, int layers, const double *h, const doub
 void *pq_diag_stream(pq_diag *d) { return d ? (void *)d->stream : nullptr; }
 
 int pq_diag_energy_dev(pq_diag *d, const void *psi, long ld1, long ld2, double *out_dev, void *stream) {
+  return pq::diag_energy_dev_es(d, psi, ld1, ld2, out_dev, stream, 8);
+}
+
+}  // extern "C"
+
+int pq::diag_energy_dev_es(pq_diag *d, const void *psi, long ld1, long ld2, double *out_dev,
+                           void *stream, int es) {
   PQ_CHECK_ARG(d && psi && out_dev, "pq_diag_energy_dev: null argument");
   PQ_CHECK_ARG(ld1 >= d->g && ld2 >= d->g, "pq_diag_energy_dev: leading dimensions smaller than g=%d",
                d->g);
   const hipStream_t s = pq_resolve_stream(stream, d->stream);
   EnergyArgs a = d->args;
-  a.psi = (const double *)psi;
+  a.psi = psi;
   a.ld1 = ld1;
   a.lstride = ld1 * ld2;
-  hipLaunchKernelGGL(k_energy_part, d->grid, dim3(kRows * kWaves), 0, s, a, d->part);
+  if (es == 4)
+    hipLaunchKernelGGL(k_energy_part<float>, d->grid, dim3(kRows * kWaves), 0, s, a, d->part);
+  else
+    hipLaunchKernelGGL(k_energy_part<double>, d->grid, dim3(kRows * kWaves), 0, s, a, d->part);
   hipLaunchKernelGGL(k_energy_final, dim3(d->L + 1), dim3(256), 0, s, a, (const double *)d->part,
                      out_dev);
   hipError_t e = hipGetLastError();
@@ -256,6 +270,8 @@ int pq_diag_energy_dev(pq_diag *d, const void *psi, long ld1, long ld2, double *
   }
   return PQ_OK;
 }
+
+extern "C" {
 
 int pq_diag_energy(pq_diag *d, const double *psi, int ld1, int ld2, double *ekin, double *epot) {
   PQ_CHECK_ARG(d && psi && ekin && epot, "pq_diag_energy: null argument");
@@ -272,12 +288,24 @@ int pq_diag_energy(pq_diag *d, const double *psi, int ld1, int ld2, double *ekin
 
 int pq_psi_av_add_dev(int g, int layers, double dt, const void *psi, long ld1, long ld2, double *psi_av,
                       long la1, long la2, void *stream) {
+  return pq::psi_av_add_es(g, layers, dt, psi, ld1, ld2, psi_av, la1, la2, stream, 8);
+}
+
+}  // extern "C"
+
+int pq::psi_av_add_es(int g, int layers, double dt, const void *psi, long ld1, long ld2, double *psi_av,
+                      long la1, long la2, void *stream, int es) {
   PQ_CHECK_ARG(psi && psi_av, "pq_psi_av_add_dev: null argument");
   PQ_CHECK_ARG(g >= 1 && layers >= 1 && ld1 >= g && ld2 >= g && la1 >= g && la2 >= g,
                "pq_psi_av_add_dev: bad size g=%d layers=%d", g, layers);
   const hipStream_t s = pq_resolve_stream(stream, (hipStream_t)0);
-  hipLaunchKernelGGL(k_psi_av_add, dim3((g + 63) / 64, (g + 3) / 4, layers), dim3(256), 0, s, g, dt,
-                     (const double *)psi, ld1, ld1 * ld2, psi_av, la1, la1 * la2);
+  const dim3 grid((g + 63) / 64, (g + 3) / 4, layers);
+  if (es == 4)
+    hipLaunchKernelGGL(k_psi_av_add<float>, grid, dim3(256), 0, s, g, dt, (const float *)psi, ld1,
+                       ld1 * ld2, psi_av, la1, la1 * la2);
+  else
+    hipLaunchKernelGGL(k_psi_av_add<double>, grid, dim3(256), 0, s, g, dt, (const double *)psi, ld1,
+                       ld1 * ld2, psi_av, la1, la1 * la2);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     pq_set_error("pq_psi_av_add_dev: kernel launch failed: %s", hipGetErrorString(e));
@@ -285,5 +313,3 @@ int pq_psi_av_add_dev(int g, int layers, double dt, const void *psi, long ld1, l
   }
   return PQ_OK;
 }
-
-}  // extern "C"

@@ -66,6 +66,12 @@ void model_account_step(pq_model *m);
 size_t model_device_bytes(const pq_model *m);
 
 // ---- diagnostics (pq_diag.hip)
-int diag_energy_sync(pq_diag *d, const void *psi, long ld1, long ld2, double *ekin, double *epot);
+// psi of es-byte values (8: FP64, 4: FP32, read widened).
+int diag_energy_sync(pq_diag *d, const void *psi, long ld1, long ld2, double *ekin, double *epot,
+                     int es = 8);
+int diag_energy_dev_es(pq_diag *d, const void *psi, long ld1, long ld2, double *out_dev, void *stream,
+                       int es);
+int psi_av_add_es(int g, int layers, double dt, const void *psi, long ld1, long ld2, double *psi_av,
+                  long la1, long la2, void *stream, int es);
 
 }  // namespace pq

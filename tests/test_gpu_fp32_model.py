@@ -126,7 +126,7 @@ def test_fp32_models_in_ensemble_match_standalone(dev, graph):
 def test_fp32_model_files_and_restart(dev, tmp_path):
     """The snapshot files of an FP32 model hold its values widened to FP64
     (byte for byte the oracle's writer on them), and a START_PICKUP restart
-    continues bit for bit; time mean and energy need FP64 storage."""
+    continues bit for bit."""
     print("fp32 model io", flush=True)
     g, N, M = 33, 5, 4
     c, st, nd, kw, q = _setup(g)
@@ -155,10 +155,33 @@ def test_fp32_model_files_and_restart(dev, tmp_path):
     r.step(M)
     for name in ("psi", "zeta", "zeta_old", "ui", "uj", "zeta_fi", "zeta_fj", "res", "res_beta"):
         np.testing.assert_array_equal(r.get(name), a.get(name), err_msg=name)
-    with pytest.raises(PqError):
-        a.average(True, 0.0)
-    with pytest.raises(PqError):
-        a.energy()
+
+
+def test_fp32_model_time_mean_and_energy(dev):
+    """The FP64 time mean and energy of an FP32 model (its psi read widened):
+    against the FP64 oracle's model within 1e-6 (the FP32 state's error),
+    and the energy exactly the oracle's energy of the model's own psi."""
+    print("fp32 model diagnostics", flush=True)
+    c, st, nd, kw, q = _setup()
+    gm = qg.Model(c.g, st, nd["beta"], c.wind(), precision="fp32", **kw)
+    om = O.Model(c.g, st.s1, st.s2, st.ss, st.modes_to_z, st.z_to_modes, nd["beta"], c.wind(), **kw)
+    gm.init(q)
+    om.init(q)
+    dt = nd["dt"]
+    max_spin = gm.scalars()["model_time"] + 2.2 * dt
+    gm.average(True, max_spin)
+    om.average(max_spin)
+    gm.step(7)
+    om.step(7)
+    a, t = gm.psi_av()
+    b, tb = om.psi_av()
+    assert t == tb
+    assert _rel(a, b) <= 1e-6
+    ekin, epot = gm.energy()
+    ek2, ep2 = O.energy(om.a["psi"], st.h, st.s2)
+    assert np.all(np.abs(ekin - ek2) <= 1e-6 * np.abs(ek2)) and abs(epot - ep2) <= 1e-6 * ep2
+    ek3, ep3 = O.energy(gm.get("psi"), st.h, st.s2)
+    assert np.all(np.abs(ekin - ek3) <= 1e-12 * np.abs(ek3)) and abs(epot - ep3) <= 1e-12 * ep3
 
 
 def test_fp32_model_served_matches_standalone(dev):
