@@ -71,11 +71,14 @@ struct Member {
   int step_kind = 0;  // model_step_kind at the capture
   hipEvent_t ev = nullptr;
   long long running = -1;
+  int left = 0;  // steps of the running advance not yet launched
 };
 
 // An advance of n steps replays a graph of kStepsPerGraph steps n / k times
 // and the one-step graph for the rest: one launch per step would make the
-// scheduler thread's launch rate the engine's limit.
+// scheduler thread's launch rate the engine's limit.  The chunks of one
+// advance are launched one at a time, each after the previous completes, so
+// inversions that arrive meanwhile are not queued behind the whole advance.
 constexpr int kStepsPerGraph = 10;
 
 struct Class {
@@ -137,15 +140,41 @@ bool pq_srv::pass() {
       progress = true;
     }
   }
+  // one chunk of a model member's advance (kStepsPerGraph steps, or the
+  // remaining single steps), then its completion event
+  auto launch_chunk = [&](Member &m) -> int {
+    hipStream_t s = model_stream(m.model);
+    // a captured step goes stale when pq_model_average changed what a step does
+    if (model_step_kind(m.model) != m.step_kind) return PQ_ERR_ARG;
+    if (m.left >= kStepsPerGraph) {
+      if (hipGraphLaunch(m.exec_k, s) != hipSuccess) return PQ_ERR_HIP;
+      for (int q = 0; q < kStepsPerGraph; ++q) model_account_step(m.model);
+      m.left -= kStepsPerGraph;
+    } else {
+      for (; m.left > 0; --m.left) {
+        if (hipGraphLaunch(m.exec, s) != hipSuccess) return PQ_ERR_HIP;
+        model_account_step(m.model);
+      }
+    }
+    return hipEventRecord(m.ev, s) == hipSuccess ? PQ_OK : PQ_ERR_HIP;
+  };
   for (auto &mp : mem) {
     Member &m = *mp;
     if (m.kind != 1 || m.running < 0) continue;
     const hipError_t q = hipEventQuery(m.ev);
     if (q == hipErrorNotReady) continue;
-    finish(m.running, q == hipSuccess);
-    m.running = -1;
-    m.busy = false;
     progress = true;
+    if (q == hipSuccess && m.left > 0) {
+      const int rc = launch_chunk(m);
+      if (rc == PQ_OK) continue;
+      error.store(rc);
+      finish(m.running, false);
+    } else {
+      finish(m.running, q == hipSuccess);
+    }
+    m.running = -1;
+    m.left = 0;
+    m.busy = false;
   }
   // inversion batches: per class and free handle, the oldest waiting requests
   for (size_t ci = 0; ci < cls.size(); ++ci) {
@@ -201,20 +230,10 @@ bool pq_srv::pass() {
     const long long id = m.q.front();
     m.q.pop_front();
     Req &r = req[id];
-    hipStream_t s = model_stream(m.model);
-    // a captured step goes stale when pq_model_average changed what a step does
-    int rc = model_step_kind(m.model) == m.step_kind ? PQ_OK : PQ_ERR_ARG;
-    for (int left = r.steps; left > 0 && rc == PQ_OK;) {
-      const int k = left >= kStepsPerGraph ? kStepsPerGraph : 1;
-      if (hipGraphLaunch(k > 1 ? m.exec_k : m.exec, s) != hipSuccess) {
-        rc = PQ_ERR_HIP;
-      } else {
-        for (int q = 0; q < k; ++q) model_account_step(m.model);
-        left -= k;
-      }
-    }
-    if (rc == PQ_OK && hipEventRecord(m.ev, s) != hipSuccess) rc = PQ_ERR_HIP;
+    m.left = r.steps;
+    const int rc = launch_chunk(m);
     if (rc != PQ_OK) {
+      m.left = 0;
       error.store(rc);
       finish(id, false);
     } else {
