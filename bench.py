@@ -176,6 +176,47 @@ def ensemble_step_rates(torch, dev, precision, log, members=64, steps=200, ref_s
             "dt_range_nondim": [round(min(dts), 5), round(max(dts), 5)]}
 
 
+def large_rates(torch, dev, log, batch=2, reps=20):
+    """The 6-layer 2047^2 workload (dx 2.5 km, H = [150, 250, 400, 700, 1200,
+    2300] m, g' 0.03 -> 0.005): inversions/s in FP32 (storage and
+    transforms), FP32 storage with FP64 transforms, and the FP64 drop-in, at
+    batch `batch`; the reduced precisions' max relative psi error against the
+    FP64 result of the same field (itself pinned to the oracle by the
+    tests)."""
+    from pequod_amd import qg, synth
+
+    c = synth.config("qg6_2047")
+    st = c.stratification()
+    q = c.q()
+    base = torch.from_numpy(np.ascontiguousarray(q.transpose(2, 1, 0))).to(dev)
+    out, ref = {}, None
+    log(f"[bench] 6-layer 2047^2: batch {batch}")
+    for prec in ("fp64", "fp32", "fp32s"):
+        dt = torch.float64 if prec == "fp64" else torch.float32
+        hc = qg.HelmholtzCache(c.g, st, prec, batch_max=batch)
+        zs = [(base * (1.0 + 0.01 * k)).to(dt) for k in range(batch)]
+        ps = [torch.empty_like(z) for z in zs]
+        hc.run_dev(zs, ps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hc.run_dev(zs, ps)
+        torch.cuda.synchronize()
+        rate = batch * reps / (time.perf_counter() - t0)
+        assert hc.status()[0] == 0
+        r = {"inversions_per_s": round(rate, 1)}
+        p0 = ps[0].double()
+        if prec == "fp64":
+            ref = p0
+        else:
+            r["max_rel_psi_err_vs_fp64"] = float(max((p0[l] - ref[l]).abs().max() / ref[l].abs().max()
+                                                     for l in range(ref.shape[0])))
+        out[prec] = r
+        hc.deallocate()
+    return {"workload": "6-layer QG, 2047^2 interior (G=2049), dx 2.5 km, PV inversion",
+            "batch": batch, **out}
+
+
 def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11, load=0.5):
     """Serving leg: Poisson arrivals (mean `rate` per s over `window` s), 70 %
     inversions of field members and 30 % `steps`-step advances of CABARET
@@ -318,6 +359,8 @@ def main():
     ap.add_argument("--no-cabaret", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-member mixed-size ensemble leg")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 6-layer 2047^2 leg")
     ap.add_argument("--no-ensemble-step", action="store_true",
                     help="skip the 64-member ensemble time-step (CABARET models) leg")
     ap.add_argument("--no-serving", action="store_true",
@@ -517,6 +560,10 @@ def main():
     if not args.no_ensemble_step:
         ens_step = ensemble_step_rates(torch, dev, args.ensemble_precision, log)
 
+    large = None
+    if not args.no_large:
+        large = large_rates(torch, dev, log)
+
     chan = None
     if not args.no_channel:
         chan = channel_rates(torch, dev, c, st, log, check=not args.no_cpu)
@@ -588,6 +635,7 @@ def main():
         "ensemble_64_mixed": ens,
         "ensemble_64_mixed_steps": ens_step,
         "channel": chan,
+        "qg6_2047": large,
         "serving": serving,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
