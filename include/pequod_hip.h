@@ -191,7 +191,8 @@ int pq_inv_pass_times(pq_inv *h, int batch, const void *const *zeta,
  * asynchronous on `stream`.
  * ==================================================================== */
 /* cabaret_cfl (qg3p_aux.F90:920-943): writes max|u| * dt to *cfl_host
- * (synchronous).                                                         */
+ * (synchronous; the maximum is formed in an internal buffer of the
+ * stream's, allocated on its first call).                                */
 int pq_cabaret_cfl_dev(int g, int layers, double dt, const double *ui,
                        const double *uj, double *cfl_host, void *stream);
 int pq_cabaret_predictor_dev(int g, int layers, double dt,
@@ -210,14 +211,16 @@ int pq_cabaret_cell_boundaries_dev(int g, int layers, double tweight_u,
 int pq_cabaret_cell_boundaries_velocity_dev(int g, int layers,
                                             const double *psi, double *ui,
                                             double *uj, void *stream);
-/* zeta_fi / zeta_fj are updated in place; scratch must hold one i-face and
- * one j-face array ((G+3)(G+2)L + (G+2)(G+3)L doubles) or be NULL (the
- * library then uses a per-size internal buffer).                          */
+/* cabaret_extrapolator (qg3p_aux.F90:1317-1544): zeta_fi / zeta_fj are
+ * updated in place; work must hold one i-face and one j-face array
+ * ((G+3)(G+2)L + (G+2)(G+3)L doubles) or be NULL: the library then uses an
+ * internal buffer of the stream's, allocated on the first call and grown
+ * only when a later call on that stream needs more.                      */
 int pq_cabaret_extrapolator_dev(int g, int layers, double dt,
                                 const double *zeta, const double *zeta_old,
                                 const double *ui, const double *uj,
                                 double *zeta_fi, double *zeta_fj,
-                                double *scratch, void *stream);
+                                double *work, void *stream);
 int pq_cabaret_flux_dev(int g, int layers, const double *ui, const double *uj,
                         const double *zeta_fi, const double *zeta_fj,
                         double *flux_i, double *flux_j, void *stream);
@@ -290,6 +293,14 @@ int pq_model_init(pq_model *m, const double *zeta_sdata_host);
  * write (1, 2, 1, ...; pq_model_init leaves level 1 next, after the
  * forward Euler start into level 2); final_partial must be 0.             */
 int pq_model_step(pq_model *m, int nsteps, int final_partial);
+/* Status of the model's PV inversions since the last check (the reference
+ * checks psiC in every pv_inversion, qg3p_aux.F90:504-517): waits for the
+ * model's work, returns PQ_ERR_NAN after a NaN psiC (the reference aborts)
+ * or PQ_WARN_LARGE after |psiC| > 1e100, and clears it.  pq_model_step is
+ * asynchronous and returns no such status, except that with adapt_dt, whose
+ * CFL maximum comes back to the host every step, a step stops with
+ * PQ_ERR_NAN once an earlier inversion gave a NaN psiC.                    */
+int pq_model_status(pq_model *m);
 /* Copy a field to the host: which = 0 psi, 1 zeta, 2 rel, 3 zeta_old,
  * 4 ui, 5 uj, 6 zeta_fi, 7 zeta_fj, 8 res, 9 res_beta, 10 uimo, 11 ujmo,
  * 12 flux_i, 13 flux_j (CABARET); a centred-scheme model has 0-3 for time
@@ -444,7 +455,11 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p,
 pq_model *pq_srv_model(pq_srv *s, int member);  /* e.g. for pq_model_get */
 /* kind 0: inversion of a field member; 1: `steps` steps of a model member. */
 int pq_srv_submit(pq_srv *s, int member, int kind, int steps, long long *request);
-/* 1 done (latency_s = completion - submission), 0 pending, < 0 failed.    */
+/* 1 done (latency_s = completion - submission), 0 pending, or minus the
+ * failure's code: -PQ_ERR_NAN when the request's inversion gave a NaN psiC
+ * (the reference aborts, qg3p_aux.F90:504-509), -PQ_ERR_HIP when a launch
+ * or the device failed.  A done request whose inversion gave |psiC| >
+ * 1e100 returns 1 and leaves the reference's warning in pq_last_error.    */
 int pq_srv_poll(pq_srv *s, long long request, double *latency_s);
 /* Waits for every submitted request (at most timeout_s seconds).          */
 int pq_srv_drain(pq_srv *s, double timeout_s);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "pq_model_destroy": (_i, [_vp]),
     "pq_model_init": (_i, [_vp, _dp]),
     "pq_model_step": (_i, [_vp, _i, _i]),
+    "pq_model_status": (_i, [_vp]),
     "pq_model_get": (_i, [_vp, _i, _dp, _sz]),
     "pq_model_set": (_i, [_vp, _i, _dp, _sz]),
     "pq_model_scalars": (_i, [_vp, _dp]),

@@ -19,6 +19,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -664,7 +667,54 @@ int check_launch(const char *what) {
 #define CHECK_GL(g, L, fn)                                                     \
   PQ_CHECK_ARG((g) >= 5 && (L) >= 1 && (L) <= 64, "%s: bad size g=%d L=%d", fn, g, L)
 
+// Work buffers of the stage entry points that need one (the CFL maximum,
+// and the extrapolator's new face values when the caller passes no
+// scratch), one per (device, stream).  A buffer grows when a larger grid
+// comes and is kept until the process ends, so a call allocates only when
+// it needs more than any earlier call on its stream.  Work on one stream is
+// ordered, so calls on a stream share its buffer safely; the growth frees
+// the old buffer in stream order.
+struct StageBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_stage_mu;
+std::map<std::pair<int, hipStream_t>, StageBuf> g_stage_bufs;
+
+int stage_buffer(hipStream_t s, size_t bytes, void **out) {
+  int dev = 0;
+  PQ_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  StageBuf &b = g_stage_bufs[std::make_pair(dev, s)];
+  if (b.bytes < bytes) {
+    if (b.p) PQ_HIP(hipFreeAsync(b.p, s));
+    b.p = nullptr;
+    b.bytes = 0;
+    PQ_HIP(hipMallocAsync(&b.p, bytes, s));
+    b.bytes = bytes;
+  }
+  *out = b.p;
+  return PQ_OK;
+}
+
+// The extrapolator into a work area tmp (one i-face and one j-face array),
+// then the new face values copied over zeta_fi / zeta_fj
+// (cabaret_extrapolator, qg3p_aux.F90:1317-1544, whose zeta_fin / zeta_fjn
+// are `save` arrays).  The copy reads exactly the faces the first kernel
+// wrote, so tmp needs no clearing.
+int extrapolator_launch(const Dims &d, double dt, const double *zeta, const double *zeta_old,
+                        const double *ui, const double *uj, double *zfi, double *zfj, double *tmp,
+                        hipStream_t s) {
+  const int g = d.g;
+  hipLaunchKernelGGL(k_extrapolator<double>, grid_for(g - 1, g - 1, d.L), dim3(BX, BY), 0, s, d, dt,
+                     zeta, zeta_old, ui, uj, zfi, zfj, tmp, tmp + d.n_fi());
+  hipLaunchKernelGGL(k_face_copy<double>, grid_for(g - 1, g - 1, d.L), dim3(BX, BY), 0, s, d, tmp,
+                     tmp + d.n_fi(), zfi, zfj);
+  return PQ_OK;
+}
+
 }  // namespace
+
 
 // -------------------------------------------------------------------- C ABI
 extern "C" {
@@ -675,12 +725,12 @@ int pq_cabaret_cfl_dev(int g, int L, double dt, const double *ui, const double *
   PQ_CHECK_ARG(ui && uj && cfl_host, "pq_cabaret_cfl_dev: null pointer");
   Dims d{g, L};
   unsigned long long *dm = nullptr;
-  PQ_HIP(hipMallocAsync((void **)&dm, sizeof(unsigned long long), S(stream)));
+  int rc = stage_buffer(S(stream), sizeof(unsigned long long), (void **)&dm);
+  if (rc) return rc;
   PQ_HIP(hipMemsetAsync(dm, 0, sizeof(unsigned long long), S(stream)));
   hipLaunchKernelGGL(k_maxabs<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, ui, uj, dm);
   unsigned long long bits = 0;
   PQ_HIP(hipMemcpyAsync(&bits, dm, sizeof(bits), hipMemcpyDeviceToHost, S(stream)));
-  PQ_HIP(hipFreeAsync(dm, S(stream)));
   PQ_HIP(hipStreamSynchronize(S(stream)));
   double m;
   std::memcpy(&m, &bits, sizeof(m));
@@ -730,18 +780,18 @@ int pq_cabaret_cell_boundaries_velocity_dev(int g, int L, const double *psi, dou
 
 int pq_cabaret_extrapolator_dev(int g, int L, double dt, const double *zeta,
                                 const double *zeta_old, const double *ui, const double *uj,
-                                double *zfi, double *zfj, double *scratch, void *stream) {
+                                double *zfi, double *zfj, double *work, void *stream) {
   CHECK_GL(g, L, "pq_cabaret_extrapolator_dev");
   Dims d{g, L};
-  double *tmp = scratch;
-  if (!tmp) PQ_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (d.n_fi() + d.n_fj()), S(stream)));
-  hipLaunchKernelGGL(k_extrapolator<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, dt,
-                     zeta, zeta_old, ui, uj, zfi, zfj, tmp, tmp + d.n_fi());
-  hipLaunchKernelGGL(k_face_copy<double>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, S(stream), d, tmp,
-                     tmp + d.n_fi(), zfi, zfj);
-  if (!scratch) PQ_HIP(hipFreeAsync(tmp, S(stream)));
+  double *tmp = work;
+  if (!tmp) {
+    int rc = stage_buffer(S(stream), sizeof(double) * (d.n_fi() + d.n_fj()), (void **)&tmp);
+    if (rc) return rc;
+  }
+  extrapolator_launch(d, dt, zeta, zeta_old, ui, uj, zfi, zfj, tmp, S(stream));
   return check_launch("pq_cabaret_extrapolator_dev");
 }
+
 
 int pq_cabaret_flux_dev(int g, int L, const double *ui, const double *uj, const double *zfi,
                         const double *zfj, double *flux_i, double *flux_j, void *stream) {
@@ -929,7 +979,8 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
     m->bytes += a.n * sizeof(double);
   }
   if (hipMalloc((void **)&m->maxu, sizeof(unsigned long long)) != hipSuccess) return fail(PQ_ERR_NOMEM);
-  if (hipHostMalloc((void **)&m->maxu_host, sizeof(unsigned long long), 0) != hipSuccess)
+  // [0] the maximum's bits, [1] the inversion status word read with it
+  if (hipHostMalloc((void **)&m->maxu_host, 2 * sizeof(unsigned long long), 0) != hipSuccess)
     return fail(PQ_ERR_NOMEM);
   (void)hipMemcpy(m->fw, p->force_wind, sizeof(double) * p->g * p->g, hipMemcpyHostToDevice);
   (void)hipMemcpy(m->s1, p->s1, sizeof(double) * p->layers, hipMemcpyHostToDevice);
@@ -1136,7 +1187,16 @@ template <typename TS> static int model_adapt_dt(pq_model *m) {
                      d, fld<TS>(m->ui), fld<TS>(m->uj), m->maxu);
   PQ_HIP(hipMemcpyAsync(m->maxu_host, m->maxu, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         m->stream));
+  // the status of the inversions since the last check comes back with the
+  // maximum: a NaN psiC stops the model, as the reference aborts at its
+  // first one (qg3p_aux.F90:504-509)
+  int *st_host = reinterpret_cast<int *>(m->maxu_host + 1);
+  PQ_HIP(hipMemcpyAsync(st_host, pq::inv_status_dev(m->inv), sizeof(int), hipMemcpyDeviceToHost, m->stream));
   PQ_HIP(hipStreamSynchronize(m->stream));
+  if (*st_host & 1) {
+    pq_set_error("NaN psiC encountered");
+    return PQ_ERR_NAN;
+  }
   double max_u;
   std::memcpy(&max_u, m->maxu_host, sizeof(double));
   const double old_dt = m->dt;
@@ -1252,6 +1312,7 @@ int model_step_kind(const pq_model *m) {
   if (!m->average) return 0;
   return m->model_time + 0.5 * m->dt > m->max_spin ? 1 : -1;
 }
+int *model_status_dev(pq_model *m) { return inv_status_dev(m->inv); }
 size_t model_device_bytes(const pq_model *m) {
   return m->bytes + (m->psi_av ? sizeof(double) * m->d.n_rs() : 0) + inv_device_bytes(m->inv);
 }
@@ -1345,6 +1406,11 @@ int pq_model_set(pq_model *m, int which, const double *host, size_t count) {
   PQ_CHECK_ARG(count >= n, "pq_model_set: field %d needs %zu values, buffer holds %zu", which, n, count);
   PQ_HIP(hipStreamSynchronize(m->stream));
   return field_upload(m, *f, host, n, m->stream);
+}
+
+int pq_model_status(pq_model *m) {
+  PQ_CHECK_ARG(m != nullptr, "pq_model_status: null model");
+  return pq_inv_status(m->inv, nullptr);
 }
 
 int pq_model_scalars(pq_model *m, double *out4) {

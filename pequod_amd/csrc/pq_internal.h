@@ -26,6 +26,11 @@ int inv_run_members(pq_inv *h, int batch, const void *const *zeta, long ldz1, lo
                     const int *layers, hipStream_t s);
 // Device bytes held by the handle (tables, scratch, coefficients).
 size_t inv_device_bytes(const pq_inv *h);
+// The handle's per-slot status words on the device (inv_batch_max ints):
+// bit 0 NaN psiC, bit 1 |psiC| > 1e100 (qg3p_aux.F90:504-517); the
+// inversion ORs into them and only pq_inv_status / the caller clears them.
+int *inv_status_dev(pq_inv *h);
+int inv_batch_max(const pq_inv *h);
 
 // ---- CABARET model (pq_cabaret.hip)
 int model_step_pre(pq_model *m, hipStream_t s);
@@ -64,6 +69,8 @@ int model_step_kind(const pq_model *m);
 // Host-side bookkeeping of one replayed step (clocks as model_step_post).
 void model_account_step(pq_model *m);
 size_t model_device_bytes(const pq_model *m);
+// The model inversion's status word on the device (see inv_status_dev).
+int *model_status_dev(pq_model *m);
 
 // ---- diagnostics (pq_diag.hip)
 // psi of es-byte values (8: FP64, 4: FP32, read widened).

@@ -635,6 +635,9 @@ int inv_run_members(pq_inv *h, int batch, const void *const *zeta, long ldz1, lo
   return inv_launch(h, batch, zeta, ldz1, ldz2, psi, ldp1, ldp2, s, 7, coef, layers);
 }
 
+int *inv_status_dev(pq_inv *h) { return h->status; }
+int inv_batch_max(const pq_inv *h) { return h->batch_max; }
+
 size_t inv_device_bytes(const pq_inv *h) {
   const size_t N = (size_t)h->N, L = (size_t)h->L, B = (size_t)h->batch_max;
   return h->scr_member * B +

@@ -51,6 +51,10 @@ struct Req {
   int member = -1, kind = 0, steps = 0;
   double t_sub = 0, t_start = 0, t_done = 0;
   std::atomic<int> state{0};  // 0 queued, 1 running, 2 done, 3 failed
+  // why a request failed (PQ_ERR_NAN: its inversion gave a NaN psiC,
+  // PQ_ERR_HIP: a launch or the device failed), or PQ_WARN_LARGE for a done
+  // request whose inversion gave |psiC| > 1e100
+  int code = PQ_OK;
 };
 
 struct Member {
@@ -72,6 +76,7 @@ struct Member {
   hipEvent_t ev = nullptr;
   long long running = -1;
   int left = 0;  // steps of the running advance not yet launched
+  int *st_host = nullptr;  // pinned: the model inversion's status after an advance
 };
 
 // An advance of n steps replays a graph of kStepsPerGraph steps n / k times
@@ -86,9 +91,24 @@ struct Class {
   long ldz1 = 0, ldz2 = 0, ldp1 = 0, ldp2 = 0;
   pq_inv *h[kHandlesPerClass] = {};
   hipEvent_t ev[kHandlesPerClass] = {};
+  // pinned: the handle's per-slot status words of the running batch, copied
+  // back (and cleared on the device) behind the batch, before its event
+  int *st_host[kHandlesPerClass] = {};
   std::vector<long long> running[kHandlesPerClass];
   size_t bytes = 0;
 };
+
+// Status bits of an inversion (bit 0 NaN psiC, bit 1 |psiC| > 1e100) as a
+// request's code.
+int status_code(int st) { return (st & 1) ? PQ_ERR_NAN : (st & 2) ? PQ_WARN_LARGE : PQ_OK; }
+
+// Enqueues, behind the work on s, the copy of n status words to the pinned
+// host array and their clearing on the device.
+int status_take(int *dev, int *host, int n, hipStream_t s) {
+  if (hipMemcpyAsync(host, dev, sizeof(int) * n, hipMemcpyDeviceToHost, s) != hipSuccess) return PQ_ERR_HIP;
+  if (hipMemsetAsync(dev, 0, sizeof(int) * n, s) != hipSuccess) return PQ_ERR_HIP;
+  return PQ_OK;
+}
 
 }  // namespace
 
@@ -119,9 +139,12 @@ bool pq_srv::pass() {
   for (long long id : fresh) mem[req[id].member]->q.push_back(id);
   bool progress = false;
   const double t = now_s();
-  auto finish = [&](long long id, bool ok) {
+  // code: PQ_OK or PQ_WARN_LARGE (done), else the failure
+  auto finish = [&](long long id, int code) {
     Req &r = req[id];
+    const bool ok = code == PQ_OK || code == PQ_WARN_LARGE;
     r.t_done = t;
+    r.code = code;
     r.state.store(ok ? 2 : 3, std::memory_order_release);
     (ok ? done : failed).fetch_add(1);
   };
@@ -132,8 +155,10 @@ bool pq_srv::pass() {
       if (c.running[k].empty()) continue;
       const hipError_t q = hipEventQuery(c.ev[k]);
       if (q == hipErrorNotReady) continue;
-      for (long long id : c.running[k]) {
-        finish(id, q == hipSuccess);
+      // slot b of the batch is the b-th request launched
+      for (size_t b = 0; b < c.running[k].size(); ++b) {
+        const long long id = c.running[k][b];
+        finish(id, q == hipSuccess ? status_code(c.st_host[k][b]) : PQ_ERR_HIP);
         mem[req[id].member]->busy = false;
       }
       c.running[k].clear();
@@ -156,6 +181,11 @@ bool pq_srv::pass() {
         model_account_step(m.model);
       }
     }
+    // after the advance's last chunk, the status of its inversions
+    if (m.left == 0) {
+      const int rc = status_take(model_status_dev(m.model), m.st_host, 1, s);
+      if (rc) return rc;
+    }
     return hipEventRecord(m.ev, s) == hipSuccess ? PQ_OK : PQ_ERR_HIP;
   };
   for (auto &mp : mem) {
@@ -168,9 +198,9 @@ bool pq_srv::pass() {
       const int rc = launch_chunk(m);
       if (rc == PQ_OK) continue;
       error.store(rc);
-      finish(m.running, false);
+      finish(m.running, rc);
     } else {
-      finish(m.running, q == hipSuccess);
+      finish(m.running, q == hipSuccess ? status_code(*m.st_host) : PQ_ERR_HIP);
     }
     m.running = -1;
     m.left = 0;
@@ -205,13 +235,14 @@ bool pq_srv::pass() {
       hipStream_t s = (hipStream_t)pq_inv_stream(c.h[k]);
       int rc = inv_run_members(c.h[k], (int)cand.size(), z.data(), c.ldz1, c.ldz2, p.data(), c.ldp1,
                                c.ldp2, cf.data(), lay.data(), s);
+      if (rc == PQ_OK) rc = status_take(inv_status_dev(c.h[k]), c.st_host[k], (int)cand.size(), s);
       if (rc == PQ_OK && hipEventRecord(c.ev[k], s) != hipSuccess) rc = PQ_ERR_HIP;
       for (int mi : cand) {
         Member &m = *mem[mi];
         const long long id = m.q.front();
         m.q.pop_front();
         if (rc != PQ_OK) {
-          finish(id, false);
+          finish(id, rc);
           continue;
         }
         req[id].t_start = t;
@@ -235,7 +266,7 @@ bool pq_srv::pass() {
     if (rc != PQ_OK) {
       m.left = 0;
       error.store(rc);
-      finish(id, false);
+      finish(id, rc);
     } else {
       r.t_start = t;
       r.state.store(1, std::memory_order_release);
@@ -254,6 +285,37 @@ void pq_srv::loop() {
 }
 
 namespace {
+
+void destroy_class(Class &c) {
+  for (int k = 0; k < kHandlesPerClass; ++k) {
+    pq_inv_destroy(c.h[k]);
+    if (c.ev[k]) (void)hipEventDestroy(c.ev[k]);
+    if (c.st_host[k]) (void)hipHostFree(c.st_host[k]);
+    c.h[k] = nullptr;
+    c.ev[k] = nullptr;
+    c.st_host[k] = nullptr;
+  }
+}
+
+// Destroys a member's device objects (not its footprint accounting).
+void destroy_member(Member &m) {
+  if (m.coef) (void)hipFree(m.coef);
+  if (m.exec) (void)hipGraphExecDestroy(m.exec);
+  if (m.graph) (void)hipGraphDestroy(m.graph);
+  if (m.exec_k) (void)hipGraphExecDestroy(m.exec_k);
+  if (m.graph_k) (void)hipGraphDestroy(m.graph_k);
+  if (m.ev) (void)hipEventDestroy(m.ev);
+  if (m.st_host) (void)hipHostFree(m.st_host);
+  if (m.model) pq_model_destroy(m.model);
+  m.coef = nullptr;
+  m.exec = nullptr;
+  m.graph = nullptr;
+  m.exec_k = nullptr;
+  m.graph_k = nullptr;
+  m.ev = nullptr;
+  m.st_host = nullptr;
+  m.model = nullptr;
+}
 
 int class_for(pq_srv *s, const Member &m, size_t *new_bytes) {
   auto key = std::make_tuple(m.g, m.ldz1, m.ldz2, m.ldp1, m.ldp2);
@@ -282,11 +344,10 @@ int class_for(pq_srv *s, const Member &m, size_t *new_bytes) {
                            s->max_batch);
     if (rc == PQ_OK && hipEventCreateWithFlags(&c->ev[k], hipEventDisableTiming) != hipSuccess)
       rc = PQ_ERR_HIP;
+    if (rc == PQ_OK && hipHostMalloc((void **)&c->st_host[k], sizeof(int) * s->max_batch, 0) != hipSuccess)
+      rc = PQ_ERR_NOMEM;
     if (rc) {
-      for (int q = 0; q <= k; ++q) {
-        pq_inv_destroy(c->h[q]);
-        if (c->ev[q]) (void)hipEventDestroy(c->ev[q]);
-      }
+      destroy_class(*c);
       return -rc;
     }
     c->bytes += inv_device_bytes(c->h[k]);
@@ -345,25 +406,30 @@ int pq_srv_open_field(pq_srv *s, int g, int layers, const double *ss, const doub
   // the member's zeta / psi views (the caller's memory, resident) and its
   // coefficient block count against the cap
   m->bytes = cb + 2 * el * (size_t)g * g * layers;
-  if (s->used + cbytes + m->bytes > s->cap) {
-    if (!had) {  // undo the class just created
-      for (int q = 0; q < kHandlesPerClass; ++q) {
-        pq_inv_destroy(c.h[q]);
-        (void)hipEventDestroy(c.ev[q]);
-      }
+  // a failure below leaves the engine as it was: a class created for this
+  // member goes again
+  auto undo = [&](int code) {
+    if (m->coef) (void)hipFree(m->coef);
+    m->coef = nullptr;
+    if (!had) {
+      destroy_class(c);
       s->cls.pop_back();
       s->key2cls.erase(std::make_tuple(g, ldz1, ldz2, ldp1, ldp2));
     }
+    return code;
+  };
+  if (s->used + cbytes + m->bytes > s->cap) {
     pq_set_error("pq_srv_open_field: %zu bytes would exceed the HBM footprint cap (%zu of %zu in use)",
                  cbytes + m->bytes, s->used, s->cap);
-    return PQ_ERR_NOMEM;
+    return undo(PQ_ERR_NOMEM);
   }
-  PQ_HIP(hipMalloc(&m->coef, cb));
+  if (hipMalloc(&m->coef, cb) != hipSuccess) {
+    m->coef = nullptr;
+    pq_set_error("pq_srv_open_field: hipMalloc of %zu bytes failed", cb);
+    return undo(PQ_ERR_NOMEM);
+  }
   int rc = inv_make_coefs(c.h[0], layers, ss, modes_to_z, z_to_modes, m->coef);
-  if (rc) {
-    (void)hipFree(m->coef);
-    return rc;
-  }
+  if (rc) return undo(rc);
   m->cls = k;
   s->used += cbytes + m->bytes;
   s->mem.push_back(std::move(m));
@@ -382,12 +448,7 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sd
   int rc = pq_model_create(&m->model, p);
   if (rc) return rc;
   auto fail = [&](int code) {
-    if (m->exec) (void)hipGraphExecDestroy(m->exec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
-    if (m->exec_k) (void)hipGraphExecDestroy(m->exec_k);
-    if (m->graph_k) (void)hipGraphDestroy(m->graph_k);
-    if (m->ev) (void)hipEventDestroy(m->ev);
-    pq_model_destroy(m->model);
+    destroy_member(*m);
     return code;
   };
   m->bytes = model_device_bytes(m->model);
@@ -435,6 +496,8 @@ int pq_srv_open_model(pq_srv *s, const pq_model_params *p, const double *zeta_sd
       return unreserve(PQ_ERR_HIP);
   }
   if (hipEventCreateWithFlags(&m->ev, hipEventDisableTiming) != hipSuccess) return unreserve(PQ_ERR_HIP);
+  if (hipHostMalloc((void **)&m->st_host, sizeof(int), 0) != hipSuccess) return unreserve(PQ_ERR_NOMEM);
+  *m->st_host = 0;
   std::lock_guard<std::mutex> lk(s->mu);
   s->mem.push_back(std::move(m));
   *member = (int)s->mem.size() - 1;
@@ -478,10 +541,16 @@ int pq_srv_poll(pq_srv *s, long long request, double *latency_s) {
   const Req &r = s->req[request];
   const int st = r.state.load(std::memory_order_acquire);
   if (st == 3) {
-    pq_set_error("pq_srv: request %lld failed", request);
-    return PQ_ERR_HIP;
+    if (r.code == PQ_ERR_NAN)
+      pq_set_error("pq_srv: request %lld failed: NaN psiC encountered", request);
+    else
+      pq_set_error("pq_srv: request %lld failed (status %d)", request, r.code);
+    return -r.code;
   }
   if (st != 2) return 0;
+  if (r.code == PQ_WARN_LARGE)
+    pq_set_error("pq_srv: request %lld: WARNING: Large psiC encountered; simulation may be blowing up",
+                 request);
   if (latency_s) *latency_s = r.t_done - r.t_sub;
   return 1;
 }
@@ -573,20 +642,7 @@ int pq_srv_close(pq_srv *s, int member) {
   for (long long id : s->incoming)
     if (s->req[id].member == member) pending = true;
   PQ_CHECK_ARG(!pending, "pq_srv_close: member %d has requests in flight", member);
-  if (m.coef) (void)hipFree(m.coef);
-  m.coef = nullptr;
-  if (m.exec) (void)hipGraphExecDestroy(m.exec);
-  if (m.graph) (void)hipGraphDestroy(m.graph);
-  if (m.exec_k) (void)hipGraphExecDestroy(m.exec_k);
-  if (m.graph_k) (void)hipGraphDestroy(m.graph_k);
-  if (m.ev) (void)hipEventDestroy(m.ev);
-  m.exec = nullptr;
-  m.graph = nullptr;
-  m.exec_k = nullptr;
-  m.graph_k = nullptr;
-  m.ev = nullptr;
-  if (m.model) pq_model_destroy(m.model);
-  m.model = nullptr;
+  destroy_member(m);
   m.open = false;
   s->used -= m.bytes;
   return PQ_OK;
@@ -597,22 +653,9 @@ int pq_srv_destroy(pq_srv *s) {
   (void)pq_srv_drain(s, 600.0);
   s->stop.store(true);
   if (s->th.joinable()) s->th.join();
-  for (auto &mp : s->mem) {
-    Member &m = *mp;
-    if (!m.open) continue;
-    if (m.coef) (void)hipFree(m.coef);
-    if (m.exec) (void)hipGraphExecDestroy(m.exec);
-    if (m.graph) (void)hipGraphDestroy(m.graph);
-    if (m.exec_k) (void)hipGraphExecDestroy(m.exec_k);
-    if (m.graph_k) (void)hipGraphDestroy(m.graph_k);
-    if (m.ev) (void)hipEventDestroy(m.ev);
-    if (m.model) pq_model_destroy(m.model);
-  }
-  for (auto &cp : s->cls)
-    for (int k = 0; k < kHandlesPerClass; ++k) {
-      pq_inv_destroy(cp->h[k]);
-      if (cp->ev[k]) (void)hipEventDestroy(cp->ev[k]);
-    }
+  for (auto &mp : s->mem)
+    if (mp->open) destroy_member(*mp);
+  for (auto &cp : s->cls) destroy_class(*cp);
   delete s;
   return PQ_OK;
 }

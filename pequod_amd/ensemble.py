@@ -13,7 +13,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, qg
 from ._lib import PREC, check, load
 from .strat import Stratification
 
@@ -49,6 +49,10 @@ class Ensemble:
         L, ld2, ld1 = zeta.shape
         if g is None:
             g = min(ld1, ld2)
+        # the element type of every precision's fields (fp32auto: FP32 storage)
+        es = _lib.IO_BYTES[self.precision]
+        qg.check_views([zeta], es, g, L, "zeta")
+        qg.check_views([psi], es, g, L, "psi")
         ss, m2z, z2m = _arrays(strat)
         self._keep += [ss, m2z, z2m, zeta, psi]
         idx = ctypes.c_int()

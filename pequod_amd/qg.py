@@ -40,6 +40,36 @@ def _stream_of(t):
     return ctypes.c_void_p(h) if h else PQ_STREAM_NULL
 
 
+def check_views(tensors, elem_bytes: int, g: int, layers: int, what: str,
+                require_cuda: bool = True) -> None:
+    """Checks a batch of device views before their raw pointers and the
+    first view's strides go to the C ABI, which reads and writes g x g x
+    layers elements of elem_bytes each through every pointer with those
+    strides: each tensor has shape (L, >= g, >= g) with L == layers, the
+    element size of the handle's precision, unit innermost stride and the
+    strides and device of the first.  Raises ValueError otherwise."""
+    if len(tensors) == 0:
+        raise ValueError(f"{what}: empty batch")
+    t0 = tensors[0]
+    for k, t in enumerate(tensors):
+        name = f"{what}[{k}]"
+        if require_cuda and not t.is_cuda:
+            raise ValueError(f"{name}: not a device (HIP) tensor")
+        if t.dim() != 3:
+            raise ValueError(f"{name}: {t.dim()}-d tensor; views are (L, ld2, ld1)")
+        if t.element_size() != elem_bytes:
+            raise ValueError(f"{name}: {t.element_size()}-byte elements, the handle's precision "
+                             f"stores {elem_bytes}-byte values")
+        if t.shape[0] != layers or t.shape[1] < g or t.shape[2] < g:
+            raise ValueError(f"{name}: shape {tuple(t.shape)} does not hold a {g} x {g} x {layers} view")
+        if t.stride(2) != 1 or t.stride(0) % t.stride(1) != 0:
+            raise ValueError(f"{name}: strides {t.stride()} are not a column-major (ld1, ld2) view")
+        if t.stride() != t0.stride():
+            raise ValueError(f"{name}: strides {t.stride()} differ from the batch's first {t0.stride()}")
+        if t.device != t0.device:
+            raise ValueError(f"{name}: on {t.device}, the batch's first on {t0.device}")
+
+
 def _require_cuda(*ts):
     for t in ts:
         if not t.is_cuda:
@@ -129,9 +159,15 @@ class HelmholtzCache:
 
     def run_dev(self, zetas, psis, stream=None) -> None:
         """zetas / psis: lists of device tensors of shape (L, ld2, ld1) whose
-        leading g x g blocks are the views (or 4-D views (L, ld2, ld1) offset
-        to element (1, 1))."""
+        leading g x g blocks are the views (or views of such tensors offset
+        to element (1, 1)), with the element type of the handle's precision
+        and the same strides throughout each list."""
         b = len(zetas)
+        if len(psis) != b or b > self.batch_max:
+            raise ValueError(f"{b} zeta and {len(psis)} psi views for a handle of batch {self.batch_max}")
+        es = _lib.IO_BYTES[self.precision]
+        check_views(zetas, es, self.g, self.layers, "zeta")
+        check_views(psis, es, self.g, self.layers, "psi")
         zt = (_vp * b)(*[_ptr(z) for z in zetas])
         pt = (_vp * b)(*[_ptr(p) for p in psis])
         z0, p0 = zetas[0], psis[0]
@@ -375,7 +411,15 @@ class Model:
         return check(load().pq_model_init(self._h, _np_ptr(z)), allow_warn=True)
 
     def step(self, nsteps: int = 1, final_partial: bool = False) -> None:
+        """Enqueues the steps (asynchronous; see status)."""
         check(load().pq_model_step(self._h, nsteps, int(final_partial)))
+
+    def status(self) -> int:
+        """Waits for the model's work; raises PqError on a NaN psiC in any
+        inversion since the last call (the reference aborts,
+        qg3p_aux.F90:504-509) and returns PQ_WARN_LARGE after |psiC| >
+        1e100, else PQ_OK."""
+        return check(load().pq_model_status(self._h), allow_warn=True)
 
     def shape(self, name: str):
         return field_shape(name, self.g, self.layers)

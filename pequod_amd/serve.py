@@ -16,8 +16,8 @@ import ctypes
 
 import numpy as np
 
-from . import qg
-from ._lib import PREC, check, load
+from . import _lib, qg
+from ._lib import PREC, PqError, check, load
 from .strat import Stratification
 
 _vp = ctypes.c_void_p
@@ -39,6 +39,9 @@ class Server:
         whose leading g x g blocks are the views."""
         L, ld2, ld1 = zeta.shape
         g = g or min(ld1, ld2)
+        es = _lib.IO_BYTES[self.precision]
+        qg.check_views([zeta], es, g, L, "zeta")
+        qg.check_views([psi], es, g, L, "psi")
         ss = np.ascontiguousarray(strat.ss, dtype=np.float64)
         m2z = np.asfortranarray(strat.modes_to_z, dtype=np.float64)
         z2m = np.asfortranarray(strat.z_to_modes, dtype=np.float64)
@@ -75,11 +78,13 @@ class Server:
         return r.value
 
     def poll(self, request: int):
-        """Latency [s] of a completed request, else None."""
+        """Latency [s] of a completed request, else None; raises PqError for
+        a failed request (code PQ_ERR_NAN when its inversion gave a NaN
+        psiC)."""
         lat = ctypes.c_double()
         rc = load().pq_srv_poll(self._h, request, ctypes.byref(lat))
         if rc < 0:
-            check(rc)
+            raise PqError(-rc, load().pq_last_error().decode(errors="replace"))
         return lat.value if rc == 1 else None
 
     def drain(self, timeout: float = 600.0) -> None:

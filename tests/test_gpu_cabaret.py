@@ -261,3 +261,37 @@ def test_model_step_is_the_stage_sequence(dev, L, bc):
         np.testing.assert_array_equal(gm.get(n), from_dev(T[n]), err_msg=f"final partial step: {n}")
     hc.deallocate()
     gm.close()
+
+
+@pytest.mark.parametrize("adapt", [False, True])
+def test_model_status_nan(dev, adapt):
+    """A NaN psiC in a model's inversion is reported like the reference's
+    abort (qg3p_aux.F90:504-509): by Model.status() after asynchronous steps,
+    and with adapt_dt by the next step itself."""
+    from pequod_amd._lib import PQ_ERR_NAN, PqError
+    c = synth.config("qg3_255")
+    c.n = 31
+    nd = c.nondim()
+    kw = dict(viscosity=nd["viscosity"], visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1,
+              alpha=nd["alpha"], adapt_dt=adapt, target_cfl=0.1, max_dt=nd["dt"])
+    m = qg.Model(c.g, nd["strat"], nd["beta"], c.wind(), **kw)
+    m.init(c.q())
+    m.step(2)
+    assert m.status() == 0
+    # the predictor starts from zeta_old (qg3p_aux.F90:993-1020)
+    for name in ("zeta", "zeta_old"):
+        z = m.get(name)
+        z[c.g // 2, c.g // 3, 0] = np.nan
+        m.set(name, z)
+    if adapt:
+        m.step(1)  # its inversion turns NaN; the next step's adapt_dt reads the status
+        with pytest.raises(PqError) as e:
+            m.step(1)
+        assert e.value.code == PQ_ERR_NAN
+    else:
+        m.step(2)
+        with pytest.raises(PqError) as e:
+            m.status()
+        assert e.value.code == PQ_ERR_NAN
+        assert m.status() == 0  # cleared once reported
+    m.close()

@@ -308,3 +308,28 @@ def test_inversion_six_layer_2047(dev, prec):
     print(f"{prec} rel err {err:.3e}", flush=True)
     bound = {"fp64": fp64_tol(g, st.modes_to_z), "fp32s": 1e-6, "fp32": 1e-6, "fp32w": 1e-6}[prec]
     assert err < bound
+
+
+@pytest.mark.parametrize("g,L", [(1537, 1), (1537, 3), (3073, 1), (3073, 3), (4097, 1)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_inversion_largest_lengths(dev, g, L, prec):
+    """The transform lengths N = 1536, 3072 and 4096 (the radix-24 final
+    stage and the largest LDS / register footprints) against the FP64
+    oracle on a seeded k^-3 field: FP64 to round-off, FP32 within 1e-6."""
+    import torch
+    print(f"pv_inversion {prec} g={g} L={L}", flush=True)
+    c = synth.config("qg3_1023")
+    c.n = g - 2
+    c.H, c.gprime = c.H[:L], c.gprime[:L - 1]
+    st = c.stratification()
+    q = c.q(member=g + L)
+    ref = _oracle_inv(g, st, q)
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    zt = to_dev(q, dev, dt)
+    pt = torch.zeros_like(zt)
+    hc = qg.HelmholtzCache(g, st, prec, tiles=None)
+    hc.run_dev([zt], [pt])
+    assert hc.status()[0] == 0
+    err = rel_err(from_dev(pt), ref)
+    print(f"  max rel err {err:.2e}", flush=True)
+    assert err < (fp64_tol(g, st.modes_to_z) if prec == "fp64" else 1e-6)

@@ -143,3 +143,55 @@ def test_serve_admission_by_footprint(dev):
     srv.drain(30)
     assert srv.stats()["count"] == 2
     srv.close()
+
+
+def test_serve_nan_member_fails(dev):
+    """A served inversion whose psiC is NaN fails with PQ_ERR_NAN (the
+    reference aborts, qg3p_aux.F90:504-509); the other members of its batch
+    complete, and the member's next clean request succeeds.  A model
+    member's advance whose inversions turn NaN fails the same way."""
+    import ctypes
+
+    import torch
+    from pequod_amd._lib import PQ_ERR_NAN, load
+    print("serve NaN member", flush=True)
+    srv = Server("fp32", cap_bytes=64e9, max_batch=4)
+    st, _, zt, pt = _field(33, 3, 21)
+    st2, _, zt2, pt2 = _field(33, 2, 22)
+    bad_z = zt.clone()
+    bad_z[1, 5, 7] = float("nan")
+    good = srv.open_field(st2, zt2, pt2)
+    bad = srv.open_field(st, bad_z, torch.zeros_like(pt))
+    r_bad, r_good = srv.submit(bad, Server.FIELD), srv.submit(good, Server.FIELD)
+    srv.drain(30)
+    with pytest.raises(PqError) as e:
+        srv.poll(r_bad)
+    assert e.value.code == PQ_ERR_NAN and "NaN" in str(e.value)
+    assert srv.poll(r_good) is not None
+    assert srv.stats()["failed"] == 1
+    bad_z.copy_(zt)
+    r = srv.submit(bad, Server.FIELD)
+    srv.drain(30)
+    assert srv.poll(r) is not None
+    # a model member whose state turns NaN
+    kw, q = _model_args(33, 3)
+    kw = dict(kw)
+    g = kw.pop("g")
+    m = srv.open_model(g, kw.pop("strat"), kw.pop("beta"), kw.pop("force_wind"), q, **kw)
+    r_ok = srv.submit(m, Server.MODEL, 3)
+    srv.drain(30)
+    assert srv.poll(r_ok) is not None
+    # the predictor starts from zeta_old (qg3p_aux.F90:993-1020)
+    for name in ("zeta", "zeta_old"):
+        z = srv.model_get(m, name, g, 3)
+        z[g // 2, g // 2, 1] = np.nan
+        z = np.asfortranarray(z)
+        rc = load().pq_model_set(load().pq_srv_model(srv._h, m), qg.FIELDS[name],
+                                 z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), z.size)
+        assert rc == 0
+    r_nan = srv.submit(m, Server.MODEL, 12)
+    srv.drain(30)
+    with pytest.raises(PqError) as e:
+        srv.poll(r_nan)
+    assert e.value.code == PQ_ERR_NAN
+    srv.close()
