@@ -221,6 +221,19 @@ int pq_cabaret_extrapolator_dev(int g, int layers, double dt,
                                 const double *ui, const double *uj,
                                 double *zeta_fi, double *zeta_fj,
                                 double *work, void *stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  pq_cabaret_extrapolator_dev without a work argument:
+ * it allocates nothing, keeping the new face values in a fixed 64 MiB
+ * device array of the library's (one per GPU, held until the process
+ * ends), so 2 (g+3)(g+2) layers <= 8388608 doubles (e.g. g = 1025 with up
+ * to 3 layers); larger grids return PQ_ERR_ARG before any GPU work.  Every
+ * call on a GPU shares that array: calls that overlap on one GPU (on
+ * different streams or threads) race.                                     */
+int pq_cabaret_extrapolator_noalloc_dev(int g, int layers, double dt,
+                                        const double *zeta, const double *zeta_old,
+                                        const double *ui, const double *uj,
+                                        double *zeta_fi, double *zeta_fj, void *stream);
 int pq_cabaret_flux_dev(int g, int layers, const double *ui, const double *uj,
                         const double *zeta_fi, const double *zeta_fj,
                         double *flux_i, double *flux_j, void *stream);

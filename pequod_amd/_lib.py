@@ -74,6 +74,7 @@ SIGNATURES = {
     "pq_cabaret_cell_boundaries_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pq_cabaret_cell_boundaries_velocity_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp]),
     "pq_cabaret_extrapolator_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pq_cabaret_extrapolator_noalloc_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pq_cabaret_flux_dev": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pq_cabaret_corrector_dev": (_i, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "pq_rel_from_zeta_and_psi_dev": (_i, [_i, _i, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp]),

@@ -715,6 +715,40 @@ int extrapolator_launch(const Dims &d, double dt, const double *zeta, const doub
 
 }  // namespace
 
+// ---- work area of the entry point that allocates nothing
+// pq_cabaret_extrapolator_noalloc_dev keeps the new face values here: a
+// file-scope device array of kNoAllocDoubles doubles (64 MiB), which every
+// GPU the library runs on holds until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in):
+// it uses the device address that hipGetSymbolAddress returns, looked up
+// once per device with that device current.
+static constexpr size_t kNoAllocDoubles = size_t(1) << 23;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(double **out) {
+  int dev = 0;
+  PQ_HIP(hipGetDevice(&dev));
+  PQ_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "pq_cabaret_extrapolator_noalloc_dev: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      pq_set_error("pq_cabaret_extrapolator_noalloc_dev: hipGetSymbolAddress failed: %s",
+                   hipGetErrorString(e));
+      return PQ_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return PQ_OK;
+}
+}  // namespace
 
 // -------------------------------------------------------------------- C ABI
 extern "C" {
@@ -792,6 +826,24 @@ int pq_cabaret_extrapolator_dev(int g, int L, double dt, const double *zeta,
   return check_launch("pq_cabaret_extrapolator_dev");
 }
 
+int pq_cabaret_extrapolator_noalloc_dev(int g, int L, double dt, const double *zeta,
+                                        const double *zeta_old, const double *ui, const double *uj,
+                                        double *zfi, double *zfj, void *stream) {
+  CHECK_GL(g, L, "pq_cabaret_extrapolator_noalloc_dev");
+  PQ_CHECK_ARG(zeta && zeta_old && ui && uj && zfi && zfj,
+               "pq_cabaret_extrapolator_noalloc_dev: null pointer");
+  Dims d{g, L};
+  const size_t need = d.n_fi() + d.n_fj();
+  PQ_CHECK_ARG(need <= kNoAllocDoubles,
+               "pq_cabaret_extrapolator_noalloc_dev: g=%d, %d layers need %zu doubles of work area; "
+               "the library holds %zu (2 (g+3)(g+2) layers must not exceed it)",
+               g, L, need, kNoAllocDoubles);
+  double *tmp = nullptr;
+  int rc = noalloc_scratch(&tmp);
+  if (rc) return rc;
+  extrapolator_launch(d, dt, zeta, zeta_old, ui, uj, zfi, zfj, tmp, S(stream));
+  return check_launch("pq_cabaret_extrapolator_noalloc_dev");
+}
 
 int pq_cabaret_flux_dev(int g, int L, const double *ui, const double *uj, const double *zfi,
                         const double *zfj, double *flux_i, double *flux_j, void *stream) {
