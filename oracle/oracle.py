@@ -17,6 +17,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PQ_ORACLE_LIB points the checker at another build of pq_oracle.c: the
+# mutation tests load copies with planted mistakes this way.
+_LIB_OVERRIDE = os.environ.get("PQ_ORACLE_LIB")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
@@ -32,9 +35,12 @@ def build(force: bool = False) -> str:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = ctypes.CDLL(LIB_PATH)
+        if _LIB_OVERRIDE:
+            L = ctypes.CDLL(_LIB_OVERRIDE)
+        else:
+            if not os.path.exists(LIB_PATH):
+                build()
+            L = ctypes.CDLL(LIB_PATH)
         i, d, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
         sig = {
             "pqo_sint_create": (vp, [i]),
