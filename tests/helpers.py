@@ -37,3 +37,30 @@ def to_dev(a, dev, dtype=None):
 def from_dev(t):
     """torch (L, jj, ii) -> Fortran-ordered numpy (ii, jj, L)."""
     return np.asfortranarray(t.detach().double().cpu().numpy().transpose(2, 1, 0))
+
+
+def reference_case(g, member=0):
+    """Non-dimensional inputs of the reference-trajectory fixtures
+    (tests/golden/make_reference_fixtures.py): the 3-layer 10 km workload on
+    g x g nodes with the seeded k^-3 PV field of `member`."""
+    from pequod_amd import synth
+    c = synth.config("qg3_255")
+    c.n = g - 2
+    nd = c.nondim()
+    st = nd["strat"]
+    return dict(g=g, L=3, dt=nd["dt"], beta=nd["beta"], viscosity=nd["viscosity"],
+                visc_bot=nd["visc_bot"], alpha=nd["alpha"], strat=st, force_wind=c.wind(),
+                zeta0=c.q(member=member))
+
+
+def load_reference_fixture(name):
+    """A fixture of tests/golden and its regenerated inputs (checked against
+    the checksums stored with it)."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+    fx = dict(np.load(here))
+    inp = reference_case(int(fx["g"]), int(fx["member"]))
+    assert inp["dt"] == float(fx["in_dt"])
+    assert np.sum(np.abs(inp["zeta0"])) == float(fx["in_q_abs_sum"])
+    assert np.sum(np.abs(inp["force_wind"])) == float(fx["in_wind_abs_sum"])
+    return fx, inp
