@@ -25,15 +25,40 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
 
 
+# Builds of pq_oracle.c: the checker (statement-order rounding) and the two
+# builds the CPU baseline times (oracle/Makefile).
+BUILDS = {"check": "liboracle.so", "o3": "liboracle_o3.so", "fast": "liboracle_fast.so"}
+
+
 def build(force: bool = False) -> str:
-    """Compile liboracle.so from pq_oracle.c with the committed Makefile."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+    """Compile every build of pq_oracle.c with the committed Makefile."""
+    if force or not all(os.path.exists(os.path.join(HERE, f)) for f in BUILDS.values()):
+        args = ["make", "-s", "-C", HERE] + (["-B"] if force else []) + ["all"]
+        subprocess.check_call(args)
     return LIB_PATH
 
 
-def lib():
+_builds = {}
+
+
+def lib(build_name: str = "check"):
+    """The oracle library: the checker build, or one of the CPU-baseline
+    builds ("o3", "fast"), which are timed, never used to check."""
     global _lib
+    if build_name != "check":
+        if build_name not in _builds:
+            path = os.path.join(HERE, BUILDS[build_name])
+            if not os.path.exists(path):
+                build()
+            L = ctypes.CDLL(path)
+            i, vp = ctypes.c_int, ctypes.c_void_p
+            for name, (res, args) in {"pqo_inv_create": (vp, [i, i, _dp, _dp, _dp]),
+                                      "pqo_inv_destroy": (None, [vp]),
+                                      "pqo_inv_run": (i, [vp, _dp, i, i, _dp, i, i])}.items():
+                getattr(L, name).restype = res
+                getattr(L, name).argtypes = args
+            _builds[build_name] = L
+        return _builds[build_name]
     if _lib is None:
         if _LIB_OVERRIDE:
             L = ctypes.CDLL(_LIB_OVERRIDE)
@@ -203,11 +228,12 @@ def strat_l3(rd1, rd2, h1, h2, h3):
 class Inversion:
     """helmholtz_cache + pv_inversion (qg3p_aux.F90:152, 452)."""
 
-    def __init__(self, g: int, ss, m2z, z2m):
+    def __init__(self, g: int, ss, m2z, z2m, build_name: str = "check"):
         self.g = g
         self.ss, self.m2z, self.z2m = f64(ss), f64(m2z), f64(z2m)
         self.layers = len(self.ss)
-        self.h = lib().pqo_inv_create(g, self.layers, _p(self.ss), _p(self.m2z), _p(self.z2m))
+        self._lib = lib(build_name)
+        self.h = self._lib.pqo_inv_create(g, self.layers, _p(self.ss), _p(self.m2z), _p(self.z2m))
         if not self.h:
             raise ValueError("pqo_inv_create failed")
 
@@ -216,8 +242,8 @@ class Inversion:
         g, L = self.g, self.layers
         if psi is None:
             psi = np.zeros((g, g, L), order="F")
-        rc = lib().pqo_inv_run(self.h, _p(zeta), zeta.shape[0], zeta.shape[1],
-                               _p(psi), psi.shape[0], psi.shape[1])
+        rc = self._lib.pqo_inv_run(self.h, _p(zeta), zeta.shape[0], zeta.shape[1],
+                                   _p(psi), psi.shape[0], psi.shape[1])
         return psi, rc
 
     def run_view(self, zeta_ptr, ldz1, ldz2, psi_ptr, ldp1, ldp2):
@@ -236,7 +262,7 @@ class Inversion:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().pqo_inv_destroy(self.h)
+            self._lib.pqo_inv_destroy(self.h)
             self.h = None
 
 

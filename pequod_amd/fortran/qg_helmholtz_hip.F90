@@ -29,9 +29,13 @@ module qg_helmholtz_hip
   public :: helm_cache
   public :: allocate, deallocate
 
+  ! decomp and ss are the reference's public components that qg_aux reads
+  ! (correction_function_mass, qg3p_aux.F90:196-230); h is the library's
+  ! handle.
   type helm_cache
     type(c_ptr) :: h = c_null_ptr
     type(decomposition), pointer :: decomp => null()
+    real(kind = real_8), dimension(:), pointer :: ss => null()
   end type helm_cache
 
   interface allocate
@@ -60,6 +64,8 @@ contains
       call pabort("qg_helmholtz_hip: closed basin only", __FILE__, __LINE__)
     end if
     helm_c%decomp => decomp
+    allocate(helm_c%ss(decomp%layers))
+    helm_c%ss = ss
     rc = pq_helm_create(helm_c%h, int(decomp%ii, c_int), int(decomp%jj, c_int), &
       & int(decomp%layers, c_int), ss, PQ_PREC_FP64)
     if (rc /= PQ_OK) then
@@ -76,12 +82,17 @@ contains
     rc = pq_helm_destroy(helm_c%h)
     helm_c%h = c_null_ptr
     nullify(helm_c%decomp)
+    if (associated(helm_c%ss)) deallocate(helm_c%ss)
   end subroutine deallocate_helm_c
 
   ! On input zeta is the RHS, on output the solution (qg3p_helmholtz.F90:614).
+  ! zeta is assumed-shape as in the reference; it reaches the C ABI's
+  ! assumed-size dummy by sequence association, so a contiguous actual (the
+  ! cache's phi, the reference's only caller) is passed in place and a
+  ! non-contiguous one through the compiler's copy-in / copy-out.
   subroutine helm(helm_c, zeta)
     type(helm_cache), intent(inout) :: helm_c
-    real(kind = real_8), dimension(:, :, :), contiguous, target, intent(inout) :: zeta
+    real(kind = real_8), dimension(:, :, :), intent(inout) :: zeta
 
     integer(c_int) :: rc
 
