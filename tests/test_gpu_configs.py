@@ -1,0 +1,274 @@
+"""The workloads at their configured sizes on the GPU, against the
+reference's own trajectory (tests/golden/ref_*.npz, written by pequod's
+Fortran routines) and against the oracle:
+- 3-layer 255^2 interior (G = 257): 100 steps with the reference's adaptive
+  time step; dt = 1800 s is Courant number ~1.5 on the seeded field, so
+  the reference's default adapt_dt (target 0.1) sets the steps;
+- every field after 1, 2 and 10 steps on 17^2 x 3 (three boundary
+  conditions) and 300 adaptive dt on 65^2 x 3, as the reference gives them;
+- 3-layer 1023^2 interior in FP64: 50 adaptive steps against the oracle's
+  model (same dt at every step, state to round-off) and 1000 adaptive
+  steps against the oracle's committed dt trajectory (step-count parity);
+- the 64-member ensemble's size classes {255, 511, 767, 1023}^2 with 2..8
+  layers: FP32 inversions within 1e-6 of the oracle, and 200 graph-replayed
+  ensemble steps of FP64 models equal to standalone models bit for bit;
+- served members at 1023^2 x 3 and 2047^2 x 6 within 1e-6 of the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pequod_amd import qg, synth
+
+from helpers import from_dev, load_reference_fixture, rel_err, to_dev
+
+pytestmark = pytest.mark.gpu
+
+NODE = ("psi", "zeta", "rel", "zeta_old")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel_dev(a, b):
+    return max(np.max(np.abs(a[..., l] - b[..., l])) / max(np.max(np.abs(b[..., l])), 1e-300)
+               for l in range(b.shape[-1]))
+
+
+def gpu_model(inp, bc=1, adapt=False, precision="fp64"):
+    m = qg.Model(inp["g"], inp["strat"], inp["beta"], inp["force_wind"],
+                 viscosity=inp["viscosity"], visc_bot=inp["visc_bot"], dt=inp["dt"], bc_type=bc,
+                 alpha=inp["alpha"], adapt_dt=adapt, target_cfl=0.1, max_dt=48.0 * inp["dt"],
+                 max_dt_increase=1.5, precision=precision)
+    assert m.init(inp["zeta0"]) == 0
+    return m
+
+
+@pytest.mark.parametrize("bc", [1, 2, 3])
+def test_model_matches_reference_fields_17(dev, bc):
+    print(f"reference fields 17^2 bc {bc}", flush=True)
+    fx, inp = load_reference_fixture(f"ref_traj_17_bc{bc}.npz")
+    m = gpu_model(inp, bc)
+    done = 0
+    for s in (1, 2, 10):
+        if f"s{s}_psi" not in fx:
+            continue
+        m.step(s - done)
+        done = s
+        assert m.status() == 0
+        sc = m.scalars()
+        for k in ("dt", "tweight_a", "tweight_u", "model_time"):
+            assert sc[k] == pytest.approx(float(fx[f"s{s}_{k}"]), rel=1e-14), (s, k)
+        for name in qg.FIELDS:
+            key = f"s{s}_{name}"
+            if key not in fx:
+                continue
+            a, b = m.get(name), fx[key]
+            if name in NODE:  # owned nodes; the reference leaves halo cells unset
+                a, b = a[1:-1, 1:-1], b[1:-1, 1:-1]
+            assert rel_dev(a, b) < 1e-12, (s, name, rel_dev(a, b))
+    m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32s"])
+def test_c1_255_interior_100_adaptive_steps(dev, precision):
+    """The CPU-runnable workload at size on the GPU: 100 steps, dt by dt as
+    the reference chose them; FP64 fields to round-off, FP32 fields within
+    the FP32 state bound."""
+    print(f"C1 255^2 x 3, 100 adaptive steps, {precision}", flush=True)
+    fx, inp = load_reference_fixture("ref_traj_257_adapt.npz")
+    m = gpu_model(inp, adapt=True, precision=precision)
+    dts = []
+    for _ in range(int(fx["nsteps"])):
+        m.step(1)
+        dts.append(m.scalars()["dt"])
+    assert m.status() == 0
+    tol_dt, tol = (1e-11, 1e-10) if precision == "fp64" else (1e-5, 2e-5)
+    d_dt = float(np.max(np.abs(np.array(dts) / fx["dts"] - 1.0)))
+    devs = {name: rel_dev(m.get(name)[::16, ::16, :], fx[f"s100_{name}_sub"])
+            for name in ("psi", "zeta", "zeta_fi", "ui")}
+    print(f"  dt {d_dt:.2e}, state {devs}", flush=True)
+    assert d_dt < tol_dt
+    assert all(d < tol for n, d in devs.items() if n in ("psi", "zeta")), devs
+    # faces difference the state: a few digits of cancellation
+    assert all(d < 50 * tol for d in devs.values()), devs
+    m.close()
+
+
+def test_dt_sequence_matches_reference_65(dev):
+    fx, inp = load_reference_fixture("ref_dt_65_adapt.npz")
+    m = gpu_model(inp, adapt=True)
+    dts = []
+    for _ in range(int(fx["nsteps"])):
+        m.step(1)
+        dts.append(m.scalars()["dt"])
+    np.testing.assert_allclose(dts, fx["dts"], rtol=1e-11)
+    assert m.scalars()["model_time"] == pytest.approx(float(fx["model_time"]), rel=1e-12)
+    m.close()
+
+
+def _c2():
+    c = synth.config("qg3_1023")
+    nd = c.nondim()
+    inp = dict(g=c.g, dt=nd["dt"], beta=nd["beta"], viscosity=nd["viscosity"],
+               visc_bot=nd["visc_bot"], alpha=nd["alpha"], strat=nd["strat"],
+               force_wind=c.wind(), zeta0=c.q())
+    return c, inp
+
+
+def test_c2_1023_fp64_adaptive_steps_vs_oracle(dev):
+    """50 adaptive steps of the 1023^2 x 3 FP64 drop-in against the oracle's
+    model: the same dt at every step and the state to round-off."""
+    print("C2 1023^2 x 3 FP64, 50 adaptive steps vs oracle", flush=True)
+    c, inp = _c2()
+    st = inp["strat"]
+    om = O.Model(c.g, st.s1, st.s2, st.ss, st.modes_to_z, st.z_to_modes, inp["beta"],
+                 inp["force_wind"], viscosity=inp["viscosity"], visc_bot=inp["visc_bot"],
+                 dt=inp["dt"], bc_type=1, alpha=inp["alpha"], adapt_dt=True, target_cfl=0.1,
+                 max_dt=48.0 * inp["dt"], max_dt_increase=1.5)
+    assert om.init(inp["zeta0"]) == 0
+    m = gpu_model(inp, adapt=True)
+    d_dt = 0.0
+    for _ in range(50):
+        om.step(1)
+        m.step(1)
+        d_dt = max(d_dt, abs(m.scalars()["dt"] / om.scalars()["dt"] - 1.0))
+    assert m.status() == 0
+    dp = rel_dev(m.get("psi")[1:-1, 1:-1], om.a["psi"][1:-1, 1:-1])
+    dz = rel_dev(m.get("zeta")[1:-1, 1:-1], om.a["zeta"][1:-1, 1:-1])
+    print(f"  dt {d_dt:.2e}, psi {dp:.2e}, zeta {dz:.2e}", flush=True)
+    assert d_dt < 1e-12 and dp < 1e-10 and dz < 1e-10
+    m.close()
+
+
+def test_c2_1023_fp64_1000_steps_step_count(dev):
+    """1000 adaptive steps of the 1023^2 x 3 FP64 model: every dt as the
+    oracle's committed trajectory has it, so both reach any model time in
+    the same number of steps."""
+    print("C2 1023^2 x 3 FP64, 1000 adaptive steps", flush=True)
+    fx = dict(np.load(os.path.join(GOLDEN, "oracle_dt_1023_adapt.npz")))
+    c, inp = _c2()
+    assert inp["dt"] == float(fx["in_dt"])
+    assert np.sum(np.abs(inp["zeta0"])) == float(fx["in_q_abs_sum"])
+    m = gpu_model(inp, adapt=True)
+    dts = []
+    for _ in range(int(fx["nsteps"])):
+        m.step(1)
+        dts.append(m.scalars()["dt"])
+    assert m.status() == 0
+    dts = np.array(dts)
+    d = float(np.max(np.abs(dts / fx["dts"] - 1.0)))
+    print(f"  max dt deviation {d:.2e}, model time {m.scalars()['model_time']:.6g}", flush=True)
+    assert d < 1e-9
+    t_ref = np.cumsum(fx["dts"])
+    t_gpu = np.cumsum(dts)
+    for target in t_ref[[99, 499, 998]]:
+        assert np.searchsorted(t_gpu, target * (1 - 1e-12)) == np.searchsorted(t_ref, target * (1 - 1e-12))
+    psi = m.get("psi")[1:-1, 1:-1, :]
+    assert rel_dev(psi[::64, ::64, :], fx["psi_sub"]) < 1e-8
+    m.close()
+
+
+def _class_members():
+    """The first member of each size class of the 64-member ensemble."""
+    by_n = {}
+    for k in range(64):
+        c = synth.ensemble_member(k)
+        by_n.setdefault(c.n, c)
+    assert sorted(by_n) == [255, 511, 767, 1023]
+    return [by_n[n] for n in sorted(by_n)]
+
+
+def test_c4_ensemble_size_classes_vs_oracle(dev):
+    import torch
+    from pequod_amd.ensemble import Ensemble
+    print("C4 ensemble size classes vs oracle", flush=True)
+    cfgs = _class_members()
+    ens = Ensemble("fp32auto")
+    zs, ps, refs = [], [], []
+    for c in cfgs:
+        st = c.stratification()
+        q = c.q(member=c.extra["member"])
+        refs.append(O.Inversion(c.g, st.ss, st.modes_to_z, st.z_to_modes).run(q)[0])
+        zs.append(to_dev(q, "cuda", torch.float32))
+        ps.append(torch.zeros_like(zs[-1]))
+        ens.add(st, zs[-1], ps[-1])
+    ens.finalize()
+    ens.capture(Ensemble.INVERT)
+    for rep in range(2):
+        for p in ps:
+            p.zero_()
+        ens.invert()
+        ens.synchronize()
+        assert ens.status() == 0
+        for c, p, ref in zip(cfgs, ps, refs):
+            err = rel_err(from_dev(p), ref)
+            print(f"  n {c.n} layers {c.layers}: {err:.2e}", flush=True)
+            assert err < 1e-6, (c.n, c.layers, err)
+
+
+def _cfl_dt(c, q, cfl=0.3):
+    nd = c.nondim()
+    m = qg.Model(c.g, nd["strat"], nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                 visc_bot=nd["visc_bot"], dt=nd["dt"], bc_type=1, alpha=nd["alpha"])
+    m.init(q)
+    psi = m.get("psi")[1:-1, 1:-1, :]
+    m.close()
+    umax = max(np.max(np.abs(np.diff(psi, axis=0))), np.max(np.abs(np.diff(psi, axis=1))))
+    return min(nd["dt"], cfl / umax)
+
+
+def test_c4_ensemble_200_graph_steps_match_standalone(dev):
+    """One FP64 CABARET model per size class (2..8 layers), 200 ensemble
+    steps replayed from a captured graph: every member's state equals its
+    standalone model's after 200 single steps, bit for bit."""
+    from pequod_amd.ensemble import Ensemble
+    print("C4 ensemble 200 graph-replayed steps", flush=True)
+    cfgs = _class_members()
+    mods, refs = [], []
+    for c in cfgs:
+        nd = c.nondim()
+        q = c.q(member=c.extra["member"])
+        dt = _cfl_dt(c, q)
+        for lst in (mods, refs):
+            m = qg.Model(c.g, nd["strat"], nd["beta"], c.wind(), viscosity=nd["viscosity"],
+                         visc_bot=nd["visc_bot"], dt=dt, bc_type=1, alpha=nd["alpha"])
+            m.init(q)
+            lst.append((m, nd["strat"]))
+    ens = Ensemble("fp64")
+    for m, st in mods:
+        ens.add_model(m, st)
+    ens.finalize(tiles=None)
+    ens.capture(Ensemble.STEP)
+    ens.step(200)
+    ens.synchronize()
+    for (m, _), (r, _), c in zip(mods, refs, cfgs):
+        r.step(200)
+        assert r.status() == 0
+        for name in ("psi", "zeta", "zeta_fi", "ui"):
+            np.testing.assert_array_equal(m.get(name), r.get(name), err_msg=f"{c.n} {name}")
+        assert m.scalars()["model_time"] == r.scalars()["model_time"]
+    for m, _ in mods + refs:
+        m.close()
+
+
+@pytest.mark.parametrize("name", ["qg3_1023", "qg6_2047"])
+def test_c5_served_member_vs_oracle(dev, name):
+    import torch
+    from pequod_amd.serve import Server
+    print(f"C5 served member {name}", flush=True)
+    c = synth.config(name)
+    st = c.stratification()
+    q = c.q()
+    ref = O.Inversion(c.g, st.ss, st.modes_to_z, st.z_to_modes).run(q)[0]
+    srv = Server("fp32", cap_bytes=200e9, max_batch=4)
+    zt = to_dev(q, "cuda", torch.float32)
+    pt = torch.zeros_like(zt)
+    mid = srv.open_field(st, zt, pt)
+    r = srv.submit(mid, Server.FIELD)
+    srv.drain(120)
+    assert srv.poll(r) is not None
+    err = rel_err(from_dev(pt), ref)
+    print(f"  {c.g}^2 x {c.layers}: {err:.2e}", flush=True)
+    assert err < 1e-6
+    srv.close()
