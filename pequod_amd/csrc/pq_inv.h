@@ -14,6 +14,10 @@ namespace pq {
 //                         odd p = 2h + 1 (closed basin); the channel keeps
 //                         inv_corr_a of mode m >= 1 at tab + 4 m (2 x 2,
 //                         column-major)
+//   tri[L * N * 4]: the tridiagonal column pass's constants of mode m and
+//                   column p at (m N + p) * 4: r, ln r, 1 / (r - 1/r), r^N,
+//                   with r + 1/r = 2 + mu, mu = 4 sin^2(p pi / 2N) + ss_m,
+//                   0 < r < 1 (p = 0, an all-zero column: r = 1/2)
 struct CoefLayout {
   int L, N;
   __host__ __device__ int D() const { return 0; }
@@ -23,10 +27,29 @@ struct CoefLayout {
   __host__ __device__ int b() const { return 2 * L * L + 2 * L; }
   __host__ __device__ int ssN() const { return 2 * L * L + 3 * L; }
   __host__ __device__ int tab() const { return 2 * L * L + 4 * L; }
-  __host__ __device__ size_t size() const {
+  __host__ __device__ size_t tri() const {
     return (size_t)tab() + (size_t)(L > 1 ? L - 1 : 0) * (N / 2) * 3;
   }
+  __host__ __device__ size_t size() const { return tri() + (size_t)L * N * 4; }
 };
+
+// Tridiagonal column pass: each thread solves E consecutive rows j of one
+// column for every mode, so a column takes N / E threads.  E = 8 (4 above
+// 4 layers, for registers) for N = 2^a, 12 (6) for N = 3 * 2^a, and never
+// more than 512 threads per column.
+__host__ __device__ constexpr int tri_e(int N, int LT) {
+  return N % 3 == 0 ? (LT <= 4 ? 12 : 6) : (LT <= 4 || N >= 4096 ? 8 : 4);
+}
+// Template layer bound of the tridiagonal column pass for L layers.
+__host__ __device__ constexpr int tri_lt(int L) { return L <= 4 ? 4 : 8; }
+// Threads per workgroup and columns per workgroup of the tridiagonal pass.
+__host__ __device__ constexpr int tri_tpc(int N, int L) { return N / tri_e(N, tri_lt(L)); }
+// (a power of two dividing N: the psiC partials are kept per block of columns)
+__host__ __device__ constexpr int tri_cols(int N, int L) {
+  return tri_tpc(N, L) >= 256 ? 1
+                              : (256 / tri_tpc(N, L) < (N & -N) ? 256 / tri_tpc(N, L) : (N & -N));
+}
+__host__ __device__ constexpr int tri_threads(int N, int L) { return tri_cols(N, L) * tri_tpc(N, L); }
 
 // Member arrays of one launch (passed by value).  Members may have fewer
 // layers than the handle (ragged batches): member b has lay[b] layers, its
@@ -87,6 +110,7 @@ struct InvArgs {
   int per;           // 0 closed basin (N rows of j), 1 periodic channel (2N rows)
   int twm;           // length of the twiddle table tw64
   int R, lgR;        // rows per tile of the row passes (scratch block height)
+  int tri;           // closed basin: 1 tridiagonal column pass, 0 sine transforms along j
 };
 
 // Static properties of a transform engine (per precision and length).

@@ -421,6 +421,218 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
   }
 }
 
+// ------------------------------------------------------------ K2 tridiagonal
+// The closed basin's column pass as the reference's own method solves it:
+// a tridiagonal system along j per column p and mode m (the Thomas sweep
+// of h1d / ge_local, qg3p_helmholtz.F90:1028-1170), in a form that runs in
+// parallel along j.  With x_l(p, j) the row pass's unnormalised sine
+// transform, the mode's right-hand side is
+//   z_m(j) = (1 / 2N) sum_l D(m,l) x_l(p, j)                (z(0) = 0),
+// and y_m solves y(j-1) - (2 + mu) y(j) + y(j+1) = z(j), y(0) = y(N) = 0,
+// mu = 4 sin^2(p pi / 2N) + ss_m (the sine transforms along j of k_cols
+// give the same y).  With r + 1/r = 2 + mu, 0 < r < 1, the free-space
+// Green's function r^|k| / (r - 1/r) gives the particular solution
+//   yp(j) = (F(j) + B(j) - z(j)) / (r - 1/r),
+//   F(j) = r F(j-1) + z(j),  B(j) = r B(j+1) + z(j),
+// two first-order recurrences damped by r, and the walls add
+// alpha r^j + beta r^(N-j) with yp(0) + alpha + beta r^N = 0 and
+// yp(N) + alpha r^N + beta = 0.  Each thread holds E consecutive rows of
+// one column in registers; the recurrences run locally, then as an affine
+// scan across the column's threads (shuffles inside a wavefront, LDS
+// across wavefronts).  FP64 arithmetic throughout; the scratch keeps the
+// handle's element type.  The pass also forms the psiC partials
+// sum_p shat_p sum_j y_m(p, j) and writes x_l = sum_m M(l,m) y_m back in
+// place.  Grid: per member N / CW workgroups of CW columns x N / E threads.
+template <int W>
+__device__ __forceinline__ double seg_shfl_up(double v, int d) {
+  return __shfl_up(v, d, W);
+}
+template <int W>
+__device__ __forceinline__ double seg_shfl_down(double v, int d) {
+  return __shfl_down(v, d, W);
+}
+
+// N is a template argument although every kernel unit has one length: the
+// units are linked into one library, and a kernel's symbol must differ per
+// length (otherwise the units' launches all resolve to one unit's kernel).
+template <typename Ts, int LT, int N>
+__global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
+  constexpr int E = tri_e(N, LT), TPC = N / E;
+  constexpr int W = TPC < 64 ? TPC : 64;  // lanes of a column inside one wavefront
+  constexpr int NWC = TPC / W;            // wavefronts per column
+  __shared__ double red[64];
+  __shared__ double s_tot[2][8];          // per wavefront: scan totals (forward, backward)
+  __shared__ double s_bc[2][256];         // per column: yp(0) and yp(N)
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int b = member_of(ptrs.cum_c, ptrs.nb, bid);
+  const int L = ptrs.lay[b], lgC = ptrs.lgc[b];
+  const int pblk = bid - ptrs.cum_c[b];
+  const int col = threadIdx.x / TPC, lc = threadIdx.x % TPC;
+  const int p = (pblk << lgC) + col;
+  const int j0 = lc * E;
+  const int lw = lc & (W - 1), wc = lc / W;  // lane and wavefront inside the column
+  const int wv = threadIdx.x >> 6;           // wavefront of the workgroup
+  const int R = a.R, lgR = a.lgR;
+  const CoefLayout CL{L, N};
+  const double *coef = ptrs.coef[b];
+  Ts *scr = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride);
+  const size_t cbase = (size_t)p * R;
+
+  // the thread's rows of layer l: contiguous when they lie in one block
+  // (E divides R), else element by element
+  const bool contig = E <= R && (R % E) == 0;
+  auto elem = [&](int l, int j) -> size_t {
+    return (size_t)l * N * N + (size_t)(j >> lgR) * N * R + cbase + (j & (R - 1));
+  };
+
+  // right-hand sides of the modes
+  double z[LT][E];
+#pragma unroll
+  for (int m = 0; m < LT; ++m)
+#pragma unroll
+    for (int e = 0; e < E; ++e) z[m][e] = 0.0;
+  const double s2n = 1.0 / (2.0 * N);
+#pragma unroll
+  for (int l = 0; l < LT; ++l) {
+    if (l >= L) break;
+    double x[E];
+    if (contig) {
+      const Ts *src = scr + elem(l, j0);
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = double(src[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = double(scr[elem(l, j0 + e)]);
+    }
+#pragma unroll
+    for (int m = 0; m < LT; ++m) {
+      if (m >= L) break;
+      const double d = coef[CL.D() + m + L * l] * s2n;
+#pragma unroll
+      for (int e = 0; e < E; ++e) z[m][e] = fma(d, x[e], z[m][e]);
+    }
+  }
+
+  const double sp = a.shat[p];  // 0 for even p
+  double acc[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) acc[m] = 0.0;
+#pragma unroll
+  for (int m = 0; m < LT; ++m) {
+    if (m >= L) break;
+    const double *tc = coef + CL.tri() + ((size_t)m * N + p) * 4;
+    const double r = tc[0], lr = tc[1], cg = tc[2], rN = tc[3];
+    // forward recurrence inside the thread (zero carry-in)
+    double t[E];
+    double f = 0.0, bb = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      f = fma(r, f, z[m][e]);
+      t[e] = f;
+    }
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) bb = fma(r, bb, z[m][e]);
+    double rE = r;
+#pragma unroll
+    for (int e = 1; e < E; ++e) rE *= r;
+    // inclusive affine scans inside the column's lanes of this wavefront:
+    // forward (from lane 0 up) and backward (from the last lane down)
+    double vf = f, vb = bb, mul = rE;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+      const double uf = seg_shfl_up<W>(vf, d), ub = seg_shfl_down<W>(vb, d);
+      if (lw >= d) vf = fma(mul, uf, vf);
+      if (lw + d < W) vb = fma(mul, ub, vb);
+      mul *= mul;
+    }
+    // carries: F just before this thread's rows, B just after them
+    const double pf = seg_shfl_up<W>(vf, 1), pb = seg_shfl_down<W>(vb, 1);
+    double cf = lw == 0 ? 0.0 : pf, cb = lw == W - 1 ? 0.0 : pb;
+    if constexpr (NWC > 1) {
+      // across the column's wavefronts: totals through LDS, combined in order
+      if (lw == W - 1) s_tot[0][wv] = vf;
+      if (lw == 0) s_tot[1][wv] = vb;
+      __syncthreads();
+      const double rW = mul;  // r^(E W)
+      const int w0 = wv - wc;  // the column's first wavefront
+      double cwf = 0.0, cwb = 0.0;
+      for (int q = 0; q < wc; ++q) cwf = fma(rW, cwf, s_tot[0][w0 + q]);
+      for (int q = NWC - 1; q > wc; --q) cwb = fma(rW, cwb, s_tot[1][w0 + q]);
+      // the wavefront's carry reaches lane lw through r^(E lw) (forward) and
+      // r^(E (W - 1 - lw)) (backward)
+      cf = fma(exp(double(E * lw) * lr), cwf, cf);
+      cb = fma(exp(double(E * (W - 1 - lw)) * lr), cwb, cb);
+      __syncthreads();
+    }
+    // full recurrences and the particular solution at the walls
+    double rp = r;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      t[e] = fma(rp, cf, t[e]);  // F(j0 + e)
+      rp *= r;
+    }
+    double bj = cb;
+    double yp[E];
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+      bj = fma(r, bj, z[m][e]);  // B(j0 + e)
+      yp[e] = (t[e] + bj - z[m][e]) * cg;
+    }
+    if (lc == 0) s_bc[0][col] = yp[0];
+    if (lc == TPC - 1) s_bc[1][col] = r * t[E - 1] * cg;  // yp(N): F(N) = r F(N - 1)
+    __syncthreads();
+    const double y0 = s_bc[0][col], yN = s_bc[1][col];
+    __syncthreads();
+    const double w = 1.0 / (1.0 - rN * rN);
+    const double al = (rN * yN - y0) * w, be = (rN * y0 - yN) * w;
+    // y(j) = yp(j) + alpha r^j + beta r^(N - j)
+    double ru = exp(double(j0) * lr), rd = exp(double(N - j0) * lr);
+    const double ri = 1.0 / r;
+    double sum = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double y = fma(al, ru, fma(be, rd, yp[e]));
+      z[m][e] = (j0 + e == 0) ? 0.0 : y;  // Dirichlet row j = 0
+      sum += z[m][e];
+      ru *= r;
+      rd *= ri;
+    }
+    acc[m] = sp * sum;
+  }
+
+  // psiC partials of this workgroup's columns (fixed-order FP64 sum)
+  if (a.mode == 0) {
+    block_sum(acc, L, red);
+    if (threadIdx.x == 0) {
+      double *part = a.part + (size_t)b * a.part_stride + (size_t)pblk * L;
+      for (int m = 0; m < L; ++m) part[m] = acc[m];
+    }
+  }
+  // back to layers, in place
+#pragma unroll
+  for (int l = 0; l < LT; ++l) {
+    if (l >= L) break;
+    double o[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) o[e] = 0.0;
+#pragma unroll
+    for (int m = 0; m < LT; ++m) {
+      if (m >= L) break;
+      const double mm = coef[CL.M() + l + L * m];
+#pragma unroll
+      for (int e = 0; e < E; ++e) o[e] = fma(mm, z[m][e], o[e]);
+    }
+    if (contig) {
+      Ts *dst = scr + elem(l, j0);
+#pragma unroll
+      for (int e = 0; e < E; ++e) dst[e] = Ts(o[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) scr[elem(l, j0 + e)] = Ts(o[e]);
+    }
+  }
+}
+
 // ------------------------------------------------------------ K2 channel
 // The channel's column pass: as k_cols, with the periodic transform of
 // length 2N along j (CE, a PerRfft engine) in place of the sine transform,
@@ -778,6 +990,9 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
   } else if (pass == 1 && a.per) {
     lds_limit(k_cols_chan<CE, Ts>, lds);
     hipLaunchKernelGGL((k_cols_chan<CE, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
+  } else if (pass == 1 && a.tri) {
+    if (a.L <= 4) hipLaunchKernelGGL((k_cols_tri<Ts, 4, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt);
+    else hipLaunchKernelGGL((k_cols_tri<Ts, 8, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt);
   } else if (pass == 1) {
     lds_limit(k_cols<Eng, Ts>, lds);
     hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);

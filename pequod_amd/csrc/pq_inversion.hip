@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -201,6 +202,9 @@ struct pq_inv {
   double *ctab = nullptr;
   int twm = 0;  // twiddle table length
   int R = 0, C = 0;
+  // closed basin: 1 tridiagonal column pass (default), 0 sine transforms
+  // along j (PQ_COLUMN_PASS=fft at creation; kept for comparison)
+  int tri = 1;
   std::vector<int> lay;  // layers of each member slot (<= L)
   KernelSet ks;
   EngineInfo ei{};
@@ -349,6 +353,20 @@ static void compute_coefs(const pq_inv *h, int L, const double *ss, const double
       ia[3] = a11 / det;
     }
   }
+  // constants of the tridiagonal column pass (mode m, column p): r with
+  // r + 1/r = 2 + mu, mu = 4 sin^2(p pi / 2N) + ss_m, as ln r =
+  // -2 asinh(sqrt(mu) / 2); 1 / (r - 1/r) = 1 / (2 sinh(ln r)); r^N.  The
+  // column p = 0 is zero (r = 1/2 keeps its arithmetic finite).
+  for (int m = 0; m < L && !h->per; ++m)
+    for (int p = 0; p < N; ++p) {
+      const double mu = -lam[p] + ss[m];
+      const double lr = (p == 0 || !(mu > 0.0)) ? std::log(0.5) : -2.0 * std::asinh(std::sqrt(mu) / 2.0);
+      double *t = c + CL.tri() + ((size_t)m * N + p) * 4;
+      t[0] = std::exp(lr);
+      t[1] = lr;
+      t[2] = 1.0 / (2.0 * std::sinh(lr));
+      t[3] = std::exp((double)N * lr);
+    }
   for (int m = 1; m < L && !h->per; ++m)
     for (int hh = 0; hh < N / 2; ++hh) {
       const int p = 2 * hh + 1;
@@ -465,6 +483,10 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   }
   h->ei = h->ks.info(precision);
   h->part_cap = N;  // C >= 1
+  {
+    const char *cp = std::getenv("PQ_COLUMN_PASS");
+    h->tri = per ? 0 : !(cp && std::strcmp(cp, "fft") == 0);
+  }
   int R, C;
   default_tiles(h, R, C);
   if (!tiles_ok(h, R, C, true)) {
@@ -532,13 +554,18 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.twm = h->twm;
   a.R = h->R;
   a.lgR = ilog2i(h->R);
+  a.tri = h->tri;
   const int rows_threads = (h->R / 2) * e.tpf;
   const int col_groups = (h->C * L + 1) / 2;
   for (int pass = 0; pass < 3; ++pass) {
     if (!(pass_mask & (1 << pass))) continue;
     int blocks, threads;
     size_t lds;
-    if (pass == 1) {
+    if (pass == 1 && h->tri) {
+      blocks = pt.cum_c[nb];
+      threads = tri_threads(N, L);
+      lds = 0;
+    } else if (pass == 1) {
       blocks = pt.cum_c[nb];
       threads = col_groups * (h->per ? e.ctpf : e.tpf);
       lds = inv_lds_bytes(e, 1, col_groups, h->per);
@@ -596,7 +623,9 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
       pt.coef[b] = coef ? coef[b0 + b] : h->coef + h->coef_size * (size_t)(b0 + b);
       const int Lb = layers ? layers[b0 + b] : h->lay[b0 + b];
       int Cb = h->C;
-      if (Lb != h->L) {
+      if (h->tri) {
+        Cb = tri_cols(h->N, h->L);  // one column geometry for the launch
+      } else if (Lb != h->L) {
         Cb = 8;
         while (Cb > 1 && ((Cb * Lb + 1) / 2 > groups || h->N % Cb)) Cb /= 2;
       }

@@ -81,3 +81,26 @@ def test_fp32_precision_choice_follows_projection_condition():
     eight = [m for m in (synth.ensemble_member(k) for k in range(16)) if m.layers == 8][0]
     assert choice(eight.stratification()) == _lib.PREC["fp32"]
     assert choice(synth.steep_member(8, 2.5).stratification()) == _lib.PREC["fp32s"]
+
+
+def test_kernel_units_have_distinct_kernel_symbols():
+    """The inversion kernels are compiled once per transform length and
+    linked into one library: a kernel whose symbol does not encode the
+    length would be defined by every unit, and every unit's launch would
+    resolve to one unit's kernel (compiled for another length).  The kernel
+    symbols of any two units must therefore be disjoint."""
+    build = os.path.join(ROOT, "pequod_amd", "_build")
+    objs = sorted(f for f in os.listdir(build) if f.startswith("pq_inv_kernels_n")) \
+        if os.path.isdir(build) else []
+    if len(objs) < 2:
+        pytest.skip("kernel unit objects not built")
+    seen = {}
+    for o in objs:
+        out = subprocess.run(["nm", os.path.join(build, o)], capture_output=True, text=True,
+                             check=True).stdout
+        # kernel handles are the unit's weak objects ("V") in namespace pq
+        kernels = set(re.findall(r"\sV\s(_ZN2pq\w+)$", out, flags=re.M))
+        assert kernels, o
+        for k in kernels:
+            assert k not in seen, f"{k} defined by {seen.get(k)} and {o}"
+            seen[k] = o
