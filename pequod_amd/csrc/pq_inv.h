@@ -14,8 +14,9 @@ namespace pq {
 //                         odd p = 2h + 1 (closed basin); the channel keeps
 //                         inv_corr_a of mode m >= 1 at tab + 4 m (2 x 2,
 //                         column-major)
-//   tri[L * N * 4]: the tridiagonal column pass's constants of mode m and
-//                   column p at (m N + p) * 4: r, ln r, 1 / (r - 1/r), r^N,
+//   tri[L * N * 8]: the tridiagonal column pass's constants of mode m and
+//                   column p at (m N + p) * 8: r, 1/r, 1 / (r - 1/r), r^N,
+//                   1 / (1 - r^2N), r^E (E of the handle's launches), 0, 0,
 //                   with r + 1/r = 2 + mu, mu = 4 sin^2(p pi / 2N) + ss_m,
 //                   0 < r < 1 (p = 0, an all-zero column: r = 1/2)
 struct CoefLayout {
@@ -30,7 +31,7 @@ struct CoefLayout {
   __host__ __device__ size_t tri() const {
     return (size_t)tab() + (size_t)(L > 1 ? L - 1 : 0) * (N / 2) * 3;
   }
-  __host__ __device__ size_t size() const { return tri() + (size_t)L * N * 4; }
+  __host__ __device__ size_t size() const { return tri() + (size_t)L * N * 8; }
 };
 
 // Tridiagonal column pass: each thread solves E consecutive rows j of one
@@ -40,8 +41,8 @@ struct CoefLayout {
 __host__ __device__ constexpr int tri_e(int N, int LT) {
   return N % 3 == 0 ? (LT <= 4 ? 12 : 6) : (LT <= 4 || N >= 4096 ? 8 : 4);
 }
-// Template layer bound of the tridiagonal column pass for L layers.
-__host__ __device__ constexpr int tri_lt(int L) { return L <= 4 ? 4 : 8; }
+// Layer count the kernel is instantiated for: exact up to 4, else 8.
+__host__ __device__ constexpr int tri_lt(int L) { return L <= 4 ? L : 8; }
 // Threads per workgroup and columns per workgroup of the tridiagonal pass.
 __host__ __device__ constexpr int tri_tpc(int N, int L) { return N / tri_e(N, tri_lt(L)); }
 // (a power of two dividing N: the psiC partials are kept per block of columns)

@@ -443,6 +443,18 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
 // handle's element type.  The pass also forms the psiC partials
 // sum_p shat_p sum_j y_m(p, j) and writes x_l = sum_m M(l,m) y_m back in
 // place.  Grid: per member N / CW workgroups of CW columns x N / E threads.
+// x^k for 0 <= k < KMAX by squaring (a fixed number of steps, so that
+// lanes with different k stay converged).
+template <int KMAX>
+__device__ __forceinline__ double powu(double x, int k) {
+  double r = 1.0;
+#pragma unroll
+  for (int bit = 1; bit < KMAX; bit <<= 1) {
+    if (k & bit) r *= x;
+    x *= x;
+  }
+  return r;
+}
 template <int W>
 __device__ __forceinline__ double seg_shfl_up(double v, int d) {
   return __shfl_up(v, d, W);
@@ -520,9 +532,10 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
 #pragma unroll
   for (int m = 0; m < LT; ++m) {
     if (m >= L) break;
-    const double *tc = coef + CL.tri() + ((size_t)m * N + p) * 4;
-    const double r = tc[0], lr = tc[1], cg = tc[2], rN = tc[3];
-    // forward recurrence inside the thread (zero carry-in)
+    const double *tc = coef + CL.tri() + ((size_t)m * N + p) * 8;
+    const double r = tc[0], ri = tc[1], cg = tc[2], rN = tc[3], w = tc[4], rE = tc[5];
+    // forward recurrence inside the thread (zero carry-in); the backward
+    // one only for its value at the thread's first row
     double t[E];
     double f = 0.0, bb = 0.0;
 #pragma unroll
@@ -532,9 +545,6 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
     }
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) bb = fma(r, bb, z[m][e]);
-    double rE = r;
-#pragma unroll
-    for (int e = 1; e < E; ++e) rE *= r;
     // inclusive affine scans inside the column's lanes of this wavefront:
     // forward (from lane 0 up) and backward (from the last lane down)
     double vf = f, vb = bb, mul = rE;
@@ -560,38 +570,37 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
       for (int q = NWC - 1; q > wc; --q) cwb = fma(rW, cwb, s_tot[1][w0 + q]);
       // the wavefront's carry reaches lane lw through r^(E lw) (forward) and
       // r^(E (W - 1 - lw)) (backward)
-      cf = fma(exp(double(E * lw) * lr), cwf, cf);
-      cb = fma(exp(double(E * (W - 1 - lw)) * lr), cwb, cb);
+      cf = fma(powu<W>(rE, lw), cwf, cf);
+      cb = fma(powu<W>(rE, W - 1 - lw), cwb, cb);
       __syncthreads();
     }
-    // full recurrences and the particular solution at the walls
+    // full recurrences, then the particular solution in place of F
     double rp = r;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       t[e] = fma(rp, cf, t[e]);  // F(j0 + e)
       rp *= r;
     }
+    const double fl = t[E - 1];  // F(N - 1) on the column's last thread
     double bj = cb;
-    double yp[E];
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
       bj = fma(r, bj, z[m][e]);  // B(j0 + e)
-      yp[e] = (t[e] + bj - z[m][e]) * cg;
+      t[e] = (t[e] + bj - z[m][e]) * cg;
     }
-    if (lc == 0) s_bc[0][col] = yp[0];
-    if (lc == TPC - 1) s_bc[1][col] = r * t[E - 1] * cg;  // yp(N): F(N) = r F(N - 1)
+    if (lc == 0) s_bc[0][col] = t[0];
+    if (lc == TPC - 1) s_bc[1][col] = r * fl * cg;  // yp(N): F(N) = r F(N - 1)
     __syncthreads();
     const double y0 = s_bc[0][col], yN = s_bc[1][col];
     __syncthreads();
-    const double w = 1.0 / (1.0 - rN * rN);
     const double al = (rN * yN - y0) * w, be = (rN * y0 - yN) * w;
-    // y(j) = yp(j) + alpha r^j + beta r^(N - j)
-    double ru = exp(double(j0) * lr), rd = exp(double(N - j0) * lr);
-    const double ri = 1.0 / r;
+    // y(j) = yp(j) + alpha r^j + beta r^(N - j): r^j0 = (r^E)^lc,
+    // r^(N - j0) = (r^E)^(TPC - lc)
+    double ru = powu<TPC + 1>(rE, lc), rd = powu<TPC + 1>(rE, TPC - lc);
     double sum = 0.0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const double y = fma(al, ru, fma(be, rd, yp[e]));
+      const double y = fma(al, ru, fma(be, rd, t[e]));
       z[m][e] = (j0 + e == 0) ? 0.0 : y;  // Dirichlet row j = 0
       sum += z[m][e];
       ru *= r;
@@ -991,8 +1000,13 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
     lds_limit(k_cols_chan<CE, Ts>, lds);
     hipLaunchKernelGGL((k_cols_chan<CE, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   } else if (pass == 1 && a.tri) {
-    if (a.L <= 4) hipLaunchKernelGGL((k_cols_tri<Ts, 4, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt);
-    else hipLaunchKernelGGL((k_cols_tri<Ts, 8, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt);
+    switch (a.L) {
+      case 1: hipLaunchKernelGGL((k_cols_tri<Ts, 1, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
+      case 2: hipLaunchKernelGGL((k_cols_tri<Ts, 2, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
+      case 3: hipLaunchKernelGGL((k_cols_tri<Ts, 3, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
+      case 4: hipLaunchKernelGGL((k_cols_tri<Ts, 4, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
+      default: hipLaunchKernelGGL((k_cols_tri<Ts, 8, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
+    }
   } else if (pass == 1) {
     lds_limit(k_cols<Eng, Ts>, lds);
     hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);

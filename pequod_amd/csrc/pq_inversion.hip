@@ -354,18 +354,23 @@ static void compute_coefs(const pq_inv *h, int L, const double *ss, const double
     }
   }
   // constants of the tridiagonal column pass (mode m, column p): r with
-  // r + 1/r = 2 + mu, mu = 4 sin^2(p pi / 2N) + ss_m, as ln r =
-  // -2 asinh(sqrt(mu) / 2); 1 / (r - 1/r) = 1 / (2 sinh(ln r)); r^N.  The
-  // column p = 0 is zero (r = 1/2 keeps its arithmetic finite).
+  // r + 1/r = 2 + mu, mu = 4 sin^2(p pi / 2N) + ss_m, from ln r =
+  // -2 asinh(sqrt(mu) / 2); 1/r; 1 / (r - 1/r) = 1 / (2 sinh(ln r)); r^N;
+  // 1 / (1 - r^2N); r^E for the E of the handle's launches.  The column
+  // p = 0 is zero (r = 1/2 keeps its arithmetic finite).
+  const int E = tri_e(N, tri_lt(h->L));
   for (int m = 0; m < L && !h->per; ++m)
     for (int p = 0; p < N; ++p) {
       const double mu = -lam[p] + ss[m];
       const double lr = (p == 0 || !(mu > 0.0)) ? std::log(0.5) : -2.0 * std::asinh(std::sqrt(mu) / 2.0);
-      double *t = c + CL.tri() + ((size_t)m * N + p) * 4;
+      double *t = c + CL.tri() + ((size_t)m * N + p) * 8;
+      const double rN = std::exp((double)N * lr);
       t[0] = std::exp(lr);
-      t[1] = lr;
+      t[1] = std::exp(-lr);
       t[2] = 1.0 / (2.0 * std::sinh(lr));
-      t[3] = std::exp((double)N * lr);
+      t[3] = rN;
+      t[4] = 1.0 / (1.0 - rN * rN);
+      t[5] = std::exp((double)E * lr);
     }
   for (int m = 1; m < L && !h->per; ++m)
     for (int hh = 0; hh < N / 2; ++hh) {
