@@ -125,23 +125,6 @@ __global__ void k_beta(Dims d, double dt, double tw_a, const double *__restrict_
   res_beta[d.rs(i, j, l)] = rb;
 }
 
-// cabaret_predictor followed by cabaret_beta, one pass (the beta update of
-// data node (i, j) only reads that node's predicted value).
-template <typename TS>
-__global__ void k_predictor_beta(Dims d, double dt, double tw_a,
-                                 const double *__restrict__ beta,
-                                 R<TS> zeta_old,
-                                 R<TS> res,
-                                 R<TS> ui, W<TS> zeta,
-                                 W<TS> res_beta) {
-  PQ_IDX(3, d.g - 2, 3, d.g - 2);
-  double z = zeta_old[d.nd(i, j, l)] + 0.5 * dt * res[d.rs(i - 1, j - 1, l)];
-  const double rb = dt * (0.5 * (ui[d.fi(i + 1, j, l)] + ui[d.fi(i, j, l)]) * beta[l]);
-  z = z + (1.0 + tw_a) * rb - tw_a * res_beta[d.rs(i - 1, j - 1, l)];
-  zeta[d.nd(i, j, l)] = z;
-  res_beta[d.rs(i - 1, j - 1, l)] = rb;
-}
-
 // cabaret_forcing with WIND_FORCING (qg3p_aux.F90:1065-1164)
 template <typename TS>
 __global__ void k_forcing(Dims d, double dt, const double *__restrict__ fw, double visc,
@@ -408,7 +391,8 @@ __global__ void k_corrector_to(Dims d, double dt, R<TS> flux_i,
 // i-face (i, j) (i in 3..g+1, j in 3..g) and j-face (i, j) into fin / fjn,
 // and writes the fluxes of the cabaret_flux ranges with the new face values
 // (the boundary faces i or j = 2, g+2 keep theirs).  zeta_fi / zeta_fj are
-// only read here (k_post_faces copies fin / fjn back).
+// only read here (k_face_copy copies fin / fjn back); the model runs this
+// pass only in a final partial step.
 template <typename TS>
 __global__ void k_post_extrap_flux(Dims d, double dt, R<TS> zeta, R<TS> zeta_old, R<TS> ui,
                                    R<TS> uj, R<TS> zfi, R<TS> zfj, W<TS> fin, W<TS> fjn,
@@ -451,33 +435,6 @@ __global__ void k_post_extrap_flux(Dims d, double dt, R<TS> zeta, R<TS> zeta_old
     }
     flux_j[a] = uj[a] * f;
   }
-}
-
-// cabaret_corrector, zeta_old <- zeta and the face copy of one model step
-// in one pass: thread (i, j) over 1..g+2 corrects node (i, j) (interior
-// 3..g; res), writes the result to zeta and zeta_old, and copies the new
-// face values of i- and j-face (i, j) into zeta_fi / zeta_fj.
-template <typename TS>
-__global__ void k_post_correct(Dims d, double dt, R<TS> flux_i, R<TS> flux_j, W<TS> res,
-                               W<TS> zeta, W<TS> zeta_old, R<TS> fin, R<TS> fjn, W<TS> zfi,
-                               W<TS> zfj) {
-  PQ_IDX(1, d.g + 2, 1, d.g + 2);
-  const int g = d.g;
-  const size_t a = d.nd(i, j, l);
-  double v;
-  if (i >= 3 && i <= g && j >= 3 && j <= g) {
-    const double r = -(flux_i[d.fi(i + 1, j, l)] - flux_i[d.fi(i, j, l)] +
-                       flux_j[d.fj(i, j + 1, l)] - flux_j[d.fj(i, j, l)]);
-    res[d.rs(i - 1, j - 1, l)] = r;
-    v = zeta[a] + 0.5 * dt * r;
-  } else {
-    v = zeta[a];
-  }
-  const TS t = TS(v);
-  zeta_old[a] = double(t);
-  zeta[a] = double(t);
-  if (i >= 3 && i <= g + 1 && j >= 3 && j <= g) zfi[d.fi(i, j, l)] = fin[d.fi(i, j, l)];
-  if (i >= 3 && i <= g && j >= 3 && j <= g + 1) zfj[d.fj(i, j, l)] = fjn[d.fj(i, j, l)];
 }
 
 // rel_from_zeta_and_psi + apply_rel_bc (qg3p_aux.F90:738-915) over owned
@@ -623,6 +580,219 @@ __global__ void k_post_rel_cells(Dims d, const double *__restrict__ s1,
       uj[f] = (1.0 + tw) * ujm - tw * ujmo[f];
       ujmo[f] = ujm;
     }
+  }
+}
+
+// ---- the model's tiled step kernels
+// The model keeps zeta_old and the face values zeta_fi / zeta_fj in two
+// buffers each and a device word `par` that names the current one: the
+// corrector pass reads the current values around each tile (a one-node /
+// one-face halo) and writes the new ones into the other buffer, so that no
+// workgroup overwrites what a neighbouring one still reads.  k_step_rel_force
+// flips par (it reads none of these fields); the predictor and every pass
+// after k_step_rel_force read it on the device, so a captured step replays
+// correctly any number of times.
+template <typename TS> struct Pair {
+  TS *p[2];
+};
+
+// Tile of the step kernels: TX nodes along i (one per thread column) by TY
+// rows of j, 64 x 4 threads, layer blockIdx.z (remapped by pq_tile).
+constexpr int TX = 64, TYS = 8;
+static_assert(TX == BX && BX * BY == 256, "the step kernels take one thread column per node column");
+inline dim3 step_grid(int g, int L) { return dim3((g + 2 + TX - 1) / TX, (g + 2 + TYS - 1) / TYS, L); }
+
+// cabaret_predictor followed by cabaret_beta (qg3p_aux.F90:993-1061) in
+// one pass, with zeta_old read from the current buffer: the beta update of
+// data node (i, j) only reads that node's predicted value.
+template <typename TS>
+__global__ void k_step_predictor(Dims d, double dt, double tw_a, const double *__restrict__ beta,
+                                 Pair<TS> zo, const int *__restrict__ par, R<TS> res, R<TS> ui,
+                                 W<TS> zeta, W<TS> res_beta) {
+  PQ_IDX(3, d.g - 2, 3, d.g - 2);
+  const R<TS> zeta_old(zo.p[*par]);
+  double z = zeta_old[d.nd(i, j, l)] + 0.5 * dt * res[d.rs(i - 1, j - 1, l)];
+  const double rb = dt * (0.5 * (ui[d.fi(i + 1, j, l)] + ui[d.fi(i, j, l)]) * beta[l]);
+  z = z + (1.0 + tw_a) * rb - tw_a * res_beta[d.rs(i - 1, j - 1, l)];
+  zeta[d.nd(i, j, l)] = z;
+  res_beta[d.rs(i - 1, j - 1, l)] = rb;
+}
+
+// rel_from_zeta_and_psi + update_zeta_bc + cabaret_forcing +
+// cabaret_cell_boundaries_time_integrated of one model step (the
+// statements of k_post_rel_cells and k_forcing, in their order and
+// rounding) over a TX x TYS tile of nodes 1..g+2.  rel of the tile and a
+// one-node halo is formed in LDS (the halo recomputed from psi and zeta,
+// which this pass does not write), so the forcing's Laplacian reads no
+// rel from HBM.  The forced zeta (boundary nodes: update_zeta_bc's value,
+// halo ring: zeta) goes to zf; zeta itself keeps the predicted values.
+template <typename TS>
+__global__ void __launch_bounds__(256)
+k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict__ s2, int bc_type,
+                 double alpha, double tw, double dt, const double *__restrict__ fw, double visc,
+                 double visc_bot, R<TS> psi, R<TS> zeta, W<TS> rel, W<TS> ui, W<TS> uj, W<TS> uimo,
+                 W<TS> ujmo, W<TS> zf, int *__restrict__ par) {
+  constexpr int HX = TX + 2, HY = TYS + 2;
+  __shared__ double rs[HY][HX];
+  int bx, by, l;
+  pq_tile(bx, by, l);
+  const int g = d.g, L = d.L;
+  const int i0 = 1 + bx * TX, j0 = 1 + by * TYS;
+  const int tid = threadIdx.x + BX * threadIdx.y;
+  if (tid == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) *par = 1 - *par;
+  // rel of the owned nodes among the tile and its halo, rounded to TS
+  for (int s = tid; s < HX * HY; s += BX * BY) {
+    const int a = s % HX, b = s / HX;
+    const int i = i0 - 1 + a, j = j0 - 1 + b;
+    double r = 0.0;
+    if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1)
+      r = double(TS(rel_node(d, s1, s2, bc_type, alpha, zeta, psi, i, j, l)));
+    rs[b][a] = r;
+  }
+  __syncthreads();
+  const int i = i0 + (int)threadIdx.x;
+  if (i > g + 2) return;
+  for (int b = threadIdx.y; b < TYS; b += BY) {
+    const int j = j0 + b;
+    if (j > g + 2) break;
+    const size_t a = d.nd(i, j, l);
+    const int ca = threadIdx.x + 1, cb = b + 1;
+    if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1) {
+      const double r = rs[cb][ca];
+      rel[a] = r;
+      if (i == 2 || i == g + 1 || j == 2 || j == g + 1) {
+        zf[a] = zeta_bc_node(d, s1, s2, r, psi, i, j, l);
+      } else {
+        const double lap = -4.0 * rs[cb][ca] + rs[cb][ca - 1] + rs[cb][ca + 1] + rs[cb - 1][ca] +
+                           rs[cb + 1][ca];
+        const double f = fw[(size_t)(i - 2) + (size_t)g * (j - 2)];
+        double rv;
+        if (L == 1)
+          rv = dt * (f - visc_bot * r + visc * lap);
+        else if (l == 0)
+          rv = dt * (f + visc * lap);
+        else if (l < L - 1)
+          rv = dt * visc * lap;
+        else
+          rv = dt * (-visc_bot * r + visc * lap);
+        zf[a] = zeta[a] + rv;
+      }
+    } else {
+      if (L == 1) rel[a] = zeta[a];
+      zf[a] = zeta[a];
+    }
+    if (i >= 3 && i <= g + 1 && j >= 3 && j <= g + 1) {
+      const double pe = psi_edge(d, psi, i, j, l);
+      if (j <= g) {
+        const double uim = -psi_edge(d, psi, i, j + 1, l) + pe;
+        const size_t f = d.fi(i, j, l);
+        ui[f] = (1.0 + tw) * uim - tw * uimo[f];
+        uimo[f] = uim;
+      }
+      if (i <= g) {
+        const double ujm = +psi_edge(d, psi, i + 1, j, l) - pe;
+        const size_t f = d.fj(i, j, l);
+        uj[f] = (1.0 + tw) * ujm - tw * ujmo[f];
+        ujmo[f] = ujm;
+      }
+    }
+  }
+}
+
+// cabaret_extrapolator + cabaret_flux + cabaret_corrector + zeta_old <-
+// zeta + the face copy of one model step (the statements of
+// k_post_extrap_flux and k_post_correct, in their order and rounding) over
+// a TX x TYS tile of nodes 1..g+2.  The fluxes through the tile's faces
+// (TX + 1 i-faces per row, TYS + 1 j-faces per column) are formed in LDS
+// from the forced zeta zf, the current zeta_old and face values, and the
+// velocities; the corrected zeta goes to zeta and the other zeta_old
+// buffer, the tile's new face values (and the unchanged boundary faces the
+// extrapolator reads) to the other face buffers.
+template <typename TS>
+__global__ void __launch_bounds__(256)
+k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> uj, Pair<TS> zfi,
+                      Pair<TS> zfj, W<TS> res, W<TS> zeta, const int *__restrict__ par) {
+  __shared__ double fxi[TYS][TX + 1];
+  __shared__ double fxj[TYS + 1][TX];
+  int bx, by, l;
+  pq_tile(bx, by, l);
+  const int g = d.g;
+  const int i0 = 1 + bx * TX, j0 = 1 + by * TYS;
+  const int tid = threadIdx.x + BX * threadIdx.y;
+  const int pn = *par;  // flipped by k_step_rel_force: the buffers written here
+  const R<TS> zold(zo.p[1 - pn]), fic(zfi.p[1 - pn]), fjc(zfj.p[1 - pn]);
+  const W<TS> zon(zo.p[pn]), fin(zfi.p[pn]), fjn(zfj.p[pn]);
+  // i-faces i0 .. i0+TX of rows j0 .. j0+TYS-1
+  for (int s = tid; s < (TX + 1) * TYS; s += BX * BY) {
+    const int a = s % (TX + 1), b = s / (TX + 1);
+    const int i = i0 + a, j = j0 + b;
+    double fl = 0.0;
+    if (j >= 3 && j <= g && i >= 2 && i <= g + 2) {
+      const size_t f = d.fi(i, j, l);
+      const bool own = a < TX;
+      double v;
+      if (i >= 3 && i <= g + 1) {
+        const bool lo = !(i > 3), hi = !(i < g + 1);
+        const double u1 = ui[f], fi1 = fic[f];
+        const double um = lo ? 0.0 : ui[d.fi(i - 1, j, l)], fm = lo ? 0.0 : fic[d.fi(i - 1, j, l)];
+        const double up = hi ? 0.0 : ui[d.fi(i + 1, j, l)], fp = hi ? 0.0 : fic[d.fi(i + 1, j, l)];
+        const TS t = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, zf[d.nd(i - 1, j, l)],
+                                    zold[d.nd(i - 1, j, l)], zf[d.nd(i, j, l)], zold[d.nd(i, j, l)],
+                                    lo, hi));
+        v = double(t);
+      } else {
+        v = fic[f];
+      }
+      if (own) fin[f] = v;
+      fl = double(TS(ui[f] * v));
+    }
+    fxi[b][a] = fl;
+  }
+  // j-faces j0 .. j0+TYS of columns i0 .. i0+TX-1
+  for (int s = tid; s < TX * (TYS + 1); s += BX * BY) {
+    const int a = s % TX, b = s / TX;
+    const int i = i0 + a, j = j0 + b;
+    double fl = 0.0;
+    if (i >= 3 && i <= g && j >= 2 && j <= g + 2) {
+      const size_t f = d.fj(i, j, l);
+      const bool own = b < TYS;
+      double v;
+      if (j >= 3 && j <= g + 1) {
+        const bool lo = !(j > 3), hi = !(j < g + 1);
+        const double u1 = uj[f], fi1 = fjc[f];
+        const double um = lo ? 0.0 : uj[d.fj(i, j - 1, l)], fm = lo ? 0.0 : fjc[d.fj(i, j - 1, l)];
+        const double up = hi ? 0.0 : uj[d.fj(i, j + 1, l)], fp = hi ? 0.0 : fjc[d.fj(i, j + 1, l)];
+        const TS t = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, zf[d.nd(i, j - 1, l)],
+                                    zold[d.nd(i, j - 1, l)], zf[d.nd(i, j, l)], zold[d.nd(i, j, l)],
+                                    lo, hi));
+        v = double(t);
+      } else {
+        v = fjc[f];
+      }
+      if (own) fjn[f] = v;
+      fl = double(TS(uj[f] * v));
+    }
+    fxj[b][a] = fl;
+  }
+  __syncthreads();
+  const int i = i0 + (int)threadIdx.x;
+  if (i > g + 2) return;
+  for (int b = threadIdx.y; b < TYS; b += BY) {
+    const int j = j0 + b;
+    if (j > g + 2) break;
+    const size_t a = d.nd(i, j, l);
+    const int c = threadIdx.x;
+    double v;
+    if (i >= 3 && i <= g && j >= 3 && j <= g) {
+      const double r = -(fxi[b][c + 1] - fxi[b][c] + fxj[b + 1][c] - fxj[b][c]);
+      res[d.rs(i - 1, j - 1, l)] = r;
+      v = zf[a] + 0.5 * dt * r;
+    } else {
+      v = zf[a];
+    }
+    const TS t = TS(v);
+    zon[a] = double(t);
+    zeta[a] = double(t);
   }
 }
 
@@ -912,6 +1082,11 @@ struct pq_model {
   double *zfj = nullptr, *uj = nullptr, *ujmo = nullptr, *flux_j = nullptr;
   double *res = nullptr, *res_beta = nullptr, *fw = nullptr;
   double *fin = nullptr, *fjn = nullptr;
+  // the model's step: the second zeta_old buffer (fin / fjn are the second
+  // face buffers), the forced zeta of the step, and the device word naming
+  // the current buffers (0: zeta_old, zfi, zfj; 1: zeta_old_b, fin, fjn)
+  double *zeta_old_b = nullptr, *zf = nullptr;
+  int *par = nullptr;
   double *s1 = nullptr, *s2 = nullptr, *beta = nullptr;
   // centred (leapfrog) scheme: the second time level; lf is the level the
   // next step writes (1: psi/rel/zeta/zeta_old, 2: the *2 arrays)
@@ -928,6 +1103,9 @@ struct pq_model {
   double dt = 0, tw_a = 0.5, tw_u = 0.5, model_time = 0;
   double visc = 0, visc_bot = 0, alpha = 0;
   int bc_type = 1, adapt = 0;
+  // flux_i / flux_j are not written by the model's step (its fluxes stay in
+  // LDS): they are formed from the velocities and face values when read
+  bool flux_stale = false;
   double target_cfl = 0, max_dt = 0, max_dt_inc = 0;
   // time mean of psi (qg3.F90:672-681) and energy diagnostics
   int average = 0;
@@ -962,13 +1140,52 @@ static double **model_field(pq_model *m, int which, size_t *count) {
   }
 }
 
+// Index of the current zeta_old / face-value buffers (always 0 for the
+// centred scheme), read back once the model's stream is idle.
+static int model_par(pq_model *m, int *out) {
+  *out = 0;
+  if (!m->par) return PQ_OK;
+  PQ_HIP(hipMemcpyAsync(out, m->par, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  return PQ_OK;
+}
+
+// Makes buffer 0 current (copying buffer 1 into it when that one is), for
+// host code that addresses zeta_old, zeta_fi and zeta_fj directly.
+static int model_canon(pq_model *m) {
+  int p = 0;
+  int rc = model_par(m, &p);
+  if (rc || !p) return rc;
+  const Dims &d = m->d;
+  const hipStream_t s = m->stream;
+  PQ_HIP(hipMemcpyAsync(m->zeta_old, m->zeta_old_b, m->es * d.n_nd(), hipMemcpyDeviceToDevice, s));
+  PQ_HIP(hipMemcpyAsync(m->zfi, m->fin, m->es * d.n_fi(), hipMemcpyDeviceToDevice, s));
+  PQ_HIP(hipMemcpyAsync(m->zfj, m->fjn, m->es * d.n_fj(), hipMemcpyDeviceToDevice, s));
+  static const int zero = 0;
+  PQ_HIP(hipMemcpyAsync(m->par, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+  PQ_HIP(hipStreamSynchronize(s));
+  return PQ_OK;
+}
+
+// Both face buffers equal (after the host wrote buffer 0), so that the
+// faces the step never writes agree whichever buffer is current.
+static int model_mirror_faces(pq_model *m) {
+  const Dims &d = m->d;
+  PQ_HIP(hipMemcpyAsync(m->fin, m->zfi, m->es * d.n_fi(), hipMemcpyDeviceToDevice, m->stream));
+  PQ_HIP(hipMemcpyAsync(m->fjn, m->zfj, m->es * d.n_fj(), hipMemcpyDeviceToDevice, m->stream));
+  PQ_HIP(hipStreamSynchronize(m->stream));
+  return PQ_OK;
+}
+
 int pq_model_destroy(pq_model *m) {
   if (!m) return PQ_OK;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (double *p : {m->psi, m->rel, m->zeta, m->zeta_old, m->zfi, m->ui, m->uimo, m->flux_i,
                     m->zfj, m->uj, m->ujmo, m->flux_j, m->res, m->res_beta, m->fw, m->fin, m->fjn,
-                    m->s1, m->s2, m->beta, m->psi2, m->rel2, m->zeta2, m->zeta_old2})
+                    m->s1, m->s2, m->beta, m->psi2, m->rel2, m->zeta2, m->zeta_old2,
+                    m->zeta_old_b, m->zf})
     (void)hipFree(p);
+  (void)hipFree(m->par);
   (void)hipFree(m->psi_av);
   pq_diag_destroy(m->diag);
   (void)hipFree(m->maxu);
@@ -1015,7 +1232,8 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
     allocs.insert(allocs.end(), {{&m->zfi, d.n_fi()}, {&m->ui, d.n_fi()}, {&m->uimo, d.n_fi()},
                                  {&m->flux_i, d.n_fi()}, {&m->zfj, d.n_fj()}, {&m->uj, d.n_fj()},
                                  {&m->ujmo, d.n_fj()}, {&m->flux_j, d.n_fj()}, {&m->res, d.n_rs()},
-                                 {&m->res_beta, d.n_rs()}, {&m->fin, d.n_fi()}, {&m->fjn, d.n_fj()}});
+                                 {&m->res_beta, d.n_rs()}, {&m->fin, d.n_fi()}, {&m->fjn, d.n_fj()},
+                                 {&m->zeta_old_b, d.n_nd()}, {&m->zf, d.n_nd()}});
   else
     allocs.insert(allocs.end(), {{&m->psi2, d.n_nd()}, {&m->rel2, d.n_nd()}, {&m->zeta2, d.n_nd()},
                                  {&m->zeta_old2, d.n_nd()}});
@@ -1031,6 +1249,10 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
     m->bytes += a.n * sizeof(double);
   }
   if (hipMalloc((void **)&m->maxu, sizeof(unsigned long long)) != hipSuccess) return fail(PQ_ERR_NOMEM);
+  if (m->scheme == PQ_SCHEME_CABARET) {
+    if (hipMalloc((void **)&m->par, sizeof(int)) != hipSuccess) return fail(PQ_ERR_NOMEM);
+    if (hipMemset(m->par, 0, sizeof(int)) != hipSuccess) return fail(PQ_ERR_HIP);
+  }
   // [0] the maximum's bits, [1] the inversion status word read with it
   if (hipHostMalloc((void **)&m->maxu_host, 2 * sizeof(unsigned long long), 0) != hipSuccess)
     return fail(PQ_ERR_NOMEM);
@@ -1171,10 +1393,12 @@ template <typename TS> static int lf_init(pq_model *m) {
 // zeta boundary, zeta_old = zeta, model_time = dt / 2.
 template <typename TS> static int cabaret_init(pq_model *m) {
   const Dims &d = m->d;
-  for (double *p : {m->zfi, m->ui, m->uimo, m->flux_i})
+  for (double *p : {m->zfi, m->ui, m->uimo, m->flux_i, m->fin})
     PQ_HIP(hipMemsetAsync(p, 0, sizeof(TS) * d.n_fi(), m->stream));
-  for (double *p : {m->zfj, m->uj, m->ujmo, m->flux_j})
+  for (double *p : {m->zfj, m->uj, m->ujmo, m->flux_j, m->fjn})
     PQ_HIP(hipMemsetAsync(p, 0, sizeof(TS) * d.n_fj(), m->stream));
+  PQ_HIP(hipMemsetAsync(m->par, 0, sizeof(int), m->stream));
+  m->flux_stale = false;
   PQ_HIP(hipMemsetAsync(m->res, 0, sizeof(TS) * d.n_rs(), m->stream));
   PQ_HIP(hipMemsetAsync(m->res_beta, 0, sizeof(TS) * d.n_rs(), m->stream));
   int rc = lf_diagnose<TS>(m, level(m, 1));
@@ -1273,9 +1497,9 @@ template <typename TS> static int model_pre_t(pq_model *m, hipStream_t s) {
     if (rc) return rc;
   }
   const int g = m->g, L = m->L;
-  hipLaunchKernelGGL(k_predictor_beta<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d,
-                     m->dt, m->tw_a, m->beta, fld<TS>(m->zeta_old), fld<TS>(m->res),
-                     fld<TS>(m->ui), fld<TS>(m->zeta), fld<TS>(m->res_beta));
+  hipLaunchKernelGGL(k_step_predictor<TS>, grid_for(g - 2, g - 2, L), dim3(BX, BY), 0, s, m->d,
+                     m->dt, m->tw_a, m->beta, Pair<TS>{{fld<TS>(m->zeta_old), fld<TS>(m->zeta_old_b)}},
+                     m->par, fld<TS>(m->res), fld<TS>(m->ui), fld<TS>(m->zeta), fld<TS>(m->res_beta));
   return PQ_OK;
 }
 static int model_pre(pq_model *m, hipStream_t s) { return PQ_BY_STORAGE(m, model_pre_t, m, s); }
@@ -1291,6 +1515,29 @@ template <typename TS> static int model_post_t(pq_model *m, hipStream_t s, bool 
   TS *ui = fld<TS>(m->ui), *uj = fld<TS>(m->uj), *zfi = fld<TS>(m->zfi), *zfj = fld<TS>(m->zfj);
   TS *fin = fld<TS>(m->fin), *fjn = fld<TS>(m->fjn);
   TS *flux_i = fld<TS>(m->flux_i), *flux_j = fld<TS>(m->flux_j);
+  if (!last) {
+    // the tiled step: two passes, the fluxes in LDS
+    TS *zf = fld<TS>(m->zf);
+    hipLaunchKernelGGL(k_step_rel_force<TS>, step_grid(g, L), dim3(BX, BY), 0, s, d, m->s1, m->s2,
+                       m->bc_type, m->alpha, m->tw_u, dt, m->fw, m->visc, m->visc_bot,
+                       fld<TS>(m->psi), zeta, rel, ui, uj, fld<TS>(m->uimo), fld<TS>(m->ujmo), zf,
+                       m->par);
+    m->model_time += 0.5 * dt;
+    int rc = model_accumulate(m, m->psi, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_step_extrap_correct<TS>, step_grid(g, L), dim3(BX, BY), 0, s, d, dt, zf,
+                       Pair<TS>{{zeta_old, fld<TS>(m->zeta_old_b)}}, ui, uj, Pair<TS>{{zfi, fin}},
+                       Pair<TS>{{zfj, fjn}}, fld<TS>(m->res), zeta, m->par);
+    m->model_time += 0.5 * dt;
+    m->flux_stale = true;
+    return check_launch("pq_model_step");
+  }
+  // a final partial step: the stage kernels on buffer 0
+  {
+    int rc = model_canon(m);
+    if (rc) return rc;
+  }
+  m->flux_stale = false;
   hipLaunchKernelGGL(k_post_rel_cells<TS>, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, m->s1,
                      m->s2, m->bc_type, m->alpha, m->tw_u, fld<TS>(m->psi), zeta, rel, ui, uj,
                      fld<TS>(m->uimo), fld<TS>(m->ujmo));
@@ -1303,14 +1550,8 @@ template <typename TS> static int model_post_t(pq_model *m, hipStream_t s, bool 
                      m->visc, m->visc_bot, rel, zeta);
   hipLaunchKernelGGL(k_post_extrap_flux<TS>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, s, d, dt,
                      zeta, zeta_old, ui, uj, zfi, zfj, fin, fjn, flux_i, flux_j);
-  if (!last) {
-    hipLaunchKernelGGL(k_post_correct<TS>, grid_for(g + 2, g + 2, L), dim3(BX, BY), 0, s, d, dt,
-                       flux_i, flux_j, fld<TS>(m->res), zeta, zeta_old, fin, fjn, zfi, zfj);
-    m->model_time += 0.5 * dt;
-  } else {
-    hipLaunchKernelGGL(k_face_copy<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, fin, fjn,
-                       zfi, zfj);
-  }
+  hipLaunchKernelGGL(k_face_copy<TS>, grid_for(g - 1, g - 1, L), dim3(BX, BY), 0, s, d, fin, fjn,
+                     zfi, zfj);
   return check_launch("pq_model_step");
 }
 static int model_post(pq_model *m, hipStream_t s, bool last) {
@@ -1369,7 +1610,21 @@ size_t model_device_bytes(const pq_model *m) {
   return m->bytes + (m->psi_av ? sizeof(double) * m->d.n_rs() : 0) + inv_device_bytes(m->inv);
 }
 int model_scheme(const pq_model *m) { return m->scheme; }
+// flux_i / flux_j of the current face values (cabaret_flux's ranges): what
+// the reference's arrays hold after a step.
+template <typename TS> static int model_fluxes(pq_model *m, int p) {
+  const int g = m->g, L = m->L;
+  hipLaunchKernelGGL(k_flux<TS>, grid_for(g + 1, g + 1, L), dim3(BX, BY), 0, m->stream, m->d,
+                     fld<TS>(m->ui), fld<TS>(m->uj), fld<TS>(p ? m->fin : m->zfi),
+                     fld<TS>(p ? m->fjn : m->zfj), fld<TS>(m->flux_i), fld<TS>(m->flux_j));
+  m->flux_stale = false;
+  return check_launch("pq_model_get");
+}
 ModelView model_view(pq_model *m) {
+  // the view addresses buffer 0 of zeta_old and the face values; the file
+  // writers and readers using it run on an idle stream
+  if (model_canon(m) != PQ_OK) return ModelView{};
+  if (m->flux_stale) (void)PQ_BY_STORAGE(m, model_fluxes, m, 0);
   ModelView v{};
   v.g = m->g;
   v.L = m->L;
@@ -1424,11 +1679,12 @@ template <typename TS> static int pickup_t(pq_model *m, double dt) {
     m->model_time = dt / 2.0;
     m->tw_a = 0.5;
     m->tw_u = 0.5;
+    m->flux_stale = false;
   }
   int rc = check_launch("pickup");
   if (rc) return rc;
   PQ_HIP(hipStreamSynchronize(s));
-  return PQ_OK;
+  return m->scheme == PQ_SCHEME_CABARET ? model_mirror_faces(m) : PQ_OK;
 }
 int model_pickup(pq_model *m, double dt) { return PQ_BY_STORAGE(m, pickup_t, m, dt); }
 // the same two half-step additions as model_post, so times round alike
@@ -1447,7 +1703,18 @@ int pq_model_get(pq_model *m, int which, double *host, size_t count) {
   double **f = model_field(m, which, &n);
   PQ_CHECK_ARG(f != nullptr, "pq_model_get: unknown field %d", which);
   PQ_CHECK_ARG(count >= n, "pq_model_get: field %d needs %zu values, buffer holds %zu", which, n, count);
-  return field_download(m, host, *f, n, m->stream);
+  double *src = *f;
+  int p = 0;
+  int rc = model_par(m, &p);
+  if (rc) return rc;
+  if (p && which == 3) src = m->zeta_old_b;
+  if (p && which == 6) src = m->fin;
+  if (p && which == 7) src = m->fjn;
+  if ((which == 12 || which == 13) && m->flux_stale) {
+    rc = PQ_BY_STORAGE(m, pq::model_fluxes, m, p);
+    if (rc) return rc;
+  }
+  return field_download(m, host, src, n, m->stream);
 }
 
 int pq_model_set(pq_model *m, int which, const double *host, size_t count) {
@@ -1457,7 +1724,20 @@ int pq_model_set(pq_model *m, int which, const double *host, size_t count) {
   PQ_CHECK_ARG(f != nullptr, "pq_model_set: unknown field %d", which);
   PQ_CHECK_ARG(count >= n, "pq_model_set: field %d needs %zu values, buffer holds %zu", which, n, count);
   PQ_HIP(hipStreamSynchronize(m->stream));
-  return field_upload(m, *f, host, n, m->stream);
+  int rc = model_canon(m);
+  if (rc) return rc;
+  if (which == 12 || which == 13) {
+    // the other flux array takes its formed values first, so that it keeps
+    // them once flux_i / flux_j are no longer formed on read
+    if (m->flux_stale) {
+      rc = PQ_BY_STORAGE(m, pq::model_fluxes, m, 0);
+      if (rc) return rc;
+    }
+    m->flux_stale = false;
+  }
+  rc = field_upload(m, *f, host, n, m->stream);
+  if (rc || !(m->par && (which == 6 || which == 7))) return rc;
+  return model_mirror_faces(m);
 }
 
 int pq_model_status(pq_model *m) {

@@ -170,6 +170,7 @@ int pq_model_write_snapshot(pq_model *m, const char *dir, int index, double mode
                             double total_time_day, double l_scale, double t_scale, int supp) {
   PQ_CHECK_ARG(m != nullptr, "pq_model_write_snapshot: null model");
   const ModelView v = model_view(m);
+  if (v.g == 0) return PQ_ERR_HIP;  // model_view set the error
   const int g = v.g, L = v.L, G2 = g + 2;
   const bool lf = v.scheme == PQ_SCHEME_LEAPFROG;
   PQ_CHECK_ARG(!supp || !lf, "pq_model_write_snapshot: supplementary data are CABARET data");
@@ -257,6 +258,7 @@ int pq_model_write_mean(pq_model *m, const char *dir, double model_time_day, dou
                         double l_scale, double t_scale) {
   PQ_CHECK_ARG(m != nullptr, "pq_model_write_mean: null model");
   const ModelView v = model_view(m);
+  if (v.g == 0) return PQ_ERR_HIP;  // model_view set the error
   PQ_CHECK_ARG(v.psi_av && v.diag_time > 0.0, "pq_model_write_mean: no time-mean samples yet");
   const int g = v.g, L = v.L;
   PQ_HIP(hipStreamSynchronize(v.stream));
@@ -286,6 +288,7 @@ int pq_model_read_checkpoint(pq_model *m, const char *dir, double *model_time_da
                              double *total_time_day) {
   PQ_CHECK_ARG(m != nullptr, "pq_model_read_checkpoint: null model");
   const ModelView v = model_view(m);
+  if (v.g == 0) return PQ_ERR_HIP;  // model_view set the error
   const int g = v.g, L = v.L, G2 = g + 2;
   const bool lf = v.scheme == PQ_SCHEME_LEAPFROG;
   PQ_HIP(hipStreamSynchronize(v.stream));
