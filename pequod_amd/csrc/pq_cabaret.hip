@@ -621,10 +621,11 @@ __global__ void k_step_predictor(Dims d, double dt, double tw_a, const double *_
 // rel_from_zeta_and_psi + update_zeta_bc + cabaret_forcing +
 // cabaret_cell_boundaries_time_integrated of one model step (the
 // statements of k_post_rel_cells and k_forcing, in their order and
-// rounding) over a TX x TYS tile of nodes 1..g+2.  rel of the tile and a
-// one-node halo is formed in LDS (the halo recomputed from psi and zeta,
-// which this pass does not write), so the forcing's Laplacian reads no
-// rel from HBM.  The forced zeta (boundary nodes: update_zeta_bc's value,
+// rounding) over a TX x TYS tile of nodes 1..g+2.  psi and zeta of the tile
+// and a one-node halo are staged in LDS; rel of the owned nodes among them
+// is formed there (the halo's recomputed, as this pass writes neither psi
+// nor zeta), so the forcing's Laplacian and the cell-boundary velocities
+// read LDS only.  The forced zeta (boundary nodes: update_zeta_bc's value,
 // halo ring: zeta) goes to zf; zeta itself keeps the predicted values.
 template <typename TS>
 __global__ void __launch_bounds__(256)
@@ -633,6 +634,8 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
                  double visc_bot, R<TS> psi, R<TS> zeta, W<TS> rel, W<TS> ui, W<TS> uj, W<TS> uimo,
                  W<TS> ujmo, W<TS> zf, int *__restrict__ par) {
   constexpr int HX = TX + 2, HY = TYS + 2;
+  __shared__ TS s_psi[HY][HX];
+  __shared__ TS s_zeta[HY][HX];
   __shared__ double rs[HY][HX];
   int bx, by, l;
   pq_tile(bx, by, l);
@@ -640,23 +643,58 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
   const int i0 = 1 + bx * TX, j0 = 1 + by * TYS;
   const int tid = threadIdx.x + BX * threadIdx.y;
   if (tid == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) *par = 1 - *par;
-  // rel of the owned nodes among the tile and its halo, rounded to TS
+  // LDS (b, a) holds node (i0 - 1 + a, j0 - 1 + b)
+  for (int s = tid; s < HX * HY; s += BX * BY) {
+    const int a = s % HX, b = s / HX;
+    const int i = i0 - 1 + a, j = j0 - 1 + b;
+    TS pv = TS(0), zv = TS(0);
+    if (i >= 1 && i <= g + 2 && j >= 1 && j <= g + 2) {
+      const size_t k = d.nd(i, j, l);
+      pv = psi.p[k];
+      zv = zeta.p[k];
+    }
+    s_psi[b][a] = pv;
+    s_zeta[b][a] = zv;
+  }
+  __syncthreads();
+  // rel (rel_node's statements) of the owned nodes, rounded to TS
   for (int s = tid; s < HX * HY; s += BX * BY) {
     const int a = s % HX, b = s / HX;
     const int i = i0 - 1 + a, j = j0 - 1 + b;
     double r = 0.0;
-    if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1)
-      r = double(TS(rel_node(d, s1, s2, bc_type, alpha, zeta, psi, i, j, l)));
+    if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1) {
+      if (i == 2 || i == g + 1 || j == 2 || j == g + 1) {
+        r = rel_bc(d, bc_type, alpha, psi, i, j, l);
+      } else {
+        const double z = s_zeta[b][a], pa = s_psi[b][a];
+        if (L == 1)
+          r = z;
+        else if (l == 0)
+          r = z + s2[0] * (pa - psi[d.nd(i, j, 1)]);
+        else if (l < L - 1)
+          r = z + s1[l] * (pa - psi[d.nd(i, j, l - 1)]) + s2[l] * (pa - psi[d.nd(i, j, l + 1)]);
+        else
+          r = z + s1[L - 1] * (pa - psi[d.nd(i, j, L - 2)]);
+      }
+      r = double(TS(r));
+    }
     rs[b][a] = r;
   }
   __syncthreads();
   const int i = i0 + (int)threadIdx.x;
   if (i > g + 2) return;
+  const int ca = threadIdx.x + 1;
+  // psi at node (i + di, j + dj) of LDS row cb
+  auto P = [&](int cb, int di, int dj) { return double(s_psi[cb + dj][ca + di]); };
+  auto pedge = [&](int cb, int di, int dj) {  // psi_edge(i + di, j + dj)
+    return 0.25 * (P(cb, di, dj) + P(cb, di, dj - 1) + P(cb, di - 1, dj) + P(cb, di - 1, dj - 1));
+  };
   for (int b = threadIdx.y; b < TYS; b += BY) {
     const int j = j0 + b;
     if (j > g + 2) break;
     const size_t a = d.nd(i, j, l);
-    const int ca = threadIdx.x + 1, cb = b + 1;
+    const int cb = b + 1;
+    const double z = s_zeta[cb][ca];
     if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1) {
       const double r = rs[cb][ca];
       rel[a] = r;
@@ -675,22 +713,22 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
           rv = dt * visc * lap;
         else
           rv = dt * (-visc_bot * r + visc * lap);
-        zf[a] = zeta[a] + rv;
+        zf[a] = z + rv;
       }
     } else {
-      if (L == 1) rel[a] = zeta[a];
-      zf[a] = zeta[a];
+      if (L == 1) rel[a] = z;
+      zf[a] = z;
     }
     if (i >= 3 && i <= g + 1 && j >= 3 && j <= g + 1) {
-      const double pe = psi_edge(d, psi, i, j, l);
+      const double pe = pedge(cb, 0, 0);
       if (j <= g) {
-        const double uim = -psi_edge(d, psi, i, j + 1, l) + pe;
+        const double uim = -pedge(cb, 0, 1) + pe;
         const size_t f = d.fi(i, j, l);
         ui[f] = (1.0 + tw) * uim - tw * uimo[f];
         uimo[f] = uim;
       }
       if (i <= g) {
-        const double ujm = +psi_edge(d, psi, i + 1, j, l) - pe;
+        const double ujm = +pedge(cb, 1, 0) - pe;
         const size_t f = d.fj(i, j, l);
         uj[f] = (1.0 + tw) * ujm - tw * ujmo[f];
         ujmo[f] = ujm;
@@ -702,80 +740,125 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
 // cabaret_extrapolator + cabaret_flux + cabaret_corrector + zeta_old <-
 // zeta + the face copy of one model step (the statements of
 // k_post_extrap_flux and k_post_correct, in their order and rounding) over
-// a TX x TYS tile of nodes 1..g+2.  The fluxes through the tile's faces
-// (TX + 1 i-faces per row, TYS + 1 j-faces per column) are formed in LDS
-// from the forced zeta zf, the current zeta_old and face values, and the
-// velocities; the corrected zeta goes to zeta and the other zeta_old
+// a TX x TYS tile of nodes 1..g+2.  The forced zeta zf and the current
+// zeta_old of the tile and a one-node halo, and the velocities and current
+// face values the tile's faces read, are staged in LDS; the fluxes through
+// the tile's faces (TX + 1 i-faces per row, TYS + 1 j-faces per column)
+// are formed there.  The corrected zeta goes to zeta and the other zeta_old
 // buffer, the tile's new face values (and the unchanged boundary faces the
 // extrapolator reads) to the other face buffers.
 template <typename TS>
 __global__ void __launch_bounds__(256)
 k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> uj, Pair<TS> zfi,
                       Pair<TS> zfj, W<TS> res, W<TS> zeta, const int *__restrict__ par) {
-  __shared__ double fxi[TYS][TX + 1];
-  __shared__ double fxj[TYS + 1][TX];
+  constexpr int NX = TX + 2, NY = TYS + 2;  // nodes i0-1 .. i0+TX, j0-1 .. j0+TYS
+  constexpr int IX = TX + 3;                // i-faces i0-1 .. i0+TX+1 (rows j0 .. j0+TYS-1)
+  constexpr int JY = TYS + 3;               // j-faces j0-1 .. j0+TYS+1 (columns i0 .. i0+TX-1)
+  __shared__ TS s_zf[NY][NX], s_zo[NY][NX];
+  __shared__ TS s_ui[TYS][IX], s_fi[TYS][IX];
+  __shared__ TS s_uj[JY][TX], s_fj[JY][TX];
+  __shared__ TS fxi[TYS][TX + 1];  // fluxes (stored values: TS)
+  __shared__ TS fxj[TYS + 1][TX];
   int bx, by, l;
   pq_tile(bx, by, l);
   const int g = d.g;
   const int i0 = 1 + bx * TX, j0 = 1 + by * TYS;
   const int tid = threadIdx.x + BX * threadIdx.y;
   const int pn = *par;  // flipped by k_step_rel_force: the buffers written here
-  const R<TS> zold(zo.p[1 - pn]), fic(zfi.p[1 - pn]), fjc(zfj.p[1 - pn]);
+  const TS *zold = zo.p[1 - pn], *fic = zfi.p[1 - pn], *fjc = zfj.p[1 - pn];
   const W<TS> zon(zo.p[pn]), fin(zfi.p[pn]), fjn(zfj.p[pn]);
-  // i-faces i0 .. i0+TX of rows j0 .. j0+TYS-1
-  for (int s = tid; s < (TX + 1) * TYS; s += BX * BY) {
-    const int a = s % (TX + 1), b = s / (TX + 1);
+  for (int s = tid; s < NX * NY; s += BX * BY) {
+    const int a = s % NX, b = s / NX;
+    const int i = i0 - 1 + a, j = j0 - 1 + b;
+    TS x = TS(0), y = TS(0);
+    if (i >= 1 && i <= g + 2 && j >= 1 && j <= g + 2) {
+      const size_t k = d.nd(i, j, l);
+      x = zf.p[k];
+      y = zold[k];
+    }
+    s_zf[b][a] = x;
+    s_zo[b][a] = y;
+  }
+  for (int s = tid; s < IX * TYS; s += BX * BY) {
+    const int a = s % IX, b = s / IX;
+    const int i = i0 - 1 + a, j = j0 + b;
+    TS x = TS(0), y = TS(0);
+    if (i >= 1 && i <= g + 3 && j <= g + 2) {
+      const size_t k = d.fi(i, j, l);
+      x = ui.p[k];
+      y = fic[k];
+    }
+    s_ui[b][a] = x;
+    s_fi[b][a] = y;
+  }
+  for (int s = tid; s < TX * JY; s += BX * BY) {
+    const int a = s % TX, b = s / TX;
+    const int i = i0 + a, j = j0 - 1 + b;
+    TS x = TS(0), y = TS(0);
+    if (i <= g + 2 && j >= 1 && j <= g + 3) {
+      const size_t k = d.fj(i, j, l);
+      x = uj.p[k];
+      y = fjc[k];
+    }
+    s_uj[b][a] = x;
+    s_fj[b][a] = y;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x;
+  // i-face i0 + a of row j0 + b: LDS face column a + 1, node columns a, a + 1
+  auto iface = [&](int a, int b) {
     const int i = i0 + a, j = j0 + b;
-    double fl = 0.0;
+    TS fl = TS(0);
     if (j >= 3 && j <= g && i >= 2 && i <= g + 2) {
-      const size_t f = d.fi(i, j, l);
-      const bool own = a < TX;
+      const int c = a + 1;
       double v;
       if (i >= 3 && i <= g + 1) {
         const bool lo = !(i > 3), hi = !(i < g + 1);
-        const double u1 = ui[f], fi1 = fic[f];
-        const double um = lo ? 0.0 : ui[d.fi(i - 1, j, l)], fm = lo ? 0.0 : fic[d.fi(i - 1, j, l)];
-        const double up = hi ? 0.0 : ui[d.fi(i + 1, j, l)], fp = hi ? 0.0 : fic[d.fi(i + 1, j, l)];
-        const TS t = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, zf[d.nd(i - 1, j, l)],
-                                    zold[d.nd(i - 1, j, l)], zf[d.nd(i, j, l)], zold[d.nd(i, j, l)],
-                                    lo, hi));
+        const double u1 = s_ui[b][c], fi1 = s_fi[b][c];
+        const double um = lo ? 0.0 : double(s_ui[b][c - 1]), fm = lo ? 0.0 : double(s_fi[b][c - 1]);
+        const double up = hi ? 0.0 : double(s_ui[b][c + 1]), fp = hi ? 0.0 : double(s_fi[b][c + 1]);
+        const TS t = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, s_zf[b + 1][a], s_zo[b + 1][a],
+                                    s_zf[b + 1][a + 1], s_zo[b + 1][a + 1], lo, hi));
         v = double(t);
       } else {
-        v = fic[f];
+        v = s_fi[b][c];
       }
-      if (own) fin[f] = v;
-      fl = double(TS(ui[f] * v));
+      if (a < TX) fin[d.fi(i, j, l)] = v;
+      fl = TS(double(s_ui[b][c]) * v);
     }
     fxi[b][a] = fl;
-  }
-  // j-faces j0 .. j0+TYS of columns i0 .. i0+TX-1
-  for (int s = tid; s < TX * (TYS + 1); s += BX * BY) {
-    const int a = s % TX, b = s / TX;
+  };
+  // j-face j0 + b of column i0 + a: LDS face row b + 1, node rows b, b + 1
+  auto jface = [&](int a, int b) {
     const int i = i0 + a, j = j0 + b;
-    double fl = 0.0;
+    TS fl = TS(0);
     if (i >= 3 && i <= g && j >= 2 && j <= g + 2) {
-      const size_t f = d.fj(i, j, l);
-      const bool own = b < TYS;
+      const int c = b + 1;
       double v;
       if (j >= 3 && j <= g + 1) {
         const bool lo = !(j > 3), hi = !(j < g + 1);
-        const double u1 = uj[f], fi1 = fjc[f];
-        const double um = lo ? 0.0 : uj[d.fj(i, j - 1, l)], fm = lo ? 0.0 : fjc[d.fj(i, j - 1, l)];
-        const double up = hi ? 0.0 : uj[d.fj(i, j + 1, l)], fp = hi ? 0.0 : fjc[d.fj(i, j + 1, l)];
-        const TS t = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, zf[d.nd(i, j - 1, l)],
-                                    zold[d.nd(i, j - 1, l)], zf[d.nd(i, j, l)], zold[d.nd(i, j, l)],
-                                    lo, hi));
+        const double u1 = s_uj[c][a], fi1 = s_fj[c][a];
+        const double um = lo ? 0.0 : double(s_uj[c - 1][a]), fm = lo ? 0.0 : double(s_fj[c - 1][a]);
+        const double up = hi ? 0.0 : double(s_uj[c + 1][a]), fp = hi ? 0.0 : double(s_fj[c + 1][a]);
+        const TS t = TS(extrap_face(dt, u1, fi1, um, fm, up, fp, s_zf[b][a + 1], s_zo[b][a + 1],
+                                    s_zf[b + 1][a + 1], s_zo[b + 1][a + 1], lo, hi));
         v = double(t);
       } else {
-        v = fjc[f];
+        v = s_fj[c][a];
       }
-      if (own) fjn[f] = v;
-      fl = double(TS(uj[f] * v));
+      if (b < TYS) fjn[d.fj(i, j, l)] = v;
+      fl = TS(double(s_uj[c][a]) * v);
     }
     fxj[b][a] = fl;
+  };
+  for (int b = threadIdx.y; b < TYS; b += BY) {
+    iface(tx, b);
+    jface(tx, b);
   }
+  if (tid < TYS) iface(TX, tid);                            // the tile's last i-face column
+  else if (tid >= BX && tid < 2 * BX) jface(tid - BX, TYS);  // and last j-face row
   __syncthreads();
-  const int i = i0 + (int)threadIdx.x;
+  const int i = i0 + tx;
   if (i > g + 2) return;
   for (int b = threadIdx.y; b < TYS; b += BY) {
     const int j = j0 + b;
@@ -784,11 +867,12 @@ k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> 
     const int c = threadIdx.x;
     double v;
     if (i >= 3 && i <= g && j >= 3 && j <= g) {
-      const double r = -(fxi[b][c + 1] - fxi[b][c] + fxj[b + 1][c] - fxj[b][c]);
+      const double r = -(double(fxi[b][c + 1]) - double(fxi[b][c]) + double(fxj[b + 1][c]) -
+                         double(fxj[b][c]));
       res[d.rs(i - 1, j - 1, l)] = r;
-      v = zf[a] + 0.5 * dt * r;
+      v = double(s_zf[b + 1][c + 1]) + 0.5 * dt * r;
     } else {
-      v = zf[a];
+      v = s_zf[b + 1][c + 1];
     }
     const TS t = TS(v);
     zon[a] = double(t);

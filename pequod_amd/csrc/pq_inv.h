@@ -10,8 +10,9 @@ namespace pq {
 //   D[L*L]   z_to_modes, D(m,l) at m + L*l
 //   M[L*L]   modes_to_z, M(l,m) at l + L*m
 //   ss[L], a[L], b[L], ssN[L] = (2N)^2 ss
-//   tab[(L-1) * N/2 * 3]: (rho, gamma, eta) for mode m >= 1 (0-based) and
-//                         odd p = 2h + 1 (closed basin); the channel keeps
+//   tab[(L-1) * N/2 * 4]: (rho, gamma, eta, r = e^rho) for mode m >= 1
+//                         (0-based) and odd p = 2h + 1 (closed basin); the
+//                         channel keeps
 //                         inv_corr_a of mode m >= 1 at tab + 4 m (2 x 2,
 //                         column-major)
 //   tri[L * N * 8]: the tridiagonal column pass's constants of mode m and
@@ -29,7 +30,7 @@ struct CoefLayout {
   __host__ __device__ int ssN() const { return 2 * L * L + 3 * L; }
   __host__ __device__ int tab() const { return 2 * L * L + 4 * L; }
   __host__ __device__ size_t tri() const {
-    return (size_t)tab() + (size_t)(L > 1 ? L - 1 : 0) * (N / 2) * 3;
+    return (size_t)tab() + (size_t)(L > 1 ? L - 1 : 0) * (N / 2) * 4;
   }
   __host__ __device__ size_t size() const { return tri() + (size_t)L * N * 8; }
 };

@@ -859,8 +859,8 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[r] = 0.0;
       for (int m = 1; m < L; ++m) {
-        const double *e = tab + ((size_t)(m - 1) * (N / 2) + h) * 3;
-        const double rho = e[0], gam = beta[m] * e[1], eta = e[2];
+        const double *e = tab + ((size_t)(m - 1) * (N / 2) + h) * 4;
+        const double rho = e[0], gam = beta[m] * e[1], eta = e[2], rr = e[3];
         // r^j for the tile's rows is below e^-40 (relative 4e-18) far from
         // both edges: then only the constant part remains
         const int jmin = min(j0, N - j0 - R + 1);
@@ -870,16 +870,31 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
             if (r < R) acc[r] += gam;
           continue;
         }
-        const double rr = exp(rho);
+        // r^j0 = (r^R)^jb and r^(N - j0 - R + 1) = r (r^R)^(NB - 1 - jb), by
+        // squaring (jb is the workgroup's, so the loops do not diverge)
+        double rR = rr;
+        for (int k = 0; k < lgR; ++k) rR *= rR;
+        double u = 1.0, v = rr;
+        {
+          double x = rR;
+          for (int k = jb; k; k >>= 1) {
+            if (k & 1) u *= x;
+            x *= x;
+          }
+          x = rR;
+          for (int k = NB - 1 - jb; k; k >>= 1) {
+            if (k & 1) v *= x;
+            x *= x;
+          }
+        }
         double hh[16];
-        double u = exp(double(j0) * rho);  // r^(j0 + r), going up
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
+        for (int r = 0; r < 16; ++r)  // r^(j0 + r), going up
           if (r < R) {
             hh[r] = u;
             u *= rr;
           }
-        double v = exp(double(N - j0 - R + 1) * rho);  // r^(N - j0 - r), going down
+        // r^(N - j0 - r), going down
 #pragma unroll
         for (int r = 15; r >= 0; --r)
           if (r < R) {

@@ -378,10 +378,11 @@ static void compute_coefs(const pq_inv *h, int L, const double *ss, const double
       const double mu = lam[p] - ss[m];
       const double rho = -2.0 * asinh(std::sqrt(-mu) / 2.0);  // ln r
       const double rN = std::exp((double)N * rho);
-      double *e = c + CL.tab() + ((size_t)(m - 1) * (N / 2) + hh) * 3;
+      double *e = c + CL.tab() + ((size_t)(m - 1) * (N / 2) + hh) * 4;
       e[0] = rho;
       e[1] = shat[p] / (2.0 * N * mu);
       e[2] = 1.0 / (1.0 + rN);
+      e[3] = std::exp(rho);
     }
 }
 
