@@ -112,7 +112,6 @@ struct InvArgs {
   int per;           // 0 closed basin (N rows of j), 1 periodic channel (2N rows)
   int twm;           // length of the twiddle table tw64
   int R, lgR;        // rows per tile of the row passes (scratch block height)
-  int tri;           // closed basin: 1 tridiagonal column pass, 0 sine transforms along j
 };
 
 // Static properties of a transform engine (per precision and length).

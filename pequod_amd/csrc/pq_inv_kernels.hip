@@ -4,12 +4,11 @@
 //   k_rows_fwd  per (member, layer, tile of R rows): read zeta rows, DST-I
 //               along i, write the tile transposed as one contiguous block
 //               S[l][jb][p][r] (r fastest) of the scratch.
-//   k_cols      per (member, C adjacent columns p), all layers: read the
-//               columns (C*R contiguous elements per block), DST-I along j,
-//               project layers -> modes (z_to_modes), divide by
-//               (2N)^2 (lambda_p + lambda_q - ss_m), accumulate the FP64
-//               partial of the domain mean psiC_m, project modes -> layers
-//               (modes_to_z), inverse DST-I along j, write back in place.
+//   k_cols_tri  per (member, CW adjacent columns p), all layers: read the
+//               columns, project layers -> modes (z_to_modes), solve each
+//               mode's tridiagonal system along j (FP64), accumulate the
+//               FP64 partial of the domain mean psiC_m, project modes ->
+//               layers (modes_to_z), write back in place.
 //   k_psic      (host file) per member: psiC from the column pass's partials,
 //               kappa_l and beta_lm.
 //   k_cols_chan the periodic channel's column pass (periodic transform of
@@ -19,11 +18,11 @@
 //               helm(1)), inverse DST-I along i, subtract the constant part,
 //               write psi rows including the boundary.
 //
-// The sine transforms along i and j replace FFTPACK sint along i plus the
-// Thomas sweep along j of the reference's helm (qg3p_helmholtz.F90:614-769):
-// both solve the same 5-point system (manual/manual.tex:826-832).  The modal
-// projection commutes with the transforms, so it runs in spectral space where
-// all layers of a column are in LDS together.
+// As the reference's helm (qg3p_helmholtz.F90:614-769): a sine transform
+// along i (FFTPACK sint there), a tridiagonal solve along j per sine mode
+// (the Thomas sweep there), the inverse transform along i.  The modal
+// projection commutes with the i-transform, so it runs in the column pass,
+// where all layers of a column are at hand.
 //
 // Mass correction (qg3p_aux.F90:196-230, qg3p_diagnostics.F90:615-635): with
 // c_m = helm_m(1) (Dirichlet 0), corr_m = a_m + b_m c_m where
@@ -292,136 +291,8 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
   }
 }
 
+
 // ------------------------------------------------------------------ K2
-// Grid: sum over members of N / C_b workgroups of G * TPF threads, G >=
-// ceil(C_b L_b / 2) for every member.
-template <typename Eng, typename Ts>
-__global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
-  using T = typename Eng::T;
-  using Tc = typename Eng::Tc;
-  using VT = typename V16<Ts>::type;
-  constexpr int N = Eng::N, TPF = Eng::TPF, V = V16<Ts>::n;
-  extern __shared__ __align__(16) unsigned char smem[];
-  T *lds = reinterpret_cast<T *>(smem);
-  const int R = a.R, lgR = a.lgR;
-  // neighbouring column blocks share the cache lines of the C R runs
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
-  const int b = member_of(ptrs.cum_c, ptrs.nb, bid);
-  const int L = ptrs.lay[b], lgC = ptrs.lgc[b], C = 1 << lgC;
-  const int pblk = bid - ptrs.cum_c[b];
-  const int p0 = pblk << lgC;
-  const int ngrp = blockDim.x / TPF, nt = blockDim.x;
-  const int f = threadIdx.x / TPF, t = threadIdx.x % TPF;
-  const int nrows = L << lgC;  // row rho = l * C + c
-  typename Eng::Fwd::Ctx ctx;
-  ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
-           a.twm / Eng::Fwd::M);
-  T *region = lds + f * Eng::Region;
-  T *scan_tmp = lds + ngrp * Eng::Region + f * Eng::ScanTmp;
-  const size_t red_off = (sizeof(T) * (size_t)ngrp * (Eng::Region + Eng::ScanTmp) + 15) & ~size_t(15);
-  double *red = reinterpret_cast<double *>(smem + red_off);
-  Tc *cD = reinterpret_cast<Tc *>(red + 64);  // D, M (rows of 8), ssN in Tc (L <= 8)
-  Tc *cM = cD + 64;
-  Tc *cS = cM + 64;
-  const CoefLayout CL{L, N};
-  const double *coef = ptrs.coef[b];
-  // coefficients: loads issued now, stored to LDS after the forward
-  // transform.  Thread i < 64 takes entry (r, c) = (i / 8, i % 8) of both
-  // matrices (zero outside L x L): cD[i] = D(r,c), cM[i] = M(r,c); in the
-  // coefficient block X(r,c) sits at r + L c.
-  double cd0 = 0.0, cm0 = 0.0, cs0 = 0.0;
-  {
-    const int i = threadIdx.x, r = i >> 3, c = i & 7;
-    if (i < 64 && r < L && c < L) {
-      cd0 = coef[CL.D() + r + L * c];
-      cm0 = coef[CL.M() + r + L * c];
-    }
-  }
-  if ((int)threadIdx.x < L) cs0 = coef[CL.ssN() + threadIdx.x];
-
-  // Element j of row (l, p) sits at l N^2 + (j / R) N R + p R + j % R.
-  Ts *scr = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride);
-  const int ra = 2 * f, rb = ra + 1;
-  const int la = ra >> lgC, pa = p0 + (ra & (C - 1));
-  const int lb = rb >> lgC, pb = p0 + (rb & (C - 1));
-  const bool va = ra < nrows && pa >= 1;
-  const bool vb = rb < nrows && pb >= 1;
-  const Ts *sa = scr + (size_t)la * N * N + (size_t)pa * R;
-  const Ts *sb = scr + (size_t)lb * N * N + (size_t)pb * R;
-  const int blk = N << lgR;
-  Eng::Fwd::run(region, t, scan_tmp, ctx, [&](int k, int, T &xa, T &xb) {
-    const int off = (k >> lgR) * blk + (k & (R - 1));
-    xa = va ? T(sa[off]) : T(0);
-    xb = vb ? T(sb[off]) : T(0);
-  });
-  if ((int)threadIdx.x < 64) {
-    cD[threadIdx.x] = Tc(cd0);
-    cM[threadIdx.x] = Tc(cm0);
-  }
-  if ((int)threadIdx.x < L) cS[threadIdx.x] = Tc(cs0);
-  // small workgroups (fewer threads than coefficients)
-  for (int i = threadIdx.x + nt; i < 64; i += nt) {
-    const int r = i >> 3, c = i & 7;
-    const bool in = r < L && c < L;
-    cD[i] = in ? Tc(coef[CL.D() + r + L * c]) : Tc(0);
-    cM[i] = in ? Tc(coef[CL.M() + r + L * c]) : Tc(0);
-  }
-  for (int i = threadIdx.x + nt; i < L; i += nt) cS[i] = Tc(coef[CL.ssN() + i]);
-  __syncthreads();
-
-  // Spectral step over the points (p0 + c, q).
-  double acc[16];
-#pragma unroll
-  for (int m = 0; m < 16; ++m) acc[m] = 0.0;
-  switch (a.mode == 0 ? L : 0) {
-    case 1: spectral_step<Eng, 1>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 2: spectral_step<Eng, 2>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 3: spectral_step<Eng, 3>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 4: spectral_step<Eng, 4>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 5: spectral_step<Eng, 5>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 6: spectral_step<Eng, 6>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 7: spectral_step<Eng, 7>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    case 8: spectral_step<Eng, 8>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-    default: spectral_step<Eng, 0>(a, lds, cD, cM, cS, p0, nt, L, lgC, acc); break;
-  }
-  // psiC partial of this workgroup (fixed-order FP64 reduction, which also
-  // publishes the spectral step's LDS writes); the row inverse pass reduces
-  // the partials of a member after the kernel boundary.
-  if (a.mode == 0) {
-    block_sum(acc, L, red);
-    if (threadIdx.x == 0) {
-      double *part = a.part + (size_t)b * a.part_stride + (size_t)pblk * L;
-      for (int m = 0; m < L; ++m) part[m] = acc[m];
-    }
-  } else {
-    __syncthreads();
-  }
-
-  // Inverse DST along j, in place.
-  {
-    typename Eng::Inv::Ctx ctxi;
-    ctxi.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
-              a.twm / Eng::Inv::M);
-    Eng::Inv::run_rows(region, t, scan_tmp, ctxi);
-  }
-  __syncthreads();
-  // Write back: per layer and block, C*R contiguous elements.
-  const int per_layer = (N << lgC) / V;  // vectors per layer: N/R blocks x C x R
-  for (int ll = 0; ll < L; ++ll) {
-    for (int iv = threadIdx.x; iv < per_layer; iv += nt) {
-      const int e = iv * V;
-      const int r0 = e & (R - 1), c = (e >> lgR) & (C - 1), jb = e >> (lgR + lgC);
-      const T *src = row_ptr<Eng>(lds, (ll << lgC) + c);
-      VT v;
-#pragma unroll
-      for (int q = 0; q < V; ++q) v[q] = Ts(src[pad16((jb << lgR) + r0 + q)]);
-      Ts *dst = scr + (size_t)ll * N * N + ((size_t)jb * N + p0 + c) * R + r0;
-      *reinterpret_cast<VT *>(dst) = v;
-    }
-  }
-}
-
-// ------------------------------------------------------------ K2 tridiagonal
 // The closed basin's column pass as the reference's own method solves it:
 // a tridiagonal system along j per column p and mode m (the Thomas sweep
 // of h1d / ge_local, qg3p_helmholtz.F90:1028-1170), in a form that runs in
@@ -429,8 +300,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
 // transform, the mode's right-hand side is
 //   z_m(j) = (1 / 2N) sum_l D(m,l) x_l(p, j)                (z(0) = 0),
 // and y_m solves y(j-1) - (2 + mu) y(j) + y(j+1) = z(j), y(0) = y(N) = 0,
-// mu = 4 sin^2(p pi / 2N) + ss_m (the sine transforms along j of k_cols
-// give the same y).  With r + 1/r = 2 + mu, 0 < r < 1, the free-space
+// mu = 4 sin^2(p pi / 2N) + ss_m.  With r + 1/r = 2 + mu, 0 < r < 1, the free-space
 // Green's function r^|k| / (r - 1/r) gives the particular solution
 //   yp(j) = (F(j) + B(j) - z(j)) / (r - 1/r),
 //   F(j) = r F(j-1) + z(j),  B(j) = r B(j+1) + z(j),
@@ -443,6 +313,7 @@ __global__ void __launch_bounds__(512) k_cols(InvArgs a, PtrTab ptrs) {
 // handle's element type.  The pass also forms the psiC partials
 // sum_p shat_p sum_j y_m(p, j) and writes x_l = sum_m M(l,m) y_m back in
 // place.  Grid: per member N / CW workgroups of CW columns x N / E threads.
+
 // x^k for 0 <= k < KMAX by squaring (a fixed number of steps, so that
 // lanes with different k stay converged).
 template <int KMAX>
@@ -1014,7 +885,7 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
   } else if (pass == 1 && a.per) {
     lds_limit(k_cols_chan<CE, Ts>, lds);
     hipLaunchKernelGGL((k_cols_chan<CE, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
-  } else if (pass == 1 && a.tri) {
+  } else if (pass == 1) {
     switch (a.L) {
       case 1: hipLaunchKernelGGL((k_cols_tri<Ts, 1, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
       case 2: hipLaunchKernelGGL((k_cols_tri<Ts, 2, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
@@ -1022,9 +893,6 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
       case 4: hipLaunchKernelGGL((k_cols_tri<Ts, 4, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
       default: hipLaunchKernelGGL((k_cols_tri<Ts, 8, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
     }
-  } else if (pass == 1) {
-    lds_limit(k_cols<Eng, Ts>, lds);
-    hipLaunchKernelGGL((k_cols<Eng, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);
   } else {
     lds_limit(k_rows_inv<Eng, Tio, Ts>, lds);
     hipLaunchKernelGGL((k_rows_inv<Eng, Tio, Ts>), dim3(blocks), dim3(threads), lds, s, a, pt);

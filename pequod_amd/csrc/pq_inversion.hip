@@ -202,9 +202,6 @@ struct pq_inv {
   double *ctab = nullptr;
   int twm = 0;  // twiddle table length
   int R = 0, C = 0;
-  // closed basin: 1 tridiagonal column pass (default), 0 sine transforms
-  // along j (PQ_COLUMN_PASS=fft at creation; kept for comparison)
-  int tri = 1;
   std::vector<int> lay;  // layers of each member slot (<= L)
   KernelSet ks;
   EngineInfo ei{};
@@ -489,10 +486,6 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   }
   h->ei = h->ks.info(precision);
   h->part_cap = N;  // C >= 1
-  {
-    const char *cp = std::getenv("PQ_COLUMN_PASS");
-    h->tri = per ? 0 : !(cp && std::strcmp(cp, "fft") == 0);
-  }
   int R, C;
   default_tiles(h, R, C);
   if (!tiles_ok(h, R, C, true)) {
@@ -560,14 +553,13 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.twm = h->twm;
   a.R = h->R;
   a.lgR = ilog2i(h->R);
-  a.tri = h->tri;
   const int rows_threads = (h->R / 2) * e.tpf;
   const int col_groups = (h->C * L + 1) / 2;
   for (int pass = 0; pass < 3; ++pass) {
     if (!(pass_mask & (1 << pass))) continue;
     int blocks, threads;
     size_t lds;
-    if (pass == 1 && h->tri) {
+    if (pass == 1 && !h->per) {
       blocks = pt.cum_c[nb];
       threads = tri_threads(N, L);
       lds = 0;
@@ -629,7 +621,7 @@ static int inv_launch(pq_inv *h, int batch, const void *const *zeta, long ldz1, 
       pt.coef[b] = coef ? coef[b0 + b] : h->coef + h->coef_size * (size_t)(b0 + b);
       const int Lb = layers ? layers[b0 + b] : h->lay[b0 + b];
       int Cb = h->C;
-      if (h->tri) {
+      if (!h->per) {
         Cb = tri_cols(h->N, h->L);  // one column geometry for the launch
       } else if (Lb != h->L) {
         Cb = 8;
