@@ -437,10 +437,12 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32w", "fp32s", "mixed", "fp64"],
-                    help="fp32 (default): FP32 storage and transforms; fp32w: FP32 storage, FP64 "
-                         "arithmetic; fp32s: FP32 storage, FP64 transforms; mixed: FP64 storage, "
-                         "FP32 transforms; fp64: plain drop-in")
+    ap.add_argument("--precision", default="fp32m",
+                    choices=["fp32m", "fp32", "fp32w", "fp32s", "mixed", "fp64"],
+                    help="fp32m (default): FP32 storage, row transforms on the matrix cores (fp16 "
+                         "hi/lo split operands, FP32 accumulation); fp32: FP32 storage and "
+                         "transforms; fp32w: FP32 storage, FP64 arithmetic; fp32s: FP32 storage, "
+                         "FP64 transforms; mixed: FP64 storage, FP32 transforms; fp64: plain drop-in")
     ap.add_argument("--ensemble-precision", default="fp32auto",
                     choices=["fp32auto", "fp32", "fp32w", "fp32s", "mixed", "fp64"],
                     help="precision of the batched 64-member ensemble leg")
@@ -607,11 +609,14 @@ def main():
             lat.append(e0.elapsed_time(e1))
         p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
-    # plain FP64 drop-in rate on the same workload
-    fp64_rate = None
+    # plain FP64 drop-in rate on the same workload, and the FP32 FFT path
+    fp64_rate = fp32_rate = None
     if args.precision != "fp64":
         _, _, _, w64, _ = run_rate("fp64", max(20, args.steps // 4), 5, B)
         fp64_rate = B * max(20, args.steps // 4) / w64
+    if args.precision not in ("fp32", "fp64"):
+        _, _, _, w32, _ = run_rate("fp32", max(20, args.steps // 4), 5, B)
+        fp32_rate = B * max(20, args.steps // 4) / w32
 
     # device-resident CABARET models (same grid, FP64 and FP32 fields) and
     # the centred scheme: steps/s at a fixed dt of Courant number 0.3
@@ -693,7 +698,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "fp32s": "f32 storage, f64 transforms",
+        "dtype": {"fp32m": "f32 storage; row transforms f16 hi+lo on MFMA, f32 accumulate",
+                  "fp32": "f32", "fp32s": "f32 storage, f64 transforms",
                   "fp32w": "f32 storage, f64 arithmetic",
                   "mixed": "f64 storage, f32 transforms", "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded k^-3 PV, rms 1e-5 s^-1)",
@@ -707,6 +713,7 @@ def main():
         "helmholtz_residual_per_mode": None if acc is None else acc["helmholtz_residual_per_mode"],
         "fp64_oracle_accuracy": acc_ref,
         "fp64_dropin_inversions_per_s": None if fp64_rate is None else round(fp64_rate, 1),
+        "fp32_fft_inversions_per_s": None if fp32_rate is None else round(fp32_rate, 1),
         "pass_ms": {n: round(t, 5) for n, t in zip(names, passes)},
         "cabaret_fp64_steps_per_s": None if cab is None else round(cab, 2),
         "cabaret_fp32_steps_per_s": None if cab32 is None else round(cab32, 2),

@@ -58,6 +58,11 @@ extern "C" {
 /* FP32 storage (fields, scratch and on-chip buffers), FP64 arithmetic: the
  * transforms round to FP32 only between FFT stages.                        */
 #define PQ_PREC_FP32_WIDE 5
+/* FP32 storage and scratch; the row sine transforms (along i) as two GEMM
+ * stages on the matrix cores (fp16 hi + lo split operands, FP32
+ * accumulation), the column pass as in PQ_PREC_FP32.  Transform length
+ * g - 1 = 1024; the closed basin only.                                     */
+#define PQ_PREC_FP32_MFMA 6
 
 /* PQ_PREC_FP32 or PQ_PREC_FP32_STORAGE for an L-layer stratification: FP32
  * transforms when kappa_inf(modes_to_z) = |modes_to_z|_inf |z_to_modes|_inf

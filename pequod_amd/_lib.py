@@ -15,10 +15,12 @@ PQ_OK, PQ_ERR_ARG, PQ_ERR_HIP, PQ_ERR_NAN, PQ_WARN_LARGE, PQ_ERR_NOMEM = range(6
 # fp32auto: FP32 storage, FP32 transforms when the projection is well
 # conditioned (pq_prec_fp32_choice), else FP64 transforms
 # fp32w: FP32 storage, FP64 arithmetic (FP32 rounding between FFT stages)
-PREC = {"fp64": 0, "fp32": 1, "mixed": 2, "fp32s": 3, "fp32auto": 4, "fp32w": 5}
+# fp32m: FP32 storage, row transforms on the matrix cores (fp16 hi/lo split
+# operands, FP32 accumulation), g - 1 = 1024
+PREC = {"fp64": 0, "fp32": 1, "mixed": 2, "fp32s": 3, "fp32auto": 4, "fp32w": 5, "fp32m": 6}
 PREC_NAME = {v: k for k, v in PREC.items()}
 # element type of the zeta / psi arrays of each precision
-IO_BYTES = {"fp64": 8, "fp32": 4, "mixed": 8, "fp32s": 4, "fp32auto": 4, "fp32w": 4}
+IO_BYTES = {"fp64": 8, "fp32": 4, "mixed": 8, "fp32s": 4, "fp32auto": 4, "fp32w": 4, "fp32m": 4}
 
 _i, _d, _vp, _sz, _l = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_long
 _dp = ctypes.POINTER(ctypes.c_double)

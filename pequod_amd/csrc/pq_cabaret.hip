@@ -1288,7 +1288,7 @@ int pq_model_create(pq_model **out, const pq_model_params *p) {
                "pq_model_create: unknown scheme %d", p->scheme);
   PQ_CHECK_ARG(p->scheme == PQ_SCHEME_CABARET || !p->adapt_dt,
                "pq_model_create: adapt_dt is a CABARET option");
-  PQ_CHECK_ARG(p->precision >= 0 && p->precision <= PQ_PREC_FP32_WIDE,
+  PQ_CHECK_ARG(p->precision >= 0 && p->precision <= PQ_PREC_FP32_MFMA,
                "pq_model_create: bad precision %d", p->precision);
   pq_model *m = new pq_model();
   m->es = (p->precision == PQ_PREC_FP64 || p->precision == PQ_PREC_MIXED) ? 8 : 4;

@@ -114,7 +114,7 @@ extern "C" {
 
 int pq_ens_create(pq_ens **out, int precision) {
   PQ_CHECK_ARG(out != nullptr, "pq_ens_create: null pointer");
-  PQ_CHECK_ARG(precision >= 0 && precision <= PQ_PREC_FP32_WIDE, "pq_ens_create: bad precision %d",
+  PQ_CHECK_ARG(precision >= 0 && precision <= PQ_PREC_FP32_MFMA, "pq_ens_create: bad precision %d",
                precision);
   pq_ens *e = new pq_ens();
   e->prec = precision;

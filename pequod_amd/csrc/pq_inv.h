@@ -112,6 +112,7 @@ struct InvArgs {
   int per;           // 0 closed basin (N rows of j), 1 periodic channel (2N rows)
   int twm;           // length of the twiddle table tw64
   int R, lgR;        // rows per tile of the row passes (scratch block height)
+  const void *mtab;  // PQ_PREC_FP32_MFMA: the matrix-core engine's per-lane constants
 };
 
 // Static properties of a transform engine (per precision and length).

@@ -37,6 +37,7 @@
 // is zero.  So no correction field is read from HBM.
 #include "pq_fft.h"
 #include "pq_inv.h"
+#include "pq_mfma.h"
 
 #ifndef PQ_N
 #error "compile with -DPQ_N=<transform length>"
@@ -642,6 +643,76 @@ __global__ void __launch_bounds__(512) k_cols_chan(InvArgs a, PtrTab ptrs) {
   }
 }
 
+// The mass correction of the closed basin for odd p (even p have none),
+// subtracted from the tile's rows in the i-spectral domain:
+// x(r, p) -= sum_{m>=2} beta_lm c_m(p, j0 + r), with c_m's closed form
+// shat_p / 2N g_mp(j) from the coefficient table (rho = ln r, gamma
+// (scaled by beta_lm here), eta, r per mode and odd p), by recurrence over
+// the tile's R rows in FP64.  sub(r, p, v) subtracts v from element
+// (r, p).  All threads of the workgroup call it (tid, nt).
+template <int NN, typename Sub>
+__device__ __forceinline__ void mass_correction(const double *tab, const double (&beta)[8], int L,
+                                                int jb, int NB, int R, int lgR, int tid, int nt,
+                                                Sub sub) {
+  const int j0 = jb << lgR;
+  for (int h = tid; h < NN / 2; h += nt) {
+    const int p = 2 * h + 1;
+    double acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+    for (int m = 1; m < L; ++m) {
+      const double *e = tab + ((size_t)(m - 1) * (NN / 2) + h) * 4;
+      const double rho = e[0], gam = beta[m] * e[1], eta = e[2], rr = e[3];
+      // r^j for the tile's rows is below e^-40 (relative 4e-18) far from
+      // both edges: then only the constant part remains
+      const int jmin = min(j0, NN - j0 - R + 1);
+      if (double(jmin) * -rho > 40.0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          if (r < R) acc[r] += gam;
+        continue;
+      }
+      // r^j0 = (r^R)^jb and r^(N - j0 - R + 1) = r (r^R)^(NB - 1 - jb), by
+      // squaring (jb is the workgroup's, so the loops do not diverge)
+      double rR = rr;
+      for (int k = 0; k < lgR; ++k) rR *= rR;
+      double u = 1.0, v = rr;
+      {
+        double x = rR;
+        for (int k = jb; k; k >>= 1) {
+          if (k & 1) u *= x;
+          x *= x;
+        }
+        x = rR;
+        for (int k = NB - 1 - jb; k; k >>= 1) {
+          if (k & 1) v *= x;
+          x *= x;
+        }
+      }
+      double hh[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r)  // r^(j0 + r), going up
+        if (r < R) {
+          hh[r] = u;
+          u *= rr;
+        }
+      // r^(N - j0 - r), going down
+#pragma unroll
+      for (int r = 15; r >= 0; --r)
+        if (r < R) {
+          hh[r] += v;
+          v *= rr;
+        }
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r < R) acc[r] += gam * (1.0 - hh[r] * eta);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (r < R) sub(r, p, acc[r]);
+  }
+}
+
 // ------------------------------------------------------------------ K3
 // Grid: sum over members of L_b * (N / R) workgroups of (R / 2) * TPF threads.
 // At least 3 waves per SIMD (at most 168 VGPRs) up to N = 2048: without
@@ -723,66 +794,11 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   }
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
   if (a.mode == 0 && !a.per && L > 1) {
-    const double *tab = ptrs.coef[b] + CL.tab();
-    for (int h = threadIdx.x; h < N / 2; h += nt) {
-      const int p = 2 * h + 1;
-      double acc[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-      for (int m = 1; m < L; ++m) {
-        const double *e = tab + ((size_t)(m - 1) * (N / 2) + h) * 4;
-        const double rho = e[0], gam = beta[m] * e[1], eta = e[2], rr = e[3];
-        // r^j for the tile's rows is below e^-40 (relative 4e-18) far from
-        // both edges: then only the constant part remains
-        const int jmin = min(j0, N - j0 - R + 1);
-        if (double(jmin) * -rho > 40.0) {
-#pragma unroll
-          for (int r = 0; r < 16; ++r)
-            if (r < R) acc[r] += gam;
-          continue;
-        }
-        // r^j0 = (r^R)^jb and r^(N - j0 - R + 1) = r (r^R)^(NB - 1 - jb), by
-        // squaring (jb is the workgroup's, so the loops do not diverge)
-        double rR = rr;
-        for (int k = 0; k < lgR; ++k) rR *= rR;
-        double u = 1.0, v = rr;
-        {
-          double x = rR;
-          for (int k = jb; k; k >>= 1) {
-            if (k & 1) u *= x;
-            x *= x;
-          }
-          x = rR;
-          for (int k = NB - 1 - jb; k; k >>= 1) {
-            if (k & 1) v *= x;
-            x *= x;
-          }
-        }
-        double hh[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r)  // r^(j0 + r), going up
-          if (r < R) {
-            hh[r] = u;
-            u *= rr;
-          }
-        // r^(N - j0 - r), going down
-#pragma unroll
-        for (int r = 15; r >= 0; --r)
-          if (r < R) {
-            hh[r] += v;
-            v *= rr;
-          }
-#pragma unroll
-        for (int r = 0; r < 16; ++r)
-          if (r < R) acc[r] += gam * (1.0 - hh[r] * eta);
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r)
-        if (r < R) {
-          T *x = row_ptr<Eng>(lds, r) + pad16(p);
-          *x = T(double(*x) - acc[r]);
-        }
-    }
+    mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, lgR, threadIdx.x, nt,
+                       [&](int r, int p, double v) {
+                         T *x = row_ptr<Eng>(lds, r) + pad16(p);
+                         *x = T(double(*x) - v);
+                       });
     __syncthreads();
   }
 
@@ -829,6 +845,217 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   if (m0 && !a.per && j0 + R == N) {
     Tout *pn = ps + (size_t)N * ld1;
     for (int i = threadIdx.x; i < G; i += nt) pn[i] = Tout(-kappa);
+  }
+}
+
+
+// ------------------------------------------------------------ K1/K3 MFMA
+// PQ_PREC_FP32_MFMA row passes: the sine transform of each row on the
+// matrix cores (pq_mfma.h), one row per wavefront at a time, 4 wavefronts
+// per workgroup.  Workgroups are persistent: each loops over scratch
+// blocks (kMfmaR rows of one layer) with the grid's stride, so the
+// per-lane constants are loaded once per workgroup.  A block is staged in
+// LDS as kMfmaR rows of kMfmaRs floats: all of its loads are issued
+// together (the whole block in flight per workgroup), the rows are
+// transformed in place, and the block moves to or from the (p, r) scratch
+// layout as 16-byte vectors (the row stride keeps those transposed LDS
+// accesses at most 2-way bank conflicted).  FP32 fields and scratch; the
+// closed basin only.
+constexpr int kMfmaR = mfma::kRows;      // rows per block (the handle's R)
+constexpr int kMfmaRs = mfma::kN + 4;    // LDS row stride (floats)
+constexpr int kMfmaNt = mfma::kRowThreads;
+
+template <typename Tin>
+__global__ void __launch_bounds__(mfma::kRowThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_rows_fwd_mfma(InvArgs a, PtrTab ptrs) {
+  constexpr int N = mfma::kN, RS = kMfmaRs, R = kMfmaR, NB = N / R, NT = kMfmaNt;
+  extern __shared__ __align__(16) unsigned char smem[];
+  unsigned *a3s = reinterpret_cast<unsigned *>(smem);
+  float *st = reinterpret_cast<float *>(smem) + mfma::kA3Lds;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c = lane & 31, g = lane >> 5;
+  mfma::Consts K;
+  mfma::load_consts(static_cast<const unsigned *>(a.mtab), lane, K);
+  mfma::stage_a3(static_cast<const unsigned *>(a.mtab), a3s, tid, NT);
+  const int nblk = ptrs.cum_r[ptrs.nb];
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int b = member_of(ptrs.cum_r, ptrs.nb, blk);
+    const int rem = blk - ptrs.cum_r[b];
+    const int l = rem / NB, jb = rem - l * NB, j0 = jb * R;
+    const Tin *z = static_cast<const Tin *>(ptrs.zeta[b]) + (size_t)l * ptrs.ldz1 * ptrs.ldz2 +
+                   (size_t)j0 * ptrs.ldz1;
+    // the block's rows: x_i, i = 0..N-1 (the row's node i = N is a
+    // boundary node and not read); row j = 0 (the boundary) as zeros
+    float v[R][4];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[r][q] = float(z[(size_t)r * ptrs.ldz1 + 4 * tid + q]);
+    __syncthreads();  // the A3 copy (first block) / the previous block's reads of st
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) st[r * RS + 4 * tid + q] = (j0 + r == 0) ? 0.0f : v[r][q];
+    __syncthreads();
+    for (int r = w; r < R; r += NT / 64) {
+      if (j0 + r == 0) continue;  // the boundary row stays zero
+      float *row = st + r * RS;
+      float x[2][8];
+      const float *src = row + c + 256 * g;
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) x[t][s2] = src[512 * t + 32 * s2];
+      if (lane == 0) x[0][0] = 0.0f;  // x_0: the boundary node
+      mfma::dst_row(K, a3s, x, lane, row);
+    }
+    __syncthreads();
+    // Block (l, jb): S[p][r] = row r at p, r fastest; one contiguous N R run.
+    float *out = reinterpret_cast<float *>(a.scr + b * a.scr_stride) + (size_t)(l * NB + jb) * N * R;
+#pragma unroll
+    for (int k = 0; k < N * R / 4 / NT; ++k) {
+      const int i = tid + NT * k, e0 = 4 * i, p = e0 / R, r0 = e0 % R;
+      const float *src = st + r0 * RS + p;
+      float4 o;
+      o.x = src[0];
+      o.y = src[RS];
+      o.z = src[2 * RS];
+      o.w = src[3 * RS];
+      reinterpret_cast<float4 *>(out)[i] = o;
+    }
+  }
+}
+
+// Lowest input modes of the inverse transform taken out of the matrix-core
+// transform and added to psi in FP64 (as the FP32 engines do): the
+// solution's spectrum is dominated by them, and the transform's rounding
+// error scales with its input norm.
+constexpr int kMfmaLow = 4;
+
+// cos and sin of pi k / 16, k = 0..15 (compile-time after unrolling)
+__device__ __forceinline__ double cospi16(int k) {
+  constexpr double t[16] = {1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+                            0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173,
+                            0.19509032201612826785, 0.0, -0.19509032201612826785, -0.38268343236508977173,
+                            -0.55557023301960222474, -0.70710678118654752440, -0.83146961230254523708,
+                            -0.92387953251128675613, -0.98078528040323044913};
+  return t[k];
+}
+__device__ __forceinline__ double sinpi16(int k) { return k < 8 ? cospi16(8 - k) : cospi16(k - 8); }
+
+template <typename Tout>
+__global__ void __launch_bounds__(mfma::kRowThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
+  constexpr int N = mfma::kN, G = N + 1, RS = kMfmaRs, R = kMfmaR, NB = N / R, NT = kMfmaNt;
+  extern __shared__ __align__(16) unsigned char smem[];
+  unsigned *a3s = reinterpret_cast<unsigned *>(smem);
+  float *st = reinterpret_cast<float *>(smem) + mfma::kA3Lds;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c = lane & 31, g = lane >> 5;
+  const bool m0 = a.mode == 0;
+  // sin and cos of pi lane / N (FP64) for the psi rows' low modes
+  const cx<double> t0 = static_cast<const cx<double> *>(a.tw64)[lane];  // exp(-i pi lane / N)
+  const double sb = -t0.y, cb = t0.x;
+  mfma::Consts K;
+  mfma::load_consts(static_cast<const unsigned *>(a.mtab), lane, K);
+  mfma::stage_a3(static_cast<const unsigned *>(a.mtab), a3s, tid, NT);
+  const int nblk = ptrs.cum_r[ptrs.nb];
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int b = member_of(ptrs.cum_r, ptrs.nb, blk);
+    const int L = ptrs.lay[b];
+    const int rem = blk - ptrs.cum_r[b];
+    const int l = rem / NB, jb = rem - l * NB, j0 = jb * R;
+    const CoefLayout CL{L, N};
+    // the tile block, all loads in flight, then into LDS rows r at st + r RS
+    const float4 *in = reinterpret_cast<const float4 *>(a.scr + b * a.scr_stride) + (size_t)(l * NB + jb) * N * R / 4;
+    float4 v[N * R / 4 / NT];
+#pragma unroll
+    for (int k = 0; k < N * R / 4 / NT; ++k) v[k] = in[tid + NT * k];
+    double kappa = 0.0;
+    double beta[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) beta[m] = 0.0;
+    if (m0) {
+      const double *kb = a.kb + (size_t)b * a.kb_stride;
+      kappa = kb[l];
+#pragma unroll
+      for (int m = 1; m < 8; ++m)
+        if (m < L) beta[m] = kb[L + l * L + m];
+    }
+    __syncthreads();  // the previous block's reads of st
+#pragma unroll
+    for (int k = 0; k < N * R / 4 / NT; ++k) {
+      const int e0 = 4 * (tid + NT * k), p = e0 / R, r0 = e0 % R;
+      float *dst = st + r0 * RS + p;
+      dst[0] = v[k].x;
+      dst[RS] = v[k].y;
+      dst[2 * RS] = v[k].z;
+      dst[3 * RS] = v[k].w;
+    }
+    __syncthreads();  // publishes the tile (and the A3 copy)
+    if (m0 && L > 1) {
+      mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, ilog2c(R), tid, NT,
+                         [&](int r, int p, double x) {
+                           float *e = st + r * RS + p;
+                           *e = float(double(*e) - x);
+                         });
+      __syncthreads();
+    }
+    // Each wavefront transforms its rows in place and writes their psi rows
+    // (no barrier: a row is only touched by its wavefront).  pv_inversion
+    // (mode 0): psi = X + low - kappa, -kappa on the boundary; helm (mode
+    // 1) keeps the reference's boundary semantics (row 0 zero at
+    // 1 <= i <= N - 1, X at 1 <= i <= N - 1 and 0 at i = N elsewhere, column
+    // 0 and row N not written).  low = sum_{p <= kMfmaLow} 2 x_p
+    // sin(pi p i / N) in FP64: the lowest modes leave the matrix-core
+    // transform (zeroed in its operand), their sines by the Chebyshev
+    // recurrence from sin and cos(pi i / N).
+    Tout *ps = static_cast<Tout *>(ptrs.psi[b]) + (size_t)l * ptrs.ldp1 * ptrs.ldp2;
+    const long long ld1 = ptrs.ldp1;
+    const double sub = m0 ? kappa : 0.0;
+    for (int r = w; r < R; r += NT / 64) {
+      const int j = j0 + r;
+      Tout *pa = ps + (size_t)j * ld1;
+      float *row = st + r * RS;
+      if (j == 0) {
+        for (int i = lane; i < G; i += 64)
+          if (m0 || (i != 0 && i != N)) pa[i] = Tout(0.0 - sub);
+        continue;
+      }
+      // low = s1 P(c1), s1 = sin(pi i / N), c1 = cos(pi i / N), with
+      // sin(2t) = 2 s c, sin(3t) = s (4c^2 - 1), sin(4t) = 4 s c (2c^2 - 1):
+      // P(c) = (w1 - w3) + (2 w2 - 4 w4) c + 4 w3 c^2 + 8 w4 c^3 for the
+      // weights w_p = 2 x_p
+      static_assert(kMfmaLow == 4, "the low-mode polynomial is written for 4 modes");
+      double lw[kMfmaLow];
+#pragma unroll
+      for (int q = 0; q < kMfmaLow; ++q) lw[q] = 2.0 * double(row[1 + q]);
+      const double p0 = lw[0] - lw[2], p1 = 2.0 * lw[1] - 4.0 * lw[3], p2 = 4.0 * lw[2], p3 = 8.0 * lw[3];
+      float x[2][8];
+      const float *src = row + c + 256 * g;
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) x[t][s2] = src[512 * t + 32 * s2];
+      if (lane <= kMfmaLow) x[0][0] = 0.0f;  // x_0 and the low modes (lanes 1..kMfmaLow)
+      mfma::dst_row(K, a3s, x, lane, row);
+      // the row's psi at i = lane + 64 k (0..N - 1), then i = N; the
+      // angle pi i / N = pi lane / N + pi k / 16
+#pragma unroll 4
+      for (int k = 0; k < N / 64; ++k) {
+        const int i = lane + 64 * k;
+        const double ca = cospi16(k), sa = sinpi16(k);
+        const double s1 = fma(sb, ca, cb * sa), c1 = fma(cb, ca, -sb * sa);
+        const double pc = fma(fma(fma(p3, c1, p2), c1, p1), c1, p0);
+        const double y = i == 0 ? 0.0 : fma(s1, pc, double(row[i]));
+        if (m0 || i != 0) pa[i] = Tout(y - sub);
+      }
+      if (lane == 0) pa[N] = Tout(0.0 - sub);
+    }
+    if (m0 && j0 + R == N) {
+      Tout *pn = ps + (size_t)N * ld1;
+      for (int i = tid; i < G; i += NT) pn[i] = Tout(-kappa);
+    }
   }
 }
 
@@ -918,9 +1145,32 @@ EngineInfo PQ_CAT(inv_info_, PQ_N)(int prec) {
   }
 }
 
+// PQ_PREC_FP32_MFMA: FP32 fields and scratch, row transforms on the matrix
+// cores (lengths mfma::supported), the FP32 column pass.
+static hipError_t launch_mfma(int pass, const InvArgs &a, const PtrTab &pt, int blocks, int threads,
+                              size_t lds, hipStream_t s) {
+  if constexpr (mfma::supported(PQ_N)) {
+    if (pass == 0) {
+      lds_limit(k_rows_fwd_mfma<float>, lds);
+      hipLaunchKernelGGL((k_rows_fwd_mfma<float>), dim3(blocks), dim3(threads), lds, s, a, pt);
+      return hipGetLastError();
+    }
+    if (pass == 2) {
+      lds_limit(k_rows_inv_mfma<float>, lds);
+      hipLaunchKernelGGL((k_rows_inv_mfma<float>), dim3(blocks), dim3(threads), lds, s, a, pt);
+      return hipGetLastError();
+    }
+    return launch_pass<P32, C32, float, float>(pass, a, pt, blocks, threads, lds, s);
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
 hipError_t PQ_CAT(inv_launch_, PQ_N)(int prec, int pass, const InvArgs &a, const PtrTab &pt,
                                      int blocks, int threads, size_t lds, hipStream_t s) {
   switch (prec) {
+    case PQ_PREC_FP32_MFMA:
+      return launch_mfma(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP64:
       return launch_pass<P64, C64, double, double>(pass, a, pt, blocks, threads, lds, s);
     case PQ_PREC_FP32:

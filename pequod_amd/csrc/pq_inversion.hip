@@ -14,6 +14,7 @@
 // contiguously; the column pass moves C R contiguous elements per block.
 #include "pq_internal.h"
 #include "pq_inv.h"
+#include "pq_mfma.h"
 
 #include <hip/hip_runtime.h>
 
@@ -207,6 +208,8 @@ struct pq_inv {
   EngineInfo ei{};
   // tables
   void *tw64 = nullptr, *was = nullptr, *lam4 = nullptr;
+  unsigned *mtab = nullptr;  // PQ_PREC_FP32_MFMA: per-lane constants of the row engine
+  int ncu = 0;               // compute units of the handle's device (persistent row grids)
   double *shat = nullptr;
   std::vector<double> lam_h, shat_h;  // lam_h[p] = -4 sin^2(p pi / 2N)
   size_t coef_size = 0;  // doubles per member
@@ -384,6 +387,94 @@ static void compute_coefs(const pq_inv *h, int L, const double *ss, const double
 }
 
 // Whether rows-per-tile R and columns-per-workgroup C fit the kernels.
+
+// The matrix-core row engine's per-lane constants (pq_mfma.h): word w of
+// lane l at w * 64 + l, fp16 pairs (even slot in the low half) split as
+// hi = fp16(v), lo = fp16(v - hi) from the long double value, and FP32
+// twiddles.
+namespace pq {
+namespace mfma {
+void build_table(unsigned *tab) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  auto put_half = [&](int word, int lane, int half, _Float16 v) {
+    unsigned short bits;
+    std::memcpy(&bits, &v, sizeof(bits));
+    unsigned &w = tab[word * 64 + lane];
+    w = half ? ((w & 0xffffu) | ((unsigned)bits << 16)) : ((w & 0xffff0000u) | bits);
+  };
+  auto put_split = [&](int whi, int wlo, int lane, int slot, long double v) {
+    const _Float16 hi = (_Float16)(double)v;
+    const _Float16 lo = (_Float16)(double)(v - (long double)(double)hi);
+    put_half(whi + slot / 2, lane, slot & 1, hi);
+    put_half(wlo + slot / 2, lane, slot & 1, lo);
+  };
+  auto put_f32 = [&](int word, int lane, long double v) {
+    const float f = (float)v;
+    std::memcpy(&tab[word * 64 + lane], &f, sizeof(f));
+  };
+  for (int i = 0; i < kWords * 64; ++i) tab[i] = 0;
+  for (int lane = 0; lane < 64; ++lane) {
+    const int c = lane & 31, g = lane >> 5;
+    for (int s = 0; s < 8; ++s) {
+      // stage 1: B operand, k = i2 = 16 t + 8 g + s, column c
+      for (int t = 0; t < 2; ++t) {
+        const int i2 = 16 * t + 8 * g + s;
+        const long double re = cosl(pi * c * i2 / 32.0L);
+        const long double im = c ? sinl(pi * c * i2 / 32.0L) : ((i2 & 1) ? -1.0L : 1.0L);
+        put_split(kB1h + (2 * t + 0) * 4, kB1l + (2 * t + 0) * 4, lane, s, re);
+        put_split(kB1h + (2 * t + 1) * 4, kB1l + (2 * t + 1) * 4, lane, s, im);
+      }
+      // stage 2 and the p = 32 m rows: A operand, row m = c, k-step t,
+      // slot s <-> i1 = 8 t + 4 g + s % 4, part s / 4 (0: Re Z, 1: Im Z)
+      for (int t = 0; t < 4; ++t) {
+        const int i1 = 8 * t + 4 * g + (s & 3), part = s >> 2, m = c;
+        long double v2;
+        if (m < 16) {
+          const long double a = 2.0L * pi * m * i1 / 32.0L;
+          v2 = part ? cosl(a) : sinl(a);
+        } else {
+          const long double a = 2.0L * pi * (m - 15) * i1 / 32.0L;
+          v2 = part ? -cosl(a) : sinl(a);
+        }
+        put_split(kA2h + 4 * t, kA2l + 4 * t, lane, s, v2);
+        const long double v3 = (part == (m & 1)) ? sinl(pi * m * i1 / 32.0L) : 0.0L;
+        put_split(kA3h + 4 * t, kA3l + 4 * t, lane, s, v3);
+      }
+    }
+    // twiddles W^(c (4 g + b)) and W^(8 a c), W = exp(i pi / 1024)
+    for (int b = 0; b < 4; ++b) {
+      const long double a = pi * c * (4 * g + b) / (long double)kN;
+      put_f32(kBre + b, lane, cosl(a));
+      put_f32(kBim + b, lane, sinl(a));
+    }
+    for (int a = 1; a < 4; ++a) {
+      const long double x = pi * 8.0L * a * c / (long double)kN;
+      put_f32(kSre + a - 1, lane, cosl(x));
+      put_f32(kSim + a - 1, lane, sinl(x));
+    }
+  }
+}
+}  // namespace mfma
+}  // namespace pq
+
+static int upload_mfma_table(pq_inv *h) {
+  std::vector<unsigned> tab(mfma::kWords * 64);
+  mfma::build_table(tab.data());
+  PQ_HIP(hipMalloc(&h->mtab, sizeof(unsigned) * tab.size()));
+  PQ_HIP(hipMemcpy(h->mtab, tab.data(), sizeof(unsigned) * tab.size(), hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+// Row-pass workgroup of a handle: threads and LDS bytes of pass 0 or 2.
+static int row_threads(const pq_inv *h, int R) {
+  return h->prec == PQ_PREC_FP32_MFMA ? mfma::kRowThreads : (R / 2) * h->ei.tpf;
+}
+static size_t row_lds(const pq_inv *h, int pass, int R) {
+  if (h->prec == PQ_PREC_FP32_MFMA)  // the A3 copy, R rows of N + 4 floats
+    return sizeof(float) * ((size_t)mfma::kA3Lds + (size_t)(h->N + 4) * R);
+  return inv_lds_bytes(h->ei, pass, R / 2);
+}
+
 static bool tiles_ok(const pq_inv *h, int R, int C, bool report) {
   const EngineInfo &e = h->ei;
   const int V = 16 / e.selem;  // 16-byte vectors of the scratch type
@@ -395,8 +486,9 @@ static bool tiles_ok(const pq_inv *h, int R, int C, bool report) {
   };
   if (R < 2 || R > 16 || (R & (R - 1)) || h->N % R) return fail("R must be a power of two in 2..16 dividing N");
   if (R < V) return fail("R below the 16-byte vector width");
-  if ((R / 2) * e.tpf > kThreadsMax) return fail("row workgroup above 512 threads");
-  if (inv_lds_bytes(e, 0, R / 2) > kLdsMax || inv_lds_bytes(e, 2, R / 2) > kLdsMax)
+  if (h->prec == PQ_PREC_FP32_MFMA && R != mfma::kRows) return fail("the matrix-core row passes need R = 16");
+  if (row_threads(h, R) > kThreadsMax) return fail("row workgroup above 512 threads");
+  if (row_lds(h, 0, R) > kLdsMax || row_lds(h, 2, R) > kLdsMax)
     return fail("row workgroup above 160 KiB of LDS");
   if (C < 1 || C > 8 || (C & (C - 1)) || h->N % C) return fail("C must be a power of two in 1..8 dividing N");
   const int groups = (C * h->L + 1) / 2;
@@ -454,8 +546,11 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   PQ_CHECK_ARG(g >= 9 && pq_supported_n(g - 2),
                "pq_inv_create: unsupported grid size g=%d (need g-1 = 2^a or 3*2^a, 8 <= g-1 <= 4096)", g);
   PQ_CHECK_ARG(L >= 1 && L <= 8, "pq_inv_create: layers must be in 1..8 (got %d)", L);
-  PQ_CHECK_ARG(precision >= 0 && precision <= PQ_PREC_FP32_WIDE, "pq_inv_create: bad precision %d",
+  PQ_CHECK_ARG(precision >= 0 && precision <= PQ_PREC_FP32_MFMA, "pq_inv_create: bad precision %d",
                precision);
+  PQ_CHECK_ARG(precision != PQ_PREC_FP32_MFMA || (mfma::supported(g - 1) && !per),
+               "pq_inv_create: PQ_PREC_FP32_MFMA needs the closed basin with g - 1 = %d (got g = %d%s)",
+               mfma::kN, g, per ? ", periodic channel" : "");
   if (precision == PQ_PREC_FP32_AUTO) precision = pq_prec_fp32_choice(L, m2z, z2m);
   PQ_CHECK_ARG(batch_max >= 1, "pq_inv_create: batch_max must be >= 1");
   PQ_CHECK_ARG(ss != nullptr, "pq_inv_create: ss is null");
@@ -500,6 +595,15 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   };
   int rc = h->ei.celem == 8 ? upload_tables<double>(h, h->ei.M) : upload_tables<float>(h, h->ei.M);
   if (rc) return fail(rc);
+  if (precision == PQ_PREC_FP32_MFMA && (rc = upload_mfma_table(h))) return fail(rc);
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      pq_set_error("pq_inv_create: cannot query the device's compute units");
+      return fail(PQ_ERR_HIP);
+    }
+  }
   h->coef_size = CoefLayout{L, N}.size();
   std::vector<double> coef_h(h->coef_size);
   compute_coefs(h, L, ss, m2z, z2m, coef_h.data());
@@ -553,7 +657,8 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
   a.twm = h->twm;
   a.R = h->R;
   a.lgR = ilog2i(h->R);
-  const int rows_threads = (h->R / 2) * e.tpf;
+  a.mtab = h->mtab;
+  const int rows_threads = row_threads(h, h->R);
   const int col_groups = (h->C * L + 1) / 2;
   for (int pass = 0; pass < 3; ++pass) {
     if (!(pass_mask & (1 << pass))) continue;
@@ -569,8 +674,10 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
       lds = inv_lds_bytes(e, 1, col_groups, h->per);
     } else {
       blocks = pt.cum_r[nb];
+      // the matrix-core row passes are persistent: two workgroups per CU
+      if (h->prec == PQ_PREC_FP32_MFMA) blocks = std::min(blocks, 2 * std::max(h->ncu, 1));
       threads = rows_threads;
-      lds = inv_lds_bytes(e, pass, h->R / 2);
+      lds = row_lds(h, pass, h->R);
     }
     if (pass == 2 && h->mode == 0) {
       if (h->per) hipLaunchKernelGGL(k_chan_corr, dim3(nb), dim3(256), 0, s, a, pt, N);
@@ -699,6 +806,7 @@ int pq_inv_destroy(pq_inv *h) {
   (void)hipFree(h->tw64);
   (void)hipFree(h->was);
   (void)hipFree(h->lam4);
+  (void)hipFree(h->mtab);
   (void)hipFree(h->shat);
   (void)hipFree(h->ctab);
   (void)hipFree(h->coef);
