@@ -928,9 +928,11 @@ k_rows_fwd_mfma(InvArgs a, PtrTab ptrs) {
 
 // Lowest input modes of the inverse transform taken out of the matrix-core
 // transform and added to psi in FP64 (as the FP32 engines do): the
-// solution's spectrum is dominated by them, and the transform's rounding
-// error scales with its input norm.
-constexpr int kMfmaLow = 4;
+// solution's spectrum is dominated by them, and the split operands' error
+// scales with the row's largest input.  Two modes take most of the gain
+// (the spectrum falls off like p^-3.5; measured on the 1023^2 workload:
+// 5.1e-7 with none, 1.3e-7 with two, 1.2e-7 with four).
+constexpr int kMfmaLow = 2;
 
 // cos and sin of pi k / 16, k = 0..15 (compile-time after unrolling)
 __device__ __forceinline__ double cospi16(int k) {
@@ -1022,15 +1024,10 @@ k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
           if (m0 || (i != 0 && i != N)) pa[i] = Tout(0.0 - sub);
         continue;
       }
-      // low = s1 P(c1), s1 = sin(pi i / N), c1 = cos(pi i / N), with
-      // sin(2t) = 2 s c, sin(3t) = s (4c^2 - 1), sin(4t) = 4 s c (2c^2 - 1):
-      // P(c) = (w1 - w3) + (2 w2 - 4 w4) c + 4 w3 c^2 + 8 w4 c^3 for the
-      // weights w_p = 2 x_p
-      static_assert(kMfmaLow == 4, "the low-mode polynomial is written for 4 modes");
-      double lw[kMfmaLow];
-#pragma unroll
-      for (int q = 0; q < kMfmaLow; ++q) lw[q] = 2.0 * double(row[1 + q]);
-      const double p0 = lw[0] - lw[2], p1 = 2.0 * lw[1] - 4.0 * lw[3], p2 = 4.0 * lw[2], p3 = 8.0 * lw[3];
+      // low = s1 (w1 + 2 w2 c1), s1 = sin(pi i / N), c1 = cos(pi i / N)
+      // (sin 2t = 2 sin t cos t), for the weights w_p = 2 x_p
+      static_assert(kMfmaLow == 2, "the low-mode sum is written for 2 modes");
+      const double p0 = 2.0 * double(row[1]), p1 = 4.0 * double(row[2]);
       float x[2][8];
       const float *src = row + c + 256 * g;
 #pragma unroll
@@ -1046,7 +1043,7 @@ k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
         const int i = lane + 64 * k;
         const double ca = cospi16(k), sa = sinpi16(k);
         const double s1 = fma(sb, ca, cb * sa), c1 = fma(cb, ca, -sb * sa);
-        const double pc = fma(fma(fma(p3, c1, p2), c1, p1), c1, p0);
+        const double pc = fma(p1, c1, p0);
         const double y = i == 0 ? 0.0 : fma(s1, pc, double(row[i]));
         if (m0 || i != 0) pa[i] = Tout(y - sub);
       }
