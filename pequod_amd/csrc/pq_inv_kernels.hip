@@ -460,11 +460,20 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
       bj = fma(r, bj, z[m][e]);  // B(j0 + e)
       t[e] = (t[e] + bj - z[m][e]) * cg;
     }
-    if (lc == 0) s_bc[0][col] = t[0];
-    if (lc == TPC - 1) s_bc[1][col] = r * fl * cg;  // yp(N): F(N) = r F(N - 1)
-    __syncthreads();
-    const double y0 = s_bc[0][col], yN = s_bc[1][col];
-    __syncthreads();
+    double y0, yN;
+    if constexpr (TPC <= 64) {
+      // the column is one segment of a wavefront: its first and last
+      // lanes' values by shuffles
+      y0 = __shfl(t[0], 0, W);
+      yN = __shfl(r * fl * cg, W - 1, W);  // yp(N): F(N) = r F(N - 1)
+    } else {
+      if (lc == 0) s_bc[0][col] = t[0];
+      if (lc == TPC - 1) s_bc[1][col] = r * fl * cg;  // yp(N): F(N) = r F(N - 1)
+      __syncthreads();
+      y0 = s_bc[0][col];
+      yN = s_bc[1][col];
+      __syncthreads();
+    }
     const double al = (rN * yN - y0) * w, be = (rN * y0 - yN) * w;
     // y(j) = yp(j) + alpha r^j + beta r^(N - j): r^j0 = (r^E)^lc,
     // r^(N - j0) = (r^E)^(TPC - lc)
