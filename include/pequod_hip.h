@@ -53,7 +53,9 @@ extern "C" {
 #define PQ_PREC_MIXED 2   /* FP64 storage, FP32 transforms                 */
 #define PQ_PREC_FP32_STORAGE 3 /* FP32 storage, FP64 transforms             */
 /* FP32 storage; FP32 transforms when the layer <-> mode projection is well
- * conditioned (pq_prec_fp32_choice), else FP64 transforms.                */
+ * conditioned (pq_prec_fp32_choice), else FP64 transforms; the FP32
+ * transforms run on the matrix cores where PQ_PREC_FP32_MFMA exists for
+ * the grid (pq_prec_fp32_auto).                                            */
 #define PQ_PREC_FP32_AUTO 4
 /* FP32 storage (fields, scratch and on-chip buffers), FP64 arithmetic: the
  * transforms round to FP32 only between FFT stages.                        */
@@ -69,6 +71,12 @@ extern "C" {
  * <= 32 (the FP32 transform error can reach psi amplified by it).         */
 int pq_prec_fp32_choice(int layers, const double *modes_to_z,
                         const double *z_to_modes);
+/* What PQ_PREC_FP32_AUTO resolves to for a g x g closed basin:
+ * PQ_PREC_FP32_MFMA where pq_prec_fp32_choice picks FP32 transforms and the
+ * matrix-core transforms exist for g - 1, else pq_prec_fp32_choice's pick.
+ * No counterpart in the reference (its precision is fixed at build time).  */
+int pq_prec_fp32_auto(int g, int layers, const double *modes_to_z,
+                      const double *z_to_modes);
 
 const char *pq_last_error(void);
 /* Library/runtime information: device name and CU count into buf.         */

@@ -13,7 +13,8 @@ PQ_OK, PQ_ERR_ARG, PQ_ERR_HIP, PQ_ERR_NAN, PQ_WARN_LARGE, PQ_ERR_NOMEM = range(6
 # fp64: FP64 storage and transforms; fp32: FP32 storage and transforms;
 # mixed: FP64 storage, FP32 transforms; fp32s: FP32 storage, FP64 transforms
 # fp32auto: FP32 storage, FP32 transforms when the projection is well
-# conditioned (pq_prec_fp32_choice), else FP64 transforms
+# conditioned (pq_prec_fp32_choice; on the matrix cores where fp32m exists
+# for the grid, pq_prec_fp32_auto), else FP64 transforms
 # fp32w: FP32 storage, FP64 arithmetic (FP32 rounding between FFT stages)
 # fp32m: FP32 storage, row transforms on the matrix cores (fp16 hi/lo split
 # operands, FP32 accumulation), g - 1 = 1024
@@ -113,6 +114,7 @@ SIGNATURES = {
     "pq_ens_member_class": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "pq_ens_class_precision": (_i, [_vp, _i]),
     "pq_prec_fp32_choice": (_i, [_i, _dp, _dp]),
+    "pq_prec_fp32_auto": (_i, [_i, _i, _dp, _dp]),
     "pq_ens_stream": (_vp, [_vp]),
     "pq_ens_invert": (_i, [_vp]),
     "pq_ens_step": (_i, [_vp, _i]),

@@ -179,9 +179,9 @@ int pq_ens_finalize(pq_ens *e) {
   std::map<std::tuple<int, long, long, long, long, int>, int> key2cls;
   for (size_t i = 0; i < e->mem.size(); ++i) {
     auto &m = e->mem[i];
-    // FP32_AUTO resolves per member (pq_prec_fp32_choice)
+    // FP32_AUTO resolves per member (pq_prec_fp32_auto)
     const int prec = e->prec == PQ_PREC_FP32_AUTO
-                         ? pq_prec_fp32_choice(m.L, m.m2z.data(), m.z2m.data())
+                         ? pq_prec_fp32_auto(m.g, m.L, m.m2z.data(), m.z2m.data())
                          : e->prec;
     auto key = std::make_tuple(m.g, m.ldz1, m.ldz2, m.ldp1, m.ldp2, prec);
     auto it = key2cls.find(key);

@@ -531,6 +531,11 @@ extern "C" int pq_prec_fp32_choice(int L, const double *m2z, const double *z2m) 
   return norm_inf(m2z) * norm_inf(z2m) <= 32.0 ? PQ_PREC_FP32 : PQ_PREC_FP32_STORAGE;
 }
 
+extern "C" int pq_prec_fp32_auto(int g, int L, const double *m2z, const double *z2m) {
+  const int c = pq_prec_fp32_choice(L, m2z, z2m);
+  return c == PQ_PREC_FP32 && mfma::supported(g - 1) ? PQ_PREC_FP32_MFMA : c;
+}
+
 // Per-member sizes (doubles) of the column-pass partials, psiC and the
 // row-inverse constants.
 static size_t part_per(const pq_inv *h) { return (size_t)h->part_cap * h->L * (h->per ? 2 : 1); }
@@ -551,7 +556,8 @@ static int inv_create(pq_inv **out, int g, int L, const double *ss, const double
   PQ_CHECK_ARG(precision != PQ_PREC_FP32_MFMA || (mfma::supported(g - 1) && !per),
                "pq_inv_create: PQ_PREC_FP32_MFMA needs the closed basin with g - 1 = %d (got g = %d%s)",
                mfma::kN, g, per ? ", periodic channel" : "");
-  if (precision == PQ_PREC_FP32_AUTO) precision = pq_prec_fp32_choice(L, m2z, z2m);
+  if (precision == PQ_PREC_FP32_AUTO)
+    precision = per ? pq_prec_fp32_choice(L, m2z, z2m) : pq_prec_fp32_auto(g, L, m2z, z2m);
   PQ_CHECK_ARG(batch_max >= 1, "pq_inv_create: batch_max must be >= 1");
   PQ_CHECK_ARG(ss != nullptr, "pq_inv_create: ss is null");
   if (mode == 0) {

@@ -133,7 +133,7 @@ class HelmholtzCache:
         m2z = np.asfortranarray(strat.modes_to_z, dtype=np.float64)
         z2m = np.asfortranarray(strat.z_to_modes, dtype=np.float64)
         if precision == "fp32auto":
-            precision = _lib.PREC_NAME[load().pq_prec_fp32_choice(self.layers, _np_ptr(m2z), _np_ptr(z2m))]
+            precision = _lib.PREC_NAME[load().pq_prec_fp32_auto(g, self.layers, _np_ptr(m2z), _np_ptr(z2m))]
         self.precision = precision
         check(load().pq_inv_create(ctypes.byref(h), g, self.layers, _np_ptr(ss), _np_ptr(m2z),
                                    _np_ptr(z2m), PREC[precision], batch_max))
