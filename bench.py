@@ -360,14 +360,28 @@ def serving_rates(torch, dev, log, rate=2000.0, window=1.0, steps=50, seed=11, l
             f"({len(fields)} field members, {len(models)} model members)")
         t0 = time.perf_counter()
         last = first
-        for a, inv, tgt in zip(arrivals, kinds, targets):
+        tick = 10.0
+        for n_sub, (a, inv, tgt) in enumerate(zip(arrivals, kinds, targets)):
             while time.perf_counter() - t0 < a:
                 pass
             if inv:
                 last = srv.submit(fields[tgt], Server.FIELD)
             else:
                 last = srv.submit(models[tgt], Server.MODEL, steps)
-        srv.drain(1200)
+            if a >= tick:  # progress of long windows
+                log(f"[bench] serving: {n_sub + 1} requests submitted after {a:.0f} s")
+                tick += 10.0
+        # drain, reporting progress every 10 s (a backlog forms when the
+        # offered rate exceeds the capacity)
+        from pequod_amd._lib import PqError
+        while True:
+            try:
+                srv.drain(10.0)
+                break
+            except PqError as e:
+                if "pending" not in str(e) or time.perf_counter() - t0 > 1800:
+                    raise
+                log(f"[bench] serving: draining ({e}) at {time.perf_counter() - t0:.0f} s")
         makespan = time.perf_counter() - t0
         out = {"offered_per_s": round(rate, 1), "window_s": window, "requests": int(arrivals.size),
                "served_per_s": round(arrivals.size / makespan, 1)}
