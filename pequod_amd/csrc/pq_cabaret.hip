@@ -26,20 +26,23 @@
 
 namespace {
 
-// 1-based element addresses of the reference's arrays.
+// 1-based element addresses of the reference's arrays.  Offsets are 32-bit
+// (a field of the largest supported grid, (4097 + 3)^2 x 8 values, is below
+// 2^28), so each address is one 32-bit multiply-add and the loads take a
+// scalar base plus a 32-bit vector offset.
 struct Dims {
   int g, L;
-  __device__ __host__ size_t nd(int i, int j, int l) const {
-    return (size_t)(i - 1) + (size_t)(g + 2) * ((size_t)(j - 1) + (size_t)(g + 2) * l);
+  __device__ __host__ unsigned nd(int i, int j, int l) const {
+    return (unsigned)(i - 1) + (unsigned)(g + 2) * ((unsigned)(j - 1) + (unsigned)(g + 2) * (unsigned)l);
   }
-  __device__ __host__ size_t fi(int i, int j, int l) const {
-    return (size_t)(i - 1) + (size_t)(g + 3) * ((size_t)(j - 1) + (size_t)(g + 2) * l);
+  __device__ __host__ unsigned fi(int i, int j, int l) const {
+    return (unsigned)(i - 1) + (unsigned)(g + 3) * ((unsigned)(j - 1) + (unsigned)(g + 2) * (unsigned)l);
   }
-  __device__ __host__ size_t fj(int i, int j, int l) const {
-    return (size_t)(i - 1) + (size_t)(g + 2) * ((size_t)(j - 1) + (size_t)(g + 3) * l);
+  __device__ __host__ unsigned fj(int i, int j, int l) const {
+    return (unsigned)(i - 1) + (unsigned)(g + 2) * ((unsigned)(j - 1) + (unsigned)(g + 3) * (unsigned)l);
   }
-  __device__ __host__ size_t rs(int i, int j, int l) const {
-    return (size_t)(i - 1) + (size_t)g * ((size_t)(j - 1) + (size_t)g * l);
+  __device__ __host__ unsigned rs(int i, int j, int l) const {
+    return (unsigned)(i - 1) + (unsigned)g * ((unsigned)(j - 1) + (unsigned)g * (unsigned)l);
   }
   size_t n_nd() const { return (size_t)(g + 2) * (g + 2) * L; }
   size_t n_fi() const { return (size_t)(g + 3) * (g + 2) * L; }
@@ -89,7 +92,7 @@ __device__ __forceinline__ void pq_tile(int &bx, int &by, int &bz) {
 template <typename TS> struct R {
   const TS *__restrict__ p;
   __host__ __device__ R(const TS *q) : p(q) {}
-  __device__ __forceinline__ double operator[](size_t k) const { return double(p[k]); }
+  __device__ __forceinline__ double operator[](unsigned k) const { return double(p[k]); }
 };
 template <typename TS> struct W {
   TS *__restrict__ p;
@@ -102,7 +105,7 @@ template <typename TS> struct W {
     }
     __device__ __forceinline__ operator double() const { return double(*q); }
   };
-  __device__ __forceinline__ Ref operator[](size_t k) const { return Ref{p + k}; }
+  __device__ __forceinline__ Ref operator[](unsigned k) const { return Ref{p + k}; }
 };
 
 // cabaret_predictor (qg3p_aux.F90:993-1020): f range 3..g
@@ -120,7 +123,7 @@ __global__ void k_beta(Dims d, double dt, double tw_a, const double *__restrict_
                        W<TS> res_beta) {
   PQ_IDX(2, d.g - 2, 2, d.g - 2);
   const double rb = dt * (0.5 * (ui[d.fi(i + 2, j + 1, l)] + ui[d.fi(i + 1, j + 1, l)]) * beta[l]);
-  const size_t z = d.nd(i + 1, j + 1, l);
+  const unsigned z = d.nd(i + 1, j + 1, l);
   zeta[z] = zeta[z] + (1.0 + tw_a) * rb - tw_a * res_beta[d.rs(i, j, l)];
   res_beta[d.rs(i, j, l)] = rb;
 }
@@ -134,7 +137,7 @@ __global__ void k_forcing(Dims d, double dt, const double *__restrict__ fw, doub
   const int L = d.L;
   const double lap = -4.0 * rel[d.nd(i, j, l)] + rel[d.nd(i - 1, j, l)] + rel[d.nd(i + 1, j, l)] +
                      rel[d.nd(i, j - 1, l)] + rel[d.nd(i, j + 1, l)];
-  const double f = fw[(size_t)(i - 2) + (size_t)d.g * (j - 2)];
+  const double f = fw[(unsigned)(i - 2) + (unsigned)d.g * (unsigned)(j - 2)];
   double rv;
   if (L == 1)
     rv = dt * (f - visc_bot * rel[d.nd(i, j, 0)] + visc * lap);
@@ -183,7 +186,7 @@ __global__ void k_zeta_step(Dims d, double T, const double *__restrict__ beta,
                      q(i + 1, j) * (P(i + 1, j + 1) - P(i + 1, j - 1));
   const double lap = -4.0 * rel_p[d.nd(i, j, l)] + rel_p[d.nd(i - 1, j, l)] +
                      rel_p[d.nd(i + 1, j, l)] + rel_p[d.nd(i, j - 1, l)] + rel_p[d.nd(i, j + 1, l)];
-  const double f0 = fw[(size_t)(i - 2) + (size_t)d.g * (j - 2)];
+  const double f0 = fw[(unsigned)(i - 2) + (unsigned)d.g * (unsigned)(j - 2)];
   double f;
   if (L == 1)
     f = 0.25 * jac + f0 + visc * (lap - visc_bot * rel_p[d.nd(i, j, 0)]);
@@ -206,7 +209,7 @@ __global__ void k_ra_filter(Dims d, W<TS> phi_l, W<TS> phi_m,
   const double ra_nu = 0.02, ra_alpha = 0.5;
   const double w21 = 0.5 * ra_nu * ra_alpha, w22 = 1.0 - 2.0 * w21;
   const double w11 = -0.5 * ra_nu * (1.0 - ra_alpha), w12 = -2.0 * w11, w13 = 1.0 + w11;
-  const size_t k = d.nd(i, j, l);
+  const unsigned k = d.nd(i, j, l);
   const double lphi_l = phi_l[k], old = phi_old_l[k], pm = phi_m[k];
   phi_l[k] = w11 * old + w12 * pm + w13 * lphi_l;
   const double nm = w21 * (old + lphi_l) + w22 * pm;
@@ -231,13 +234,13 @@ __global__ void k_cell_boundaries(Dims d, double tw, R<TS> psi,
   const double pe = psi_edge(d, psi, i, j, l);
   if (j <= g) {
     const double uim = -psi_edge(d, psi, i, j + 1, l) + pe;
-    const size_t a = d.fi(i, j, l);
+    const unsigned a = d.fi(i, j, l);
     ui[a] = (1.0 + tw) * uim - tw * uimo[a];
     uimo[a] = uim;
   }
   if (i <= g) {
     const double ujm = +psi_edge(d, psi, i + 1, j, l) - pe;
-    const size_t a = d.fj(i, j, l);
+    const unsigned a = d.fj(i, j, l);
     uj[a] = (1.0 + tw) * ujm - tw * ujmo[a];
     ujmo[a] = ujm;
   }
@@ -312,7 +315,7 @@ __global__ void k_extrapolator(Dims d, double dt, R<TS> zeta,
   PQ_IDX(3, d.g - 1, 3, d.g - 1);
   const int g = d.g;
   if (j <= g) {  // i-face (i, j), i in 3..g+1
-    const size_t a = d.fi(i, j, l);
+    const unsigned a = d.fi(i, j, l);
     const bool lo = !(i > 3), hi = !(i < g + 1);
     const double u1 = ui[a], fi1 = zfi[a];
     const double um = lo ? 0.0 : ui[d.fi(i - 1, j, l)], fm = lo ? 0.0 : zfi[d.fi(i - 1, j, l)];
@@ -322,7 +325,7 @@ __global__ void k_extrapolator(Dims d, double dt, R<TS> zeta,
                          zeta_old[d.nd(i, j, l)], lo, hi);
   }
   if (i <= g) {  // j-face (i, j), j in 3..g+1
-    const size_t a = d.fj(i, j, l);
+    const unsigned a = d.fj(i, j, l);
     const bool lo = !(j > 3), hi = !(j < g + 1);
     const double u1 = uj[a], fi1 = zfj[a];
     const double um = lo ? 0.0 : uj[d.fj(i, j - 1, l)], fm = lo ? 0.0 : zfj[d.fj(i, j - 1, l)];
@@ -375,7 +378,7 @@ __global__ void k_corrector_to(Dims d, double dt, R<TS> flux_i,
                                R<TS> zeta_in, W<TS> zeta_out) {
   PQ_IDX(1, d.g + 2, 1, d.g + 2);
   const int g = d.g;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   if (i >= 3 && i <= g && j >= 3 && j <= g) {
     const double r = -(flux_i[d.fi(i + 1, j, l)] - flux_i[d.fi(i, j, l)] +
                        flux_j[d.fj(i, j + 1, l)] - flux_j[d.fj(i, j, l)]);
@@ -400,7 +403,7 @@ __global__ void k_post_extrap_flux(Dims d, double dt, R<TS> zeta, R<TS> zeta_old
   PQ_IDX(2, d.g + 1, 2, d.g + 1);
   const int g = d.g;
   if (j >= 3 && j <= g) {  // i-face (i, j), i in 2..g+2
-    const size_t a = d.fi(i, j, l);
+    const unsigned a = d.fi(i, j, l);
     double f;
     if (i >= 3 && i <= g + 1) {
       const bool lo = !(i > 3), hi = !(i < g + 1);
@@ -418,7 +421,7 @@ __global__ void k_post_extrap_flux(Dims d, double dt, R<TS> zeta, R<TS> zeta_old
     flux_i[a] = ui[a] * f;
   }
   if (i >= 3 && i <= g) {  // j-face (i, j), j in 2..g+2
-    const size_t a = d.fj(i, j, l);
+    const unsigned a = d.fj(i, j, l);
     double f;
     if (j >= 3 && j <= g + 1) {
       const bool lo = !(j > 3), hi = !(j < g + 1);
@@ -445,7 +448,7 @@ template <typename A>
 __device__ __forceinline__ double rel_bc(const Dims &d, int bc_type, double alpha, A psi, int i,
                                          int j, int l) {
   const int g = d.g;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   const bool jb = (j == 2 || j == g + 1);
   double r;
   if (bc_type == 1) {
@@ -475,7 +478,7 @@ __device__ __forceinline__ double rel_node(const Dims &d, const double *__restri
                                            const double *__restrict__ s2, int bc_type, double alpha,
                                            R<TS> zeta, R<TS> psi, int i, int j, int l) {
   const int g = d.g, L = d.L;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   double r;
   const bool jb = (j == 2 || j == g + 1), ib = (i == 2 || i == g + 1);
   if (jb || ib) {
@@ -507,7 +510,7 @@ __global__ void k_rel_from_psi(Dims d, int bc_type, double alpha, R<TS> psi,
                                W<TS> rel) {
   PQ_IDX(2, d.g, 2, d.g);
   const int g = d.g;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   if (i == 2 || i == g + 1 || j == 2 || j == g + 1)
     rel[a] = rel_bc(d, bc_type, alpha, psi, i, j, l);
   else
@@ -522,7 +525,7 @@ __device__ __forceinline__ double zeta_bc_node(const Dims &d, const double *__re
                                                const double *__restrict__ s2, double rel, R<TS> psi,
                                                int i, int j, int l) {
   const int L = d.L;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   double z;
   if (L == 1) z = rel;
   else if (l == 0) z = rel - s2[0] * (psi[a] - psi[d.nd(i, j, 1)]);
@@ -539,7 +542,7 @@ __global__ void k_zeta_bc(Dims d, const double *__restrict__ s1, const double *_
   PQ_IDX(2, d.g, 2, d.g);
   const int g = d.g;
   if (!(i == 2 || i == g + 1 || j == 2 || j == g + 1)) return;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   zeta[a] = zeta_bc_node(d, s1, s2, rel[a], psi, i, j, l);
 }
 
@@ -557,7 +560,7 @@ __global__ void k_post_rel_cells(Dims d, const double *__restrict__ s1,
                                  W<TS> uimo, W<TS> ujmo) {
   PQ_IDX(1, d.g + 2, 1, d.g + 2);
   const int g = d.g;
-  const size_t a = d.nd(i, j, l);
+  const unsigned a = d.nd(i, j, l);
   if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1) {
     const TS r = TS(rel_node(d, s1, s2, bc_type, alpha, R<TS>(zeta.p), psi, i, j, l));
     rel[a] = double(r);
@@ -570,13 +573,13 @@ __global__ void k_post_rel_cells(Dims d, const double *__restrict__ s1,
     const double pe = psi_edge(d, psi, i, j, l);
     if (j <= g) {
       const double uim = -psi_edge(d, psi, i, j + 1, l) + pe;
-      const size_t f = d.fi(i, j, l);
+      const unsigned f = d.fi(i, j, l);
       ui[f] = (1.0 + tw) * uim - tw * uimo[f];
       uimo[f] = uim;
     }
     if (i <= g) {
       const double ujm = +psi_edge(d, psi, i + 1, j, l) - pe;
-      const size_t f = d.fj(i, j, l);
+      const unsigned f = d.fj(i, j, l);
       uj[f] = (1.0 + tw) * ujm - tw * ujmo[f];
       ujmo[f] = ujm;
     }
@@ -649,7 +652,7 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
     const int i = i0 - 1 + a, j = j0 - 1 + b;
     TS pv = TS(0), zv = TS(0);
     if (i >= 1 && i <= g + 2 && j >= 1 && j <= g + 2) {
-      const size_t k = d.nd(i, j, l);
+      const unsigned k = d.nd(i, j, l);
       pv = psi.p[k];
       zv = zeta.p[k];
     }
@@ -692,7 +695,7 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
   for (int b = threadIdx.y; b < TYS; b += BY) {
     const int j = j0 + b;
     if (j > g + 2) break;
-    const size_t a = d.nd(i, j, l);
+    const unsigned a = d.nd(i, j, l);
     const int cb = b + 1;
     const double z = s_zeta[cb][ca];
     if (i >= 2 && i <= g + 1 && j >= 2 && j <= g + 1) {
@@ -703,7 +706,7 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
       } else {
         const double lap = -4.0 * rs[cb][ca] + rs[cb][ca - 1] + rs[cb][ca + 1] + rs[cb - 1][ca] +
                            rs[cb + 1][ca];
-        const double f = fw[(size_t)(i - 2) + (size_t)g * (j - 2)];
+        const double f = fw[(unsigned)(i - 2) + (unsigned)g * (unsigned)(j - 2)];
         double rv;
         if (L == 1)
           rv = dt * (f - visc_bot * r + visc * lap);
@@ -723,13 +726,13 @@ k_step_rel_force(Dims d, const double *__restrict__ s1, const double *__restrict
       const double pe = pedge(cb, 0, 0);
       if (j <= g) {
         const double uim = -pedge(cb, 0, 1) + pe;
-        const size_t f = d.fi(i, j, l);
+        const unsigned f = d.fi(i, j, l);
         ui[f] = (1.0 + tw) * uim - tw * uimo[f];
         uimo[f] = uim;
       }
       if (i <= g) {
         const double ujm = +pedge(cb, 1, 0) - pe;
-        const size_t f = d.fj(i, j, l);
+        const unsigned f = d.fj(i, j, l);
         uj[f] = (1.0 + tw) * ujm - tw * ujmo[f];
         ujmo[f] = ujm;
       }
@@ -772,7 +775,7 @@ k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> 
     const int i = i0 - 1 + a, j = j0 - 1 + b;
     TS x = TS(0), y = TS(0);
     if (i >= 1 && i <= g + 2 && j >= 1 && j <= g + 2) {
-      const size_t k = d.nd(i, j, l);
+      const unsigned k = d.nd(i, j, l);
       x = zf.p[k];
       y = zold[k];
     }
@@ -784,7 +787,7 @@ k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> 
     const int i = i0 - 1 + a, j = j0 + b;
     TS x = TS(0), y = TS(0);
     if (i >= 1 && i <= g + 3 && j <= g + 2) {
-      const size_t k = d.fi(i, j, l);
+      const unsigned k = d.fi(i, j, l);
       x = ui.p[k];
       y = fic[k];
     }
@@ -796,7 +799,7 @@ k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> 
     const int i = i0 + a, j = j0 - 1 + b;
     TS x = TS(0), y = TS(0);
     if (i <= g + 2 && j >= 1 && j <= g + 3) {
-      const size_t k = d.fj(i, j, l);
+      const unsigned k = d.fj(i, j, l);
       x = uj.p[k];
       y = fjc[k];
     }
@@ -863,7 +866,7 @@ k_step_extrap_correct(Dims d, double dt, R<TS> zf, Pair<TS> zo, R<TS> ui, R<TS> 
   for (int b = threadIdx.y; b < TYS; b += BY) {
     const int j = j0 + b;
     if (j > g + 2) break;
-    const size_t a = d.nd(i, j, l);
+    const unsigned a = d.nd(i, j, l);
     const int c = threadIdx.x;
     double v;
     if (i >= 3 && i <= g && j >= 3 && j <= g) {
