@@ -601,7 +601,7 @@ template <typename TS> struct Pair {
 
 // Tile of the step kernels: TX nodes along i (one per thread column) by TY
 // rows of j, 64 x 4 threads, layer blockIdx.z (remapped by pq_tile).
-constexpr int TX = 64, TYS = 8;
+constexpr int TX = 64, TYS = 4;
 static_assert(TX == BX && BX * BY == 256, "the step kernels take one thread column per node column");
 inline dim3 step_grid(int g, int L) { return dim3((g + 2 + TX - 1) / TX, (g + 2 + TYS - 1) / TYS, L); }
 
