@@ -315,17 +315,61 @@ __device__ __forceinline__ void spectral_step(const InvArgs &a, typename Eng::T 
 // sum_p shat_p sum_j y_m(p, j) and writes x_l = sum_m M(l,m) y_m back in
 // place.  Grid: per member N / CW workgroups of CW columns x N / E threads.
 
-// x^k for 0 <= k < KMAX by squaring (a fixed number of steps, so that
-// lanes with different k stay converged).
+// A thread's E rows of one layer in groups of V = RunVec<Ts, E>::V
+// elements (16 or 8 bytes, the widest dividing the run), each group one
+// vector access.  With V dividing both E and the block height R, a group
+// never crosses a block of R rows and starts at a multiple of V elements
+// (runs start at multiples of E, blocks and layers at multiples of R and
+// N^2 elements).  at(j) is the element index of row j.
+template <typename Ts, int E> struct RunVec {
+  static constexpr int B = (E * (int)sizeof(Ts)) % 16 == 0 ? 16 : ((E * (int)sizeof(Ts)) % 8 == 0 ? 8 : (int)sizeof(Ts));
+  static constexpr int V = B / (int)sizeof(Ts);
+  typedef Ts type __attribute__((ext_vector_type(V)));
+};
+template <typename Ts, int E, typename At>
+__device__ __forceinline__ void load_run(const Ts *base, At at, int j0, bool grouped, double (&x)[E]) {
+  using RV = RunVec<Ts, E>;
+  if (RV::V > 1 && grouped) {
+#pragma unroll
+    for (int k = 0; k < E / RV::V; ++k) {
+      const typename RV::type v = *reinterpret_cast<const typename RV::type *>(base + at(j0 + k * RV::V));
+#pragma unroll
+      for (int q = 0; q < RV::V; ++q) x[k * RV::V + q] = double(v[q]);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = double(base[at(j0 + e)]);
+  }
+}
+template <typename Ts, int E, typename At>
+__device__ __forceinline__ void store_run(Ts *base, At at, int j0, bool grouped, const double (&o)[E]) {
+  using RV = RunVec<Ts, E>;
+  if (RV::V > 1 && grouped) {
+#pragma unroll
+    for (int k = 0; k < E / RV::V; ++k) {
+      typename RV::type v;
+#pragma unroll
+      for (int q = 0; q < RV::V; ++q) v[q] = Ts(o[k * RV::V + q]);
+      *reinterpret_cast<typename RV::type *>(base + at(j0 + k * RV::V)) = v;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) base[at(j0 + e)] = Ts(o[e]);
+  }
+}
+
+// x^k1 and x^k2 for 0 <= k1, k2 < KMAX by squaring, the squarings shared (a
+// fixed number of steps, so that lanes with different k stay converged).
 template <int KMAX>
-__device__ __forceinline__ double powu(double x, int k) {
-  double r = 1.0;
+__device__ __forceinline__ void powu2(double x, int k1, int k2, double &r1, double &r2) {
+  r1 = 1.0;
+  r2 = 1.0;
 #pragma unroll
   for (int bit = 1; bit < KMAX; bit <<= 1) {
-    if (k & bit) r *= x;
+    if (k1 & bit) r1 *= x;
+    if (k2 & bit) r2 *= x;
     x *= x;
   }
-  return r;
 }
 template <int W>
 __device__ __forceinline__ double seg_shfl_up(double v, int d) {
@@ -336,21 +380,21 @@ __device__ __forceinline__ double seg_shfl_down(double v, int d) {
   return __shfl_down(v, d, W);
 }
 
-// N is a template argument although every kernel unit has one length: the
-// units are linked into one library, and a kernel's symbol must differ per
-// length (otherwise the units' launches all resolve to one unit's kernel).
+// The column pass of one member of L <= LT layers (LT fixes the thread
+// geometry).  Mode by mode, the column's wavefronts meet once (one barrier)
+// for their scan totals, which also give the wall values; each mode has its
+// own LDS slots, so no barrier guards their reuse.  The layer and mode loops
+// stop at the member's runtime L (the branches also bound how far the
+// scheduler interleaves the unrolled modes, which keeps the registers near
+// one mode's working set).
 template <typename Ts, int LT, int N>
-__global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
+__device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &ptrs, int b, int pblk, int lgC,
+                                              double *red, double *s_tot, double *s_tc, int L) {
   constexpr int E = tri_e(N, LT), TPC = N / E;
   constexpr int W = TPC < 64 ? TPC : 64;  // lanes of a column inside one wavefront
   constexpr int NWC = TPC / W;            // wavefronts per column
-  __shared__ double red[64];
-  __shared__ double s_tot[2][8];          // per wavefront: scan totals (forward, backward)
-  __shared__ double s_bc[2][256];         // per column: yp(0) and yp(N)
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
-  const int b = member_of(ptrs.cum_c, ptrs.nb, bid);
-  const int L = ptrs.lay[b], lgC = ptrs.lgc[b];
-  const int pblk = bid - ptrs.cum_c[b];
+  // s_tot[k][m][wavefront] (k = 0 forward, 1 backward): scan totals
+  auto tot = [&](int k, int m, int q) -> double & { return s_tot[(k * 8 + m) * 8 + q]; };
   const int col = threadIdx.x / TPC, lc = threadIdx.x % TPC;
   const int p = (pblk << lgC) + col;
   const int j0 = lc * E;
@@ -362,12 +406,21 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
   Ts *scr = reinterpret_cast<Ts *>(a.scr + b * a.scr_stride);
   const size_t cbase = (size_t)p * R;
 
-  // the thread's rows of layer l: contiguous when they lie in one block
-  // (E divides R), else element by element
-  const bool contig = E <= R && (R % E) == 0;
+  // the thread's rows of layer l: vector groups when the group size
+  // divides the block height, else element by element
+  const bool grouped = (R % RunVec<Ts, E>::V) == 0;
   auto elem = [&](int l, int j) -> size_t {
     return (size_t)l * N * N + (size_t)(j >> lgR) * N * R + cbase + (j & (R - 1));
   };
+
+  // The modes' constants of the workgroup's columns into LDS, their loads
+  // in flight with the field's (published by the barrier after the
+  // projection): s_tc[(m C + c) 6 + k] = k-th constant of mode m, column c.
+  constexpr int C = tri_cols(N, LT);
+  for (int t = threadIdx.x; t < L * C * 6; t += C * TPC) {
+    const int k = t % 6, mc = t / 6;
+    s_tc[t] = coef[CL.tri() + ((size_t)(mc / C) * N + (pblk << lgC) + mc % C) * 8 + k];
+  }
 
   // right-hand sides of the modes
   double z[LT][E];
@@ -380,14 +433,7 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
   for (int l = 0; l < LT; ++l) {
     if (l >= L) break;
     double x[E];
-    if (contig) {
-      const Ts *src = scr + elem(l, j0);
-#pragma unroll
-      for (int e = 0; e < E; ++e) x[e] = double(src[e]);
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) x[e] = double(scr[elem(l, j0 + e)]);
-    }
+    load_run<Ts, E>(scr, [&](int j) { return elem(l, j); }, j0, grouped, x);
 #pragma unroll
     for (int m = 0; m < LT; ++m) {
       if (m >= L) break;
@@ -396,98 +442,117 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
       for (int e = 0; e < E; ++e) z[m][e] = fma(d, x[e], z[m][e]);
     }
   }
+  __syncthreads();  // s_tc
 
   const double sp = a.shat[p];  // 0 for even p
-  double acc[8];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) acc[m] = 0.0;
+  double acc[LT];
 #pragma unroll
   for (int m = 0; m < LT; ++m) {
     if (m >= L) break;
-    const double *tc = coef + CL.tri() + ((size_t)m * N + p) * 8;
+    const double *tc = s_tc + (m * C + col) * 6;
     const double r = tc[0], ri = tc[1], cg = tc[2], rN = tc[3], w = tc[4], rE = tc[5];
-    // forward recurrence inside the thread (zero carry-in); the backward
-    // one only for its value at the thread's first row
-    double t[E];
-    double f = 0.0, bb = 0.0;
+    double cf, cb, ft = 0.0, bt = 0.0;
+    {
+      // Scans: the thread's recurrence totals (zero
+      // carry-in), inclusive affine scans inside the column's lanes of this
+      // wavefront (forward from lane 0 up, backward from the last lane
+      // down), and the in-wavefront carries: F just before this thread's
+      // rows, B just after.  A column inside one wavefront also takes its
+      // totals F(N - 1) (last lane) and B(0) (lane 0) here; wider columns
+      // publish each wavefront's totals.
+      double f = 0.0, bb = 0.0;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      f = fma(r, f, z[m][e]);
-      t[e] = f;
-    }
+      for (int e = 0; e < E; ++e) f = fma(r, f, z[m][e]);
 #pragma unroll
-    for (int e = E - 1; e >= 0; --e) bb = fma(r, bb, z[m][e]);
-    // inclusive affine scans inside the column's lanes of this wavefront:
-    // forward (from lane 0 up) and backward (from the last lane down)
-    double vf = f, vb = bb, mul = rE;
+      for (int e = E - 1; e >= 0; --e) bb = fma(r, bb, z[m][e]);
+      double vf = f, vb = bb, mul = rE;
 #pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-      const double uf = seg_shfl_up<W>(vf, d), ub = seg_shfl_down<W>(vb, d);
-      if (lw >= d) vf = fma(mul, uf, vf);
-      if (lw + d < W) vb = fma(mul, ub, vb);
-      mul *= mul;
+      for (int d = 1; d < W; d <<= 1) {
+        const double uf = seg_shfl_up<W>(vf, d), ub = seg_shfl_down<W>(vb, d);
+        if (lw >= d) vf = fma(mul, uf, vf);
+        if (lw + d < W) vb = fma(mul, ub, vb);
+        mul *= mul;
+      }
+      const double pf = seg_shfl_up<W>(vf, 1), pb = seg_shfl_down<W>(vb, 1);
+      cf = lw == 0 ? 0.0 : pf;
+      cb = lw == W - 1 ? 0.0 : pb;
+      if constexpr (NWC > 1) {
+        if (lw == W - 1) tot(0, m, wv) = vf;
+        if (lw == 0) tot(1, m, wv) = vb;
+        __syncthreads();  // the mode's wavefront totals (a slot per mode: no reuse)
+      } else {
+        ft = __shfl(vf, W - 1, W);
+        bt = __shfl(vb, 0, W);
+      }
     }
-    // carries: F just before this thread's rows, B just after them
-    const double pf = seg_shfl_up<W>(vf, 1), pb = seg_shfl_down<W>(vb, 1);
-    double cf = lw == 0 ? 0.0 : pf, cb = lw == W - 1 ? 0.0 : pb;
-    if constexpr (NWC > 1) {
-      // across the column's wavefronts: totals through LDS, combined in order
-      if (lw == W - 1) s_tot[0][wv] = vf;
-      if (lw == 0) s_tot[1][wv] = vb;
-      __syncthreads();
-      const double rW = mul;  // r^(E W)
-      const int w0 = wv - wc;  // the column's first wavefront
-      double cwf = 0.0, cwb = 0.0;
-      for (int q = 0; q < wc; ++q) cwf = fma(rW, cwf, s_tot[0][w0 + q]);
-      for (int q = NWC - 1; q > wc; --q) cwb = fma(rW, cwb, s_tot[1][w0 + q]);
-      // the wavefront's carry reaches lane lw through r^(E lw) (forward) and
-      // r^(E (W - 1 - lw)) (backward)
-      cf = fma(powu<W>(rE, lw), cwf, cf);
-      cb = fma(powu<W>(rE, W - 1 - lw), cwb, cb);
-      __syncthreads();
-    }
-    // full recurrences, then the particular solution in place of F
-    double rp = r;
+    {
+      // Then the carries across the column's wavefronts
+      // and the column's totals (wavefront totals combined in order), the
+      // full recurrences, the particular solution yp(j) = (F(j) + B(j) -
+      // z(j)) / (r - 1/r), and the walls: y = yp + alpha r^j + beta r^(N - j)
+      // with yp(0) + alpha + beta r^N = 0 and yp(N) + alpha r^N + beta = 0,
+      // where yp(0) = B(0) / (r - 1/r) (F(0) = z(0)) and yp(N) =
+      // r F(N - 1) / (r - 1/r) (z(N) = 0); r^j0 = (r^E)^lc and r^(N - j0) =
+      // (r^E)^(TPC - lc) from the carries' powers r^(E lw),
+      // r^(E (W - 1 - lw)) and those of r^(E W).
+      double cfm = cf, cbm = cb, fN = ft, b0 = bt;
+      double pl, pr;
+      powu2<W>(rE, lw, W - 1 - lw, pl, pr);
+      double ru = pl, rd = pr * rE;
+      if constexpr (NWC > 1) {
+        double rW = rE;  // r^(E W)
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      t[e] = fma(rp, cf, t[e]);  // F(j0 + e)
-      rp *= r;
-    }
-    const double fl = t[E - 1];  // F(N - 1) on the column's last thread
-    double bj = cb;
+        for (int d = 1; d < W; d <<= 1) rW *= rW;
+        const int w0 = wv - wc;  // the column's first wavefront
+        double cwf = 0.0, cwb = 0.0;
+        fN = 0.0;
+        b0 = 0.0;
 #pragma unroll
-    for (int e = E - 1; e >= 0; --e) {
-      bj = fma(r, bj, z[m][e]);  // B(j0 + e)
-      t[e] = (t[e] + bj - z[m][e]) * cg;
-    }
-    double y0, yN;
-    if constexpr (TPC <= 64) {
-      // the column is one segment of a wavefront: its first and last
-      // lanes' values by shuffles
-      y0 = __shfl(t[0], 0, W);
-      yN = __shfl(r * fl * cg, W - 1, W);  // yp(N): F(N) = r F(N - 1)
-    } else {
-      if (lc == 0) s_bc[0][col] = t[0];
-      if (lc == TPC - 1) s_bc[1][col] = r * fl * cg;  // yp(N): F(N) = r F(N - 1)
-      __syncthreads();
-      y0 = s_bc[0][col];
-      yN = s_bc[1][col];
-      __syncthreads();
-    }
-    const double al = (rN * yN - y0) * w, be = (rN * y0 - yN) * w;
-    // y(j) = yp(j) + alpha r^j + beta r^(N - j): r^j0 = (r^E)^lc,
-    // r^(N - j0) = (r^E)^(TPC - lc)
-    double ru = powu<TPC + 1>(rE, lc), rd = powu<TPC + 1>(rE, TPC - lc);
-    double sum = 0.0;
+        for (int q = 0; q < NWC; ++q) {
+          if (q == wc) cwf = fN;
+          fN = fma(rW, fN, tot(0, m, w0 + q));
+        }
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const double y = fma(al, ru, fma(be, rd, t[e]));
-      z[m][e] = (j0 + e == 0) ? 0.0 : y;  // Dirichlet row j = 0
-      sum += z[m][e];
-      ru *= r;
-      rd *= ri;
+        for (int q = NWC - 1; q >= 0; --q) {
+          if (q == wc) cwb = b0;
+          b0 = fma(rW, b0, tot(1, m, w0 + q));
+        }
+        // the wavefront's carry reaches lane lw through r^(E lw) (forward)
+        // and r^(E (W - 1 - lw)) (backward)
+        cfm = fma(pl, cwf, cfm);
+        cbm = fma(pr, cwb, cbm);
+        double qa, qb;  // r^(E W wc), r^(E W (NWC - 1 - wc))
+        powu2<NWC>(rW, wc, NWC - 1 - wc, qa, qb);
+        ru *= qa;
+        rd *= qb;
+      }
+      const double y0 = b0 * cg, yN = r * fN * cg;
+      const double al = (rN * yN - y0) * w, be = (rN * y0 - yN) * w;
+      double t[E];
+      double f = 0.0, rp = r;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        f = fma(r, f, z[m][e]);
+        t[e] = fma(rp, cfm, f);  // F(j0 + e)
+        rp *= r;
+      }
+      double bj = cbm;
+#pragma unroll
+      for (int e = E - 1; e >= 0; --e) {
+        bj = fma(r, bj, z[m][e]);  // B(j0 + e)
+        t[e] = (t[e] + bj - z[m][e]) * cg;
+      }
+      double sum = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double y = fma(al, ru, fma(be, rd, t[e]));
+        z[m][e] = (j0 + e == 0) ? 0.0 : y;  // Dirichlet row j = 0
+        sum += z[m][e];
+        ru *= r;
+        rd *= ri;
+      }
+      acc[m] = sp * sum;
     }
-    acc[m] = sp * sum;
   }
 
   // psiC partials of this workgroup's columns (fixed-order FP64 sum)
@@ -512,15 +577,25 @@ __global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
 #pragma unroll
       for (int e = 0; e < E; ++e) o[e] = fma(mm, z[m][e], o[e]);
     }
-    if (contig) {
-      Ts *dst = scr + elem(l, j0);
-#pragma unroll
-      for (int e = 0; e < E; ++e) dst[e] = Ts(o[e]);
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) scr[elem(l, j0 + e)] = Ts(o[e]);
-    }
+    store_run<Ts, E>(scr, [&](int j) { return elem(l, j); }, j0, grouped, o);
   }
+}
+
+// N is a template argument although every kernel unit has one length: the
+// units are linked into one library, and a kernel's symbol must differ per
+// length (otherwise the units' launches all resolve to one unit's kernel).
+// LT fixes the thread geometry (tri_e); members of a ragged batch may have
+// fewer layers than LT.
+template <typename Ts, int LT, int N>
+__global__ void __launch_bounds__(512) k_cols_tri(InvArgs a, PtrTab ptrs) {
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int b = member_of(ptrs.cum_c, ptrs.nb, bid);
+  const int L = ptrs.lay[b], lgC = ptrs.lgc[b];
+  const int pblk = bid - ptrs.cum_c[b];
+  __shared__ double red[64];
+  __shared__ double s_tot[2 * 8 * 8];
+  __shared__ double s_tc[LT * tri_cols(N, LT) * 6];
+  cols_tri_body<Ts, LT, N>(a, ptrs, b, pblk, lgC, red, s_tot, s_tc, L);
 }
 
 // ------------------------------------------------------------ K2 channel
