@@ -36,11 +36,11 @@ struct CoefLayout {
 };
 
 // Tridiagonal column pass: each thread solves E consecutive rows j of one
-// column for every mode, so a column takes N / E threads.  E = 8 (4 above
-// 4 layers, for registers) for N = 2^a, 12 (6) for N = 3 * 2^a, and never
+// column for every mode, so a column takes N / E threads.  E = 8 for
+// N = 2^a; 12 for N = 3 * 2^a (6 above 4 layers, for registers).  Never
 // more than 512 threads per column.
 __host__ __device__ constexpr int tri_e(int N, int LT) {
-  return N % 3 == 0 ? (LT <= 4 ? 12 : 6) : (LT <= 4 || N >= 4096 ? 8 : 4);
+  return N % 3 == 0 ? (LT <= 4 ? 12 : 6) : 8;
 }
 // Layer count the kernel is instantiated for: exact up to 4, else 8.
 __host__ __device__ constexpr int tri_lt(int L) { return L <= 4 ? L : 8; }
