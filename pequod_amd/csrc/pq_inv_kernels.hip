@@ -733,24 +733,37 @@ __global__ void __launch_bounds__(512) k_cols_chan(InvArgs a, PtrTab ptrs) {
 // shat_p / 2N g_mp(j) from the coefficient table (rho = ln r, gamma
 // (scaled by beta_lm here), eta, r per mode and odd p), by recurrence over
 // the tile's R rows in FP64.  sub(r, p, v) subtracts v from element
-// (r, p).  All threads of the workgroup call it (tid, nt).
+// (r, p).  All threads of the workgroup call it (tid, nt).  The terms in
+// r^j are dropped below e^-cut of the constant part (cut = 40 where fields
+// and transforms are FP64, 20 where either is FP32: e^-20 = 2e-9, far under
+// FP32 rounding).
 template <int NN, typename Sub>
 __device__ __forceinline__ void mass_correction(const double *tab, const double (&beta)[8], int L,
                                                 int jb, int NB, int R, int lgR, int tid, int nt,
-                                                Sub sub) {
+                                                double cut, Sub sub) {
   const int j0 = jb << lgR;
   for (int h = tid; h < NN / 2; h += nt) {
     const int p = 2 * h + 1;
     double acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+    // each mode's constants loaded one mode ahead (the index clamped to
+    // the table)
+    double2 n01 = reinterpret_cast<const double2 *>(tab + (size_t)h * 4)[0];
+    double2 n23 = reinterpret_cast<const double2 *>(tab + (size_t)h * 4)[1];
     for (int m = 1; m < L; ++m) {
-      const double *e = tab + ((size_t)(m - 1) * (NN / 2) + h) * 4;
-      const double rho = e[0], gam = beta[m] * e[1], eta = e[2], rr = e[3];
-      // r^j for the tile's rows is below e^-40 (relative 4e-18) far from
-      // both edges: then only the constant part remains
+      const double2 c01 = n01, c23 = n23;
+      {
+        const double2 *e =
+            reinterpret_cast<const double2 *>(tab + ((size_t)(min(m + 1, L - 1) - 1) * (NN / 2) + h) * 4);
+        n01 = e[0];
+        n23 = e[1];
+      }
+      const double rho = c01.x, gam = beta[m] * c01.y, eta = c23.x, rr = c23.y;
+      // r^j for the tile's rows is below e^-cut far from both edges: then
+      // only the constant part remains
       const int jmin = min(j0, NN - j0 - R + 1);
-      if (double(jmin) * -rho > 40.0) {
+      if (double(jmin) * -rho > cut) {
 #pragma unroll
         for (int r = 0; r < 16; ++r)
           if (r < R) acc[r] += gam;
@@ -879,7 +892,7 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
   if (a.mode == 0 && !a.per && L > 1) {
     mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, lgR, threadIdx.x, nt,
-                       [&](int r, int p, double v) {
+                       sizeof(T) == 4 || sizeof(Tout) == 4 ? 20.0 : 40.0, [&](int r, int p, double v) {
                          T *x = row_ptr<Eng>(lds, r) + pad16(p);
                          *x = T(double(*x) - v);
                        });
@@ -1080,7 +1093,7 @@ k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
     }
     __syncthreads();  // publishes the tile (and the A3 copy)
     if (m0 && L > 1) {
-      mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, ilog2c(R), tid, NT,
+      mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, ilog2c(R), tid, NT, 20.0,
                          [&](int r, int p, double x) {
                            float *e = st + r * RS + p;
                            *e = float(double(*e) - x);
