@@ -97,7 +97,8 @@ struct InvArgs {
   int part_stride;
   double *psic;      // psic_stride per member: psiC of the last run
   int psic_stride;
-  double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m;
+  double *kb;        // per member: kappa_l (L), then beta_lm at L + l L + m,
+                     // then rmin[N/2] and Gamma_l[N/2] (l = 0..L-1; k_psic);
                      // channel: the correction E_l at l (N + 1) + p (p = 1..N-1
                      // spectral, p = 0 and N the wall values)
   int kb_stride;
@@ -135,6 +136,10 @@ struct EngineInfo {
 #define PQ_INV_SIZES(X)                                                        \
   X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384)   \
   X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
+
+// Workgroups per member of the psiC reduction (k_psic): one 256-thread
+// workgroup per 256 of the N/2 odd columns.
+__host__ __device__ constexpr int k_psic_groups(int N) { return (N / 2 + 255) / 256; }
 
 // Entry points of a kernel unit.  prec: PQ_PREC_*; pass: 0 row forward,
 // 1 column, 2 row inverse.

@@ -736,14 +736,38 @@ __global__ void __launch_bounds__(512) k_cols_chan(InvArgs a, PtrTab ptrs) {
 // (r, p).  All threads of the workgroup call it (tid, nt).  The terms in
 // r^j are dropped below e^-cut of the constant part (cut = 40 where fields
 // and transforms are FP64, 20 where either is FP32: e^-20 = 2e-9, far under
-// FP32 rounding).
+// FP32 rounding).  Where that holds for every mode (jmin rmin(h) > cut, the
+// slowest decay rate rmin), the correction is the constant gam_l(h) that
+// the psiC reduction formed (the sum of the modes' constant parts): no mode
+// loop.  The first two columns' rmin and gam_l come preloaded (pre, loaded
+// by the caller with the tile, so that their latency overlaps the tile's)
+// unless k0 = 2.
+struct MassPre {
+  double r0, r1, g0, g1;
+};
+__device__ __forceinline__ MassPre mass_preload(const double *rmin, const double *gam_l, int NN, int tid,
+                                                int nt) {
+  const int h0 = min(tid, NN / 2 - 1), h1 = min(tid + nt, NN / 2 - 1);  // clamped: surplus unused
+  return MassPre{rmin[h0], rmin[h1], gam_l[h0], gam_l[h1]};
+}
 template <int NN, typename Sub>
 __device__ __forceinline__ void mass_correction(const double *tab, const double (&beta)[8], int L,
                                                 int jb, int NB, int R, int lgR, int tid, int nt,
-                                                double cut, Sub sub) {
+                                                double cut, const double *rmin, const double *gam_l,
+                                                MassPre pre, int k0, Sub sub) {
   const int j0 = jb << lgR;
-  for (int h = tid; h < NN / 2; h += nt) {
+  const double jmin0 = double(min(j0, NN - j0 - R + 1));
+  int k = k0;
+  for (int h = tid; h < NN / 2; h += nt, ++k) {
     const int p = 2 * h + 1;
+    const double rm = k == 0 ? pre.r0 : (k == 1 ? pre.r1 : rmin[h]);
+    if (jmin0 * rm > cut) {
+      const double gl = k == 0 ? pre.g0 : (k == 1 ? pre.g1 : gam_l[h]);
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r < R) sub(r, p, gl);
+      continue;
+    }
     double acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0;
@@ -762,8 +786,7 @@ __device__ __forceinline__ void mass_correction(const double *tab, const double 
       const double rho = c01.x, gam = beta[m] * c01.y, eta = c23.x, rr = c23.y;
       // r^j for the tile's rows is below e^-cut far from both edges: then
       // only the constant part remains
-      const int jmin = min(j0, NN - j0 - R + 1);
-      if (double(jmin) * -rho > cut) {
+      if (jmin0 * -rho > cut) {
 #pragma unroll
         for (int r = 0; r < 16; ++r)
           if (r < R) acc[r] += gam;
@@ -862,6 +885,13 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
     for (int m = 1; m < 8; ++m)
       if (m < L) beta[m] = kb[L + l * L + m];
   }
+  // the mass correction's per-column constants (rmin, then Gamma_l);
+  // preloaded except with the FP64 engine (registers)
+  constexpr int kPre = sizeof(T) == 8 ? 2 : 0;
+  const double *kbm = a.kb + (size_t)b * a.kb_stride + L + L * L;
+  MassPre mpre{};
+  if (kPre == 0 && a.mode == 0 && !a.per && L > 1)
+    mpre = mass_preload(kbm, kbm + (1 + l) * (N / 2), N, threadIdx.x, nt);
   typename Eng::Inv::Ctx ctx;
   ctx.load(t, static_cast<const cx<double> *>(a.tw64), static_cast<const Tc *>(a.was),
            a.twm / Eng::Inv::M);
@@ -892,7 +922,8 @@ k_rows_inv(InvArgs a, PtrTab ptrs) {
   // Correction field for odd p, by recurrence over the tile's rows (FP64).
   if (a.mode == 0 && !a.per && L > 1) {
     mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, lgR, threadIdx.x, nt,
-                       sizeof(T) == 4 || sizeof(Tout) == 4 ? 20.0 : 40.0, [&](int r, int p, double v) {
+                       sizeof(T) == 4 || sizeof(Tout) == 4 ? 20.0 : 40.0, kbm, kbm + (1 + l) * (N / 2), mpre,
+                       kPre, [&](int r, int p, double v) {
                          T *x = row_ptr<Eng>(lds, r) + pad16(p);
                          *x = T(double(*x) - v);
                        });
@@ -1081,6 +1112,10 @@ k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
       for (int m = 1; m < 8; ++m)
         if (m < L) beta[m] = kb[L + l * L + m];
     }
+    // the mass correction's per-column constants (rmin, then Gamma_l)
+    const double *kbm = a.kb + (size_t)b * a.kb_stride + L + L * L;
+    MassPre mpre{};
+    if (m0 && L > 1) mpre = mass_preload(kbm, kbm + (1 + l) * (N / 2), N, tid, NT);
     __syncthreads();  // the previous block's reads of st
 #pragma unroll
     for (int k = 0; k < N * R / 4 / NT; ++k) {
@@ -1093,8 +1128,8 @@ k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
     }
     __syncthreads();  // publishes the tile (and the A3 copy)
     if (m0 && L > 1) {
-      mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, ilog2c(R), tid, NT, 20.0,
-                         [&](int r, int p, double x) {
+      mass_correction<N>(ptrs.coef[b] + CL.tab(), beta, L, jb, NB, R, ilog2c(R), tid, NT, 20.0, kbm,
+                         kbm + (1 + l) * (N / 2), mpre, 0, [&](int r, int p, double x) {
                            float *e = st + r * RS + p;
                            *e = float(double(*e) - x);
                          });

@@ -59,13 +59,38 @@ static int ilog2i(int x) {
 // kappa_l = psiC_1 + sum_{m>=2} M(l,m) psiC_m a_m and
 // beta_lm = M(l,m) psiC_m b_m (qg3p_diagnostics.F90:615-635 rewritten for
 // the closed-form correction).  Status bits: 1 NaN psiC, 2 |psiC| > 1e100
-// (qg3p_aux.F90:504-517).  Grid: one 256-thread workgroup per member.
+// (qg3p_aux.F90:504-517).
+//
+// Far from both walls the correction of layer l at odd p = 2h + 1 is the
+// constant Gamma_l(h) = sum_{m>=2} beta_lm gamma_m(h) (the r^j terms have
+// decayed); with rmin(h), the slowest decay rate -rho over those modes,
+// the inverse pass uses it for blocks whose rows all lie where they have.
+// Layout of kb after beta: rmin[N/2], then Gamma_l[N/2] for l = 0..L-1.
+//
+// Grid: k_psic_groups(N) workgroups of 256 threads per member, one column h
+// per thread for Gamma and rmin; every workgroup forms psiC from the
+// partials (a few KB), the first one publishes psiC, kappa, beta and the
+// status.
 __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
   __shared__ double red[4 * 8];
   __shared__ double s_psic[8];
-  const int b = blockIdx.x, L = ptrs.lay[b], tid = threadIdx.x;
+  __shared__ double s_beta[8 * 8];
+  const int ng = k_psic_groups(N);
+  const int b = blockIdx.x / ng, grp = blockIdx.x % ng, L = ptrs.lay[b], tid = threadIdx.x;
+  const int h = grp * 256 + tid;
+  const bool lead = grp == 0;
   const int nblk = N >> ptrs.lgc[b];
   const double *part = a.part + (size_t)b * a.part_stride;
+  // the mass correction's mode constants (rho, gamma) of this thread's
+  // column, issued first: their latency overlaps the reduction (the
+  // indices clamped to the table; the surplus unused)
+  const double *tab = ptrs.coef[b] + CoefLayout{L, N}.tab();
+  double2 e[7];
+  if (L >= 2) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      e[q] = *reinterpret_cast<const double2 *>(tab + ((size_t)min(q, L - 2) * (N / 2) + min(h, N / 2 - 1)) * 4);
+  }
   double acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = 0.0;
@@ -89,11 +114,11 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
     for (int m = 0; m < L; ++m) {
       const double v = (red[m] + red[8 + m] + red[16 + m] + red[24 + m]) * inv_n2;
       s_psic[m] = v;
-      a.psic[(size_t)b * a.psic_stride + m] = v;
+      if (lead) a.psic[(size_t)b * a.psic_stride + m] = v;
       if (v != v) st |= 1;
       else if (fabs(v) > 1.0e100) st |= 2;
     }
-    if (st) atomicOr(a.status + b, st);
+    if (st && lead) atomicOr(a.status + b, st);
   }
   __syncthreads();
   const CoefLayout CL{L, N};
@@ -105,10 +130,33 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
     double kap = s_psic[0];
     for (int m = 1; m < L; ++m) {
       kap += M[l + L * m] * s_psic[m] * ca[m];
-      kb[L + l * L + m] = M[l + L * m] * s_psic[m] * cb[m];
+      const double be = M[l + L * m] * s_psic[m] * cb[m];
+      if (lead) kb[L + l * L + m] = be;
+      s_beta[l * 8 + m] = be;
     }
-    kb[l] = kap;
+    if (lead) kb[l] = kap;
   }
+  if (L < 2) return;  // uniform in the workgroup
+  __syncthreads();    // s_beta
+  if (h >= N / 2) return;
+  double *rmin = kb + L + L * L, *gam = rmin + N / 2;
+  double rm = INFINITY;
+  double g[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) g[l] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int m = q + 1;
+    if (m >= L) break;
+    rm = fmin(rm, -e[q].x);
+#pragma unroll
+    for (int l = 0; l < 8; ++l)
+      if (l < L) g[l] += s_beta[l * 8 + m] * e[q].y;
+  }
+  rmin[h] = rm;
+#pragma unroll
+  for (int l = 0; l < 8; ++l)
+    if (l < L) gam[(size_t)l * (N / 2) + h] = g[l];
 }
 
 // The channel's momentum-conserving correction (qg3p_aux.F90:524-575) per
@@ -541,7 +589,7 @@ extern "C" int pq_prec_fp32_auto(int g, int L, const double *m2z, const double *
 static size_t part_per(const pq_inv *h) { return (size_t)h->part_cap * h->L * (h->per ? 2 : 1); }
 static size_t psic_per(const pq_inv *h) { return (size_t)h->L * (h->per ? 2 : 1); }
 static size_t kb_per(const pq_inv *h) {
-  return h->per ? (size_t)h->L * (h->N + 1) : (size_t)(h->L + h->L * h->L);
+  return h->per ? (size_t)h->L * (h->N + 1) : (size_t)(h->L + h->L * h->L + (h->L + 1) * (h->N / 2));
 }
 
 static int inv_create(pq_inv **out, int g, int L, const double *ss, const double *m2z,
@@ -687,7 +735,7 @@ static int launch_chunk(pq_inv *h, int nb, int mem0, const PtrTab &pt, hipStream
     }
     if (pass == 2 && h->mode == 0) {
       if (h->per) hipLaunchKernelGGL(k_chan_corr, dim3(nb), dim3(256), 0, s, a, pt, N);
-      else hipLaunchKernelGGL(k_psic, dim3(nb), dim3(256), 0, s, a, pt, N);
+      else hipLaunchKernelGGL(k_psic, dim3(nb * k_psic_groups(N)), dim3(256), 0, s, a, pt, N);
       hipError_t e2 = hipGetLastError();
       if (e2 != hipSuccess) {
         pq_set_error("pq_inv: launch of the psiC reduction failed: %s", hipGetErrorString(e2));
