@@ -371,6 +371,70 @@ __device__ __forceinline__ void powu2(double x, int k1, int k2, double &r1, doub
     x *= x;
   }
 }
+// Lane moves of a double inside rows of 16 lanes (DPP; CTRL 0x110 + d:
+// row_shr:d, lane q reads lane q - d; 0x100 + d: row_shl:d, lane q reads
+// q + d).  Lanes whose source lies outside their row read 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int LANE>
+__device__ __forceinline__ double readlane_d(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, LANE);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), LANE);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Affine scans of one column over a full wavefront (64 lanes, one column,
+// so r is uniform): forward v(l) = rE v(l - 1) + f(l) and backward
+// v(l) = rE v(l + 1) + b(l) over lanes l, with rE = r^E.  In-row inclusive
+// scans by DPP steps of 1, 2, 4 and 8 lanes; the rows' totals read back as
+// uniform values and combined in order (r16 = rE^16 per row).  Returns the
+// exclusive carries cf (forward value of lane l - 1; 0 at lane 0) and cb
+// (backward value of lane l + 1; 0 at lane 63), the wavefront's totals ft
+// (forward at lane 63) and bt (backward at lane 0), and the powers
+// pl = rE^l and pr = rE^(63 - l).
+__device__ __forceinline__ void wave_scan64(double f, double bb, double rE, int lw, double &cf, double &cb,
+                                            double &ft, double &bt, double &pl, double &pr) {
+  const int q = lw & 15, k = lw >> 4;
+  double vf = f, vb = bb, md = rE;
+  vf = fma(md, dpp_row<0x111>(vf), vf);
+  vb = fma(md, dpp_row<0x101>(vb), vb);
+  md *= md;
+  vf = fma(md, dpp_row<0x112>(vf), vf);
+  vb = fma(md, dpp_row<0x102>(vb), vb);
+  md *= md;
+  vf = fma(md, dpp_row<0x114>(vf), vf);
+  vb = fma(md, dpp_row<0x104>(vb), vb);
+  md *= md;
+  vf = fma(md, dpp_row<0x118>(vf), vf);
+  vb = fma(md, dpp_row<0x108>(vb), vb);
+  md *= md;  // r16 = rE^16
+  // prefixes through the rows before row k (forward) and suffixes from the
+  // rows after it (backward)
+  const double T0 = readlane_d<15>(vf), T1 = readlane_d<31>(vf), T2 = readlane_d<47>(vf), T3 = readlane_d<63>(vf);
+  const double U0 = readlane_d<0>(vb), U1 = readlane_d<16>(vb), U2 = readlane_d<32>(vb), U3 = readlane_d<48>(vb);
+  const double P2 = fma(md, T0, T1), P3 = fma(md, P2, T2);
+  const double S1 = fma(md, U3, U2), S0 = fma(md, S1, U1);
+  ft = fma(md, P3, T3);
+  bt = fma(md, S0, U0);
+  const double Pk = k == 0 ? 0.0 : (k == 1 ? T0 : (k == 2 ? P2 : P3));
+  const double Sk = k == 3 ? 0.0 : (k == 2 ? U3 : (k == 1 ? S1 : S0));
+  double a1, a2, b1, b2;
+  powu2<16>(rE, q, 15 - q, a1, a2);  // rE^q, rE^(15 - q)
+  powu2<4>(md, k, 3 - k, b1, b2);    // r16^k, r16^(3 - k)
+  // lane l - 1's forward value: its in-row value plus rE^q times the rows
+  // before (q = 0: the shifted value is 0 and rE^0 = 1); likewise backward
+  cf = fma(a1, Pk, dpp_row<0x111>(vf));
+  cb = fma(a2, Sk, dpp_row<0x101>(vb));
+  pl = a1 * b1;
+  pr = a2 * b2;
+}
+
 template <int W>
 __device__ __forceinline__ double seg_shfl_up(double v, int d) {
   return __shfl_up(v, d, W);
@@ -452,6 +516,7 @@ __device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &pt
     const double *tc = s_tc + (m * C + col) * 6;
     const double r = tc[0], ri = tc[1], cg = tc[2], rN = tc[3], w = tc[4], rE = tc[5];
     double cf, cb, ft = 0.0, bt = 0.0;
+    double pl, pr;  // r^(E lw), r^(E (W - 1 - lw))
     {
       // Scans: the thread's recurrence totals (zero
       // carry-in), inclusive affine scans inside the column's lanes of this
@@ -465,24 +530,39 @@ __device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &pt
       for (int e = 0; e < E; ++e) f = fma(r, f, z[m][e]);
 #pragma unroll
       for (int e = E - 1; e >= 0; --e) bb = fma(r, bb, z[m][e]);
-      double vf = f, vb = bb, mul = rE;
-#pragma unroll
-      for (int d = 1; d < W; d <<= 1) {
-        const double uf = seg_shfl_up<W>(vf, d), ub = seg_shfl_down<W>(vb, d);
-        if (lw >= d) vf = fma(mul, uf, vf);
-        if (lw + d < W) vb = fma(mul, ub, vb);
-        mul *= mul;
-      }
-      const double pf = seg_shfl_up<W>(vf, 1), pb = seg_shfl_down<W>(vb, 1);
-      cf = lw == 0 ? 0.0 : pf;
-      cb = lw == W - 1 ? 0.0 : pb;
-      if constexpr (NWC > 1) {
-        if (lw == W - 1) tot(0, m, wv) = vf;
-        if (lw == 0) tot(1, m, wv) = vb;
-        __syncthreads();  // the mode's wavefront totals (a slot per mode: no reuse)
+      if constexpr (W == 64) {
+        // a wavefront holds one column: DPP row steps and uniform row totals
+        double wf, wb;
+        wave_scan64(f, bb, rE, lw, cf, cb, wf, wb, pl, pr);
+        if constexpr (NWC > 1) {
+          if (lw == W - 1) tot(0, m, wv) = wf;
+          if (lw == 0) tot(1, m, wv) = wb;
+          __syncthreads();  // the mode's wavefront totals (a slot per mode: no reuse)
+        } else {
+          ft = wf;
+          bt = wb;
+        }
       } else {
-        ft = __shfl(vf, W - 1, W);
-        bt = __shfl(vb, 0, W);
+        double vf = f, vb = bb, mul = rE;
+#pragma unroll
+        for (int d = 1; d < W; d <<= 1) {
+          const double uf = seg_shfl_up<W>(vf, d), ub = seg_shfl_down<W>(vb, d);
+          if (lw >= d) vf = fma(mul, uf, vf);
+          if (lw + d < W) vb = fma(mul, ub, vb);
+          mul *= mul;
+        }
+        const double pf = seg_shfl_up<W>(vf, 1), pb = seg_shfl_down<W>(vb, 1);
+        cf = lw == 0 ? 0.0 : pf;
+        cb = lw == W - 1 ? 0.0 : pb;
+        if constexpr (NWC > 1) {
+          if (lw == W - 1) tot(0, m, wv) = vf;
+          if (lw == 0) tot(1, m, wv) = vb;
+          __syncthreads();  // the mode's wavefront totals (a slot per mode: no reuse)
+        } else {
+          ft = __shfl(vf, W - 1, W);
+          bt = __shfl(vb, 0, W);
+        }
+        powu2<W>(rE, lw, W - 1 - lw, pl, pr);
       }
     }
     {
@@ -496,8 +576,6 @@ __device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &pt
       // (r^E)^(TPC - lc) from the carries' powers r^(E lw),
       // r^(E (W - 1 - lw)) and those of r^(E W).
       double cfm = cf, cbm = cb, fN = ft, b0 = bt;
-      double pl, pr;
-      powu2<W>(rE, lw, W - 1 - lw, pl, pr);
       double ru = pl, rd = pr * rE;
       if constexpr (NWC > 1) {
         double rW = rE;  // r^(E W)
