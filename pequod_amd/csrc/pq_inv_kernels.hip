@@ -525,11 +525,25 @@ __device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &pt
       // rows, B just after.  A column inside one wavefront also takes its
       // totals F(N - 1) (last lane) and B(0) (lane 0) here; wider columns
       // publish each wavefront's totals.
+      // The local recurrences Fl(e) (forward from 0 at the thread's first
+      // row) and Bl(e) (backward from 0 after its last row) also give the
+      // local part of the particular solution, (Fl + Bl - z) / (r - 1/r) =
+      // (r Fl(e - 1) + Bl(e)) / (r - 1/r), kept in z[m] in place of the
+      // right-hand side: Fl overwrites z first, and the backward sweep takes
+      // z(e) back as Fl(e) - r Fl(e - 1) (an error of an ulp of Fl, of the
+      // size of Bl's own rounding).
       double f = 0.0, bb = 0.0;
 #pragma unroll
-      for (int e = 0; e < E; ++e) f = fma(r, f, z[m][e]);
+      for (int e = 0; e < E; ++e) {
+        f = fma(r, f, z[m][e]);
+        z[m][e] = f;
+      }
 #pragma unroll
-      for (int e = E - 1; e >= 0; --e) bb = fma(r, bb, z[m][e]);
+      for (int e = E - 1; e >= 0; --e) {
+        const double g = e > 0 ? r * z[m][e - 1] : 0.0;  // r Fl(e - 1)
+        bb = fma(r, bb, z[m][e] - g);
+        z[m][e] = (g + bb) * cg;
+      }
       if constexpr (W == 64) {
         // a wavefront holds one column: DPP row steps and uniform row totals
         double wf, wb;
@@ -567,16 +581,20 @@ __device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &pt
     }
     {
       // Then the carries across the column's wavefronts
-      // and the column's totals (wavefront totals combined in order), the
-      // full recurrences, the particular solution yp(j) = (F(j) + B(j) -
-      // z(j)) / (r - 1/r), and the walls: y = yp + alpha r^j + beta r^(N - j)
-      // with yp(0) + alpha + beta r^N = 0 and yp(N) + alpha r^N + beta = 0,
-      // where yp(0) = B(0) / (r - 1/r) (F(0) = z(0)) and yp(N) =
-      // r F(N - 1) / (r - 1/r) (z(N) = 0); r^j0 = (r^E)^lc and r^(N - j0) =
-      // (r^E)^(TPC - lc) from the carries' powers r^(E lw),
+      // and the column's totals (wavefront totals combined in order), and
+      // the solution y = yp + alpha r^j + beta r^(N - j), where the
+      // particular solution yp(j) = (F(j) + B(j) - z(j)) / (r - 1/r) has
+      // F(j0 + e) = Fl(e) + r^(e+1) F(j0 - 1) and B(j0 + e) = Bl(e) +
+      // r^(E-e) B(j0 + E), and the walls satisfy yp(0) + alpha + beta r^N = 0
+      // and yp(N) + alpha r^N + beta = 0, with yp(0) = B(0) / (r - 1/r)
+      // (F(0) = z(0)) and yp(N) = r F(N - 1) / (r - 1/r) (z(N) = 0).  So
+      // y(j0 + e) = local(e) + A r^e + B r^(E - e) with A = (r F(j0 - 1) /
+      // (r - 1/r) + alpha r^j0) and B = (B(j0 + E) / (r - 1/r) +
+      // beta r^(N - j0 - E)); r^j0 = (r^E)^lc and r^(N - j0 - E) =
+      // (r^E)^(TPC - 1 - lc) from the carries' powers r^(E lw),
       // r^(E (W - 1 - lw)) and those of r^(E W).
       double cfm = cf, cbm = cb, fN = ft, b0 = bt;
-      double ru = pl, rd = pr * rE;
+      double ru = pl, rd = pr;
       if constexpr (NWC > 1) {
         double rW = rE;  // r^(E W)
 #pragma unroll
@@ -606,28 +624,16 @@ __device__ __forceinline__ void cols_tri_body(const InvArgs &a, const PtrTab &pt
       }
       const double y0 = b0 * cg, yN = r * fN * cg;
       const double al = (rN * yN - y0) * w, be = (rN * y0 - yN) * w;
-      double t[E];
-      double f = 0.0, rp = r;
+      const double A = fma(cg * r, cfm, al * ru), B = fma(cg, cbm, be * rd);
+      double sum = 0.0, pu = 1.0, pd = rE;  // r^e, r^(E - e)
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        f = fma(r, f, z[m][e]);
-        t[e] = fma(rp, cfm, f);  // F(j0 + e)
-        rp *= r;
-      }
-      double bj = cbm;
-#pragma unroll
-      for (int e = E - 1; e >= 0; --e) {
-        bj = fma(r, bj, z[m][e]);  // B(j0 + e)
-        t[e] = (t[e] + bj - z[m][e]) * cg;
-      }
-      double sum = 0.0;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const double y = fma(al, ru, fma(be, rd, t[e]));
-        z[m][e] = (j0 + e == 0) ? 0.0 : y;  // Dirichlet row j = 0
-        sum += z[m][e];
-        ru *= r;
-        rd *= ri;
+        double y = fma(A, pu, fma(B, pd, z[m][e]));
+        if (e == 0 && j0 == 0) y = 0.0;  // Dirichlet row j = 0
+        z[m][e] = y;
+        sum += y;
+        pu *= r;
+        pd *= ri;
       }
       acc[m] = sp * sum;
     }
