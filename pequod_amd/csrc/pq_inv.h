@@ -42,8 +42,9 @@ struct CoefLayout {
 __host__ __device__ constexpr int tri_e(int N, int LT) {
   return N % 3 == 0 ? (LT <= 4 ? 12 : 6) : 8;
 }
-// Layer count the kernel is instantiated for: exact up to 4, else 8.
-__host__ __device__ constexpr int tri_lt(int L) { return L <= 4 ? L : 8; }
+// Layer count the kernel is instantiated for: exact up to 4, then 6 (for
+// 5 and 6 layers: 167 VGPRs at N = 2048, 3 waves per SIMD) and 8.
+__host__ __device__ constexpr int tri_lt(int L) { return L <= 4 ? L : (L <= 6 ? 6 : 8); }
 // Threads per workgroup and columns per workgroup of the tridiagonal pass.
 __host__ __device__ constexpr int tri_tpc(int N, int L) { return N / tri_e(N, tri_lt(L)); }
 // (a power of two dividing N: the psiC partials are kept per block of columns)

@@ -1331,6 +1331,7 @@ static hipError_t launch_pass(int pass, const InvArgs &a, const PtrTab &pt, int 
       case 2: hipLaunchKernelGGL((k_cols_tri<Ts, 2, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
       case 3: hipLaunchKernelGGL((k_cols_tri<Ts, 3, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
       case 4: hipLaunchKernelGGL((k_cols_tri<Ts, 4, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
+      case 5: case 6: hipLaunchKernelGGL((k_cols_tri<Ts, 6, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
       default: hipLaunchKernelGGL((k_cols_tri<Ts, 8, PQ_N>), dim3(blocks), dim3(threads), 0, s, a, pt); break;
     }
   } else {
