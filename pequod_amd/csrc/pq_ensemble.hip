@@ -2,9 +2,10 @@
 // layer counts and stratifications on one GPU.
 //
 // Members are grouped into size classes: same grid, array leading
-// dimensions and transform precision, and either up to 4 or more than 4
-// layers (the column pass's geometry and registers follow a class's largest
-// layer count, so few-layer members do not run the many-layer kernel).
+// dimensions and transform precision, and a layer bucket: up to 4, 5-6 or
+// 7-8 layers (the column pass's geometry and registers follow a class's
+// largest layer count, so few-layer members do not run the many-layer
+// kernel).
 // Each class owns one PV-inversion handle for its largest layer count whose
 // member slots carry the members' own layer counts and stratifications
 // (pq_inv_set_member_layers), so one inversion of the whole ensemble is one
@@ -178,14 +179,14 @@ int pq_ens_add_model(pq_ens *e, pq_model *model, const double *ss, const double 
 int pq_ens_finalize(pq_ens *e) {
   PQ_CHECK_ARG(e && !e->finalized, "pq_ens_finalize: null or finalized ensemble");
   PQ_CHECK_ARG(!e->mem.empty(), "pq_ens_finalize: no members");
-  std::map<std::tuple<int, long, long, long, long, int, bool>, int> key2cls;
+  std::map<std::tuple<int, long, long, long, long, int, int>, int> key2cls;
   for (size_t i = 0; i < e->mem.size(); ++i) {
     auto &m = e->mem[i];
     // FP32_AUTO resolves per member (pq_prec_fp32_auto)
     const int prec = e->prec == PQ_PREC_FP32_AUTO
                          ? pq_prec_fp32_auto(m.g, m.L, m.m2z.data(), m.z2m.data())
                          : e->prec;
-    auto key = std::make_tuple(m.g, m.ldz1, m.ldz2, m.ldp1, m.ldp2, prec, m.L > 4);
+    auto key = std::make_tuple(m.g, m.ldz1, m.ldz2, m.ldp1, m.ldp2, prec, m.L > 4 ? (m.L > 6 ? 2 : 1) : 0);
     auto it = key2cls.find(key);
     int k;
     if (it == key2cls.end()) {

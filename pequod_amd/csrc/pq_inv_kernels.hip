@@ -1231,39 +1231,54 @@ k_rows_inv_mfma(InvArgs a, PtrTab ptrs) {
     Tout *ps = static_cast<Tout *>(ptrs.psi[b]) + (size_t)l * ptrs.ldp1 * ptrs.ldp2;
     const long long ld1 = ptrs.ldp1;
     const double sub = m0 ? kappa : 0.0;
-    for (int r = w; r < R; r += NT / 64) {
-      const int j = j0 + r;
-      Tout *pa = ps + (size_t)j * ld1;
-      float *row = st + r * RS;
-      if (j == 0) {
-        for (int i = lane; i < G; i += 64)
-          if (m0 || (i != 0 && i != N)) pa[i] = Tout(0.0 - sub);
-        continue;
-      }
+    // A wavefront takes its rows in pairs (r, r + RW): both transforms, then
+    // one pass over i that forms the low modes' sines once for the two rows.
+    constexpr int RW = NT / 64;
+    static_assert(R % (2 * RW) == 0, "row pairs of a wavefront");
+    static_assert(kMfmaLow == 2, "the low-mode sum is written for 2 modes");
+    for (int r = w; r < R; r += 2 * RW) {
       // low = s1 (w1 + 2 w2 c1), s1 = sin(pi i / N), c1 = cos(pi i / N)
-      // (sin 2t = 2 sin t cos t), for the weights w_p = 2 x_p
-      static_assert(kMfmaLow == 2, "the low-mode sum is written for 2 modes");
-      const double p0 = 2.0 * double(row[1]), p1 = 4.0 * double(row[2]);
-      float x[2][8];
-      const float *src = row + c + 256 * g;
+      // (sin 2t = 2 sin t cos t), for the weights w_p = 2 x_p; row j = 0 is
+      // zero (no transform)
+      double p0[2], p1[2];
+      bool zr[2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+      for (int h = 0; h < 2; ++h) {
+        float *row = st + (r + h * RW) * RS;
+        zr[h] = j0 + r + h * RW == 0;
+        p0[h] = 0.0;
+        p1[h] = 0.0;
+        if (zr[h]) continue;  // uniform in the wavefront
+        p0[h] = 2.0 * double(row[1]);
+        p1[h] = 4.0 * double(row[2]);
+        float x[2][8];
+        const float *src = row + c + 256 * g;
 #pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) x[t][s2] = src[512 * t + 32 * s2];
-      if (lane <= kMfmaLow) x[0][0] = 0.0f;  // x_0 and the low modes (lanes 1..kMfmaLow)
-      mfma::dst_row(K, a3s, x, lane, row);
-      // the row's psi at i = lane + 64 k (0..N - 1), then i = N; the
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int s2 = 0; s2 < 8; ++s2) x[t][s2] = src[512 * t + 32 * s2];
+        if (lane <= kMfmaLow) x[0][0] = 0.0f;  // x_0 and the low modes (lanes 1..kMfmaLow)
+        mfma::dst_row(K, a3s, x, lane, row);
+      }
+      // the rows' psi at i = lane + 64 k (0..N - 1), then i = N; the
       // angle pi i / N = pi lane / N + pi k / 16
 #pragma unroll 4
       for (int k = 0; k < N / 64; ++k) {
         const int i = lane + 64 * k;
         const double ca = cospi16(k), sa = sinpi16(k);
         const double s1 = fma(sb, ca, cb * sa), c1 = fma(cb, ca, -sb * sa);
-        const double pc = fma(p1, c1, p0);
-        const double y = i == 0 ? 0.0 : fma(s1, pc, double(row[i]));
-        if (m0 || i != 0) pa[i] = Tout(y - sub);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float *row = st + (r + h * RW) * RS;
+          const double y = (zr[h] || i == 0) ? 0.0 : fma(s1, fma(p1[h], c1, p0[h]), double(row[i]));
+          if (m0 || i != 0) ps[(size_t)(j0 + r + h * RW) * ld1 + i] = Tout(y - sub);
+        }
       }
-      if (lane == 0) pa[N] = Tout(0.0 - sub);
+      if (lane == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (m0 || !zr[h]) ps[(size_t)(j0 + r + h * RW) * ld1 + N] = Tout(0.0 - sub);
+      }
     }
     if (m0 && j0 + R == N) {
       Tout *pn = ps + (size_t)N * ld1;
