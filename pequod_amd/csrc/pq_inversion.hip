@@ -91,6 +91,20 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
     for (int q = 0; q < 7; ++q)
       e[q] = *reinterpret_cast<const double2 *>(tab + ((size_t)min(q, L - 2) * (N / 2) + min(h, N / 2 - 1)) * 4);
   }
+  // the modal coefficients the first L threads combine with psiC (M(l, m),
+  // a_m, b_m), also issued before the reduction
+  const CoefLayout CL{L, N};
+  const double *coef = ptrs.coef[b];
+  double cm[8], ca[8], cbv[8];
+  if (tid < L) {
+#pragma unroll
+    for (int m = 1; m < 8; ++m)
+      if (m < L) {
+        cm[m] = coef[CL.M() + tid + L * m];
+        ca[m] = coef[CL.a() + m];
+        cbv[m] = coef[CL.b() + m];
+      }
+  }
   double acc[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) acc[m] = 0.0;
@@ -121,16 +135,15 @@ __global__ void __launch_bounds__(256) k_psic(InvArgs a, PtrTab ptrs, int N) {
     if (st && lead) atomicOr(a.status + b, st);
   }
   __syncthreads();
-  const CoefLayout CL{L, N};
-  const double *coef = ptrs.coef[b];
-  const double *M = coef + CL.M(), *ca = coef + CL.a(), *cb = coef + CL.b();
   double *kb = a.kb + (size_t)b * a.kb_stride;
   if (tid < L) {
     const int l = tid;
     double kap = s_psic[0];
-    for (int m = 1; m < L; ++m) {
-      kap += M[l + L * m] * s_psic[m] * ca[m];
-      const double be = M[l + L * m] * s_psic[m] * cb[m];
+#pragma unroll
+    for (int m = 1; m < 8; ++m) {
+      if (m >= L) break;
+      kap += cm[m] * s_psic[m] * ca[m];
+      const double be = cm[m] * s_psic[m] * cbv[m];
       if (lead) kb[L + l * L + m] = be;
       s_beta[l * 8 + m] = be;
     }
